@@ -1,0 +1,152 @@
+// f32 arithmetic of the particle kernels (gfx950 device code).
+//
+// The WGSL builtins exp/log/sin/cos that shaders/gps_splat.wgsl relies on are
+// driver approximations with no fixed bits.  Here they are one fixed
+// algorithm built only from IEEE-754 binary32 +, -, *, / and sqrt, which the
+// device evaluates with correct rounding (compiled with -ffp-contract=off and
+// -fhip-fp32-correctly-rounded-divide-sqrt).  A CPU evaluation of the same
+// steps therefore reproduces every result bit for bit.
+//
+//   exp: Cody-Waite reduction x = k ln2 + r (ln2 in two parts, the high part
+//        with 9 trailing zero bits so k*LN2_HI is exact), degree-7 Taylor
+//        polynomial in Horner form, 2^k applied through the exponent field.
+//   log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
+//        s = (m-1)/(m+1), odd series through s^9, plus e ln2 in two parts.
+//   sin/cos: reduction by pi/2 with a three-part constant (exact products for
+//        |k| < 2^11), Taylor polynomials of degree 9 / 10 on |r| <= pi/4.
+//
+// Accuracy: <= 2 ulp against correctly rounded results over the argument
+// ranges the pipeline uses (sin/cos arguments are 2 pi u, u in [0, 1)).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GPS_DEV __device__ __forceinline__
+
+namespace gpsm {
+
+constexpr float LN2_HI = 0x1.62e4p-1f;
+constexpr float LN2_LO = 0x1.7f7d1cp-20f;
+constexpr float LOG2E = 0x1.715476p+0f;
+constexpr float PIO2_1 = 0x1.92p+0f;
+constexpr float PIO2_2 = 0x1.fb4p-12f;
+constexpr float PIO2_3 = 0x1.4442d2p-24f;
+constexpr float TWO_OVER_PI = 0x1.45f306p-1f;
+constexpr float SQRT2 = 0x1.6a09e6p+0f;
+
+GPS_DEV uint32_t bits(float x) { return __float_as_uint(x); }
+GPS_DEV float fbits(uint32_t u) { return __uint_as_float(u); }
+GPS_DEV float rint(float x) { return __builtin_rintf(x); }
+GPS_DEV float sqrt(float x) { return __builtin_sqrtf(x); }
+
+GPS_DEV float exp(float x) {
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return fbits(0x7f800000u);
+    if (x < -103.972084045410156f) return 0.0f;
+    float kf = rint(x * LOG2E);
+    int k = (int)kf;
+    float r = (x - kf * LN2_HI) - kf * LN2_LO;
+    float p = 0x1.a01a02p-13f;
+    p = 0x1.6c16c2p-10f + r * p;
+    p = 0x1.111112p-7f + r * p;
+    p = 0x1.555556p-5f + r * p;
+    p = 0x1.555556p-3f + r * p;
+    p = 0.5f + r * p;
+    p = 1.0f + r * p;
+    p = 1.0f + r * p;
+    if (k > 127) return p * fbits(0x7f000000u) * fbits((uint32_t)(k - 127 + 127) << 23);
+    if (k < -126) return p * fbits((uint32_t)(k + 64 + 127) << 23) * 0x1p-64f;
+    return p * fbits((uint32_t)(k + 127) << 23);
+}
+
+GPS_DEV float log(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return fbits(0x7fc00000u);
+    if (x == 0.0f) return fbits(0xff800000u);
+    if (x == fbits(0x7f800000u)) return x;
+    int e = 0;
+    uint32_t u = bits(x);
+    if (u < 0x00800000u) {
+        x = x * 0x1p23f;
+        u = bits(x);
+        e = -23;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = fbits((u & 0x007fffffu) | 0x3f800000u);
+    if (m > SQRT2) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float q = 0x1.c71c72p-4f;
+    q = 0x1.24924ap-3f + z * q;
+    q = 0x1.99999ap-3f + z * q;
+    q = 0x1.555556p-2f + z * q;
+    float s2 = s + s;
+    float lm = s2 + s2 * (z * q);
+    float ef = (float)e;
+    return ef * LN2_HI + (ef * LN2_LO + lm);
+}
+
+GPS_DEV float sin_poly(float r) {
+    float z = r * r;
+    float p = 0x1.71de3ap-19f;
+    p = -0x1.a01a02p-13f + z * p;
+    p = 0x1.111112p-7f + z * p;
+    p = -0x1.555556p-3f + z * p;
+    return r + r * (z * p);
+}
+
+GPS_DEV float cos_poly(float r) {
+    float z = r * r;
+    float p = -0x1.27e4fcp-22f;
+    p = 0x1.a01a02p-16f + z * p;
+    p = -0x1.6c16c2p-10f + z * p;
+    p = 0x1.555556p-5f + z * p;
+    p = -0.5f + z * p;
+    return 1.0f + z * p;
+}
+
+GPS_DEV float reduce(float x, int &q) {
+    float kf = rint(x * TWO_OVER_PI);
+    float r = ((x - kf * PIO2_1) - kf * PIO2_2) - kf * PIO2_3;
+    q = ((int)kf) & 3;
+    return r;
+}
+
+// sin and cos of the same argument (the pipeline always needs both)
+GPS_DEV void sincos(float x, float &s, float &c) {
+    if (x != x) {
+        s = x;
+        c = x;
+        return;
+    }
+    int q;
+    float r = reduce(x, q);
+    float ps = sin_poly(r), pc = cos_poly(r);
+    switch (q) {
+    case 0: s = ps; c = pc; break;
+    case 1: s = pc; c = -ps; break;
+    case 2: s = -ps; c = -pc; break;
+    default: s = -pc; c = ps; break;
+    }
+}
+
+GPS_DEV float sin(float x) { float s, c; sincos(x, s, c); return s; }
+GPS_DEV float cos(float x) { float s, c; sincos(x, s, c); return c; }
+
+// WGSL u32(f): truncate toward zero, saturate to [0, 2^32-1], NaN -> 0
+GPS_DEV uint32_t f2u(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)f;
+}
+
+GPS_DEV float fmin(float a, float b) { return a < b ? a : b; }
+GPS_DEV float fmax(float a, float b) { return a > b ? a : b; }
+GPS_DEV float clamp(float x, float lo, float hi) { return fmin(fmax(x, lo), hi); }
+GPS_DEV float mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+}  // namespace gpsm

@@ -1,0 +1,72 @@
+// Internal definitions shared by the kernel and host translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Per-view constants of one render call, passed to every kernel by value.
+// Field meaning follows the reference's FrameUniform (crates/gps-render/src/gps.rs:63-76,
+// packed by upload_frame, gps.rs:812-856); the ensemble seed is not part of it.
+struct GpsFrame {
+    float rot[3][4];        // rows of view_rot
+    float pos[4];           // camera position, w = low-pass variance (px^2)
+    float intr[4];          // fx, fy, cx, cy
+    uint32_t W, H, spp_side, N;
+    uint32_t cap, sh_stride, exact_radius, sh_degree;
+    uint32_t method, calib, flags, pad0;
+    float density, near_z, base_k, ref_len;
+    float object_depth, pad1, pad2, pad3;
+    float bg[4];
+};
+
+enum : uint32_t {
+    M_GPS = 0, M_PROPORTIONAL = 1, M_EXTINCTION = 2, M_VIEW_CONDITIONED = 3, M_OCCUPANCY = 4,
+    FLAG_RADIAL = 1, FLAG_CENTRE = 2, FLAG_JITTER = 4,
+};
+
+// per-splat count source written by the view kernel (kind in the low bits of .y)
+enum : uint32_t { KIND_NONE = 0, KIND_POISSON = 1, KIND_FIXED = 2, KIND_UNDERCOVERED = 4 };
+
+// device statistics slots (u64 each)
+enum : int {
+    ST_POINTS = 0, ST_TRUNC = 1, ST_DROPPED = 2, ST_VISIBLE = 3, ST_ORPHAN = 4, ST_CAND = 5, ST_UNDER = 6,
+    ST_SKIPPED = 7, ST_ENSEMBLES = 8, ST_PROBE = 9, ST_WORDS = 16,
+};
+
+constexpr int VIEW_VEC4 = 6;          // float4 per splat in the view cache
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr int PART_BLOCK = 256;
+constexpr int MAX_BATCH = 64;         // ensembles per batch (one depth|id buffer each)
+constexpr uint64_t EMPTY64 = ~0ull;
+
+struct BatchState {
+    uint32_t *tickets;      // [MAX_BATCH] scan tile tickets (reset by the last tile)
+    uint64_t *status;       // [MAX_BATCH * tiles] decoupled look-back words
+    uint32_t *totals;       // [MAX_BATCH] particle total of each ensemble (0 if skipped)
+    uint32_t *skip;         // [MAX_BATCH] 1 if the ensemble overflowed u32
+};
+
+// kernel launchers (gps_kernels.hip)
+void launch_view(hipStream_t s, const GpsFrame &f, const float4 *splats, const float *sh, float4 *view,
+                 uint2 *cinfo, float4 *colour);
+void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uint32_t first_seed, uint32_t nens,
+                       uint32_t *offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch, uint64_t *stats);
+void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
+                      const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
+                      const float *lut, uint64_t *stats, int grid);
+void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
+                    uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats);
+void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
+void launch_splat_pass(hipStream_t s, const GpsFrame &f, const float4 *view, const uint2 *cinfo, uint32_t seed,
+                       uint64_t *depth, const float *lut, uint64_t *stats);
+void launch_composite(hipStream_t s, const float4 *accum, uint32_t W, uint32_t H, float exposure, uint32_t gamma_on,
+                      uint32_t ow, uint32_t oh, uint32_t *out);
+void launch_zero_u32(hipStream_t s, uint32_t *p, uint64_t n);
+void launch_stats_add(hipStream_t s, uint64_t *stats, int slot, uint64_t v);
+void launch_stats_clear(hipStream_t s, uint64_t *stats, int first, int count);
+void launch_accum_clear(hipStream_t s, float4 *accum, uint64_t n);
+void launch_phase0(hipStream_t s, uint32_t seed, uint32_t pixels, uint32_t samples, uint64_t *buf);
+void launch_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ticket, uint64_t *status,
+                     uint32_t tiles, uint32_t epoch);
+void launch_math(hipStream_t s, int fn, const float *x, const float *y, float *out, uint64_t n);

@@ -1,0 +1,1031 @@
+// Particle-path kernels for gfx950.
+//
+// Reference pipeline per ensemble (crates/gps-render/src/gps.rs:884-921,
+// shaders/gps_splat.wgsl): clear -> prepare -> multi-level scan -> finalize ->
+// particle_depth -> particle_color -> resolve, one dispatch each.  Here:
+//
+//   k_view        once per render call: projection, per-splat particle law,
+//                 3D Cholesky and SH colour for the call's camera (the same
+//                 bits the reference recomputes for every ensemble)
+//   k_count_scan  per batch of ensembles: Poisson counts fused with a
+//                 single-pass exclusive scan (decoupled look-back, ticketed
+//                 tiles), particle totals and the u32-overflow skip
+//   k_particles   per batch: persistent grid-stride kernel over every
+//                 particle of every ensemble of the batch; one packed
+//                 64-bit atomicMin((depth key << 32) | splat id) per particle
+//                 replaces the reference's two regenerating passes
+//   k_resolve     per batch: ensemble accumulation in ensemble order, with
+//                 the depth|id buffers reset to empty in the same pass
+//
+// Every f32 expression keeps the operation order of the WGSL it restates;
+// see gps_devmath.h for the elementary functions.
+#include "gps_devmath.h"
+#include "gps_internal.h"
+
+namespace {
+
+using gpsm::clamp;
+using gpsm::f2u;
+using gpsm::fmax;
+using gpsm::fmin;
+using gpsm::mix;
+
+constexpr float PI_F = 3.14159265358979f;
+constexpr float DILOG_ONE = 1.64493406684823f;
+
+// ------------------------------------------------------------------ RNG
+// gps_splat.wgsl:57-76
+GPS_DEV uint32_t pcg(uint32_t v) {
+    uint32_t s = v * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+
+GPS_DEV uint32_t particle_seed(uint32_t splat, uint32_t k, uint32_t frame) {
+    uint32_t s = pcg(splat * 747796405u + k + 1u);
+    s = pcg(s ^ (frame * 2654435761u + 1u));
+    return s ? s : 1u;
+}
+
+struct Rng {
+    uint32_t s;
+    float spare;
+    bool has_spare;
+    GPS_DEV explicit Rng(uint32_t seed) : s(seed), spare(0.0f), has_spare(false) {}
+    GPS_DEV float next() {
+        s = pcg(s);
+        return (float)(s >> 8u) * (1.0f / 16777216.0f);
+    }
+    // gps_splat.wgsl:402-414 (Box-Muller, sine half kept)
+    GPS_DEV float normal() {
+        if (has_spare) {
+            has_spare = false;
+            return spare;
+        }
+        float u1 = fmax(next(), 5.96e-8f);
+        float u2 = next();
+        float r = gpsm::sqrt(-2.0f * gpsm::log(u1));
+        float th = 2.0f * PI_F * u2;
+        float sn, cs;
+        gpsm::sincos(th, sn, cs);
+        spare = r * sn;
+        has_spare = true;
+        return r * cs;
+    }
+};
+
+// ------------------------------------------------------------------ Li2 / radius law
+// gps_splat.wgsl:79-140
+GPS_DEV float dilog_series(float x) {
+    float term = x, sum = x;
+    for (uint32_t k = 2u; k < 32u; k++) {
+        term = term * x;
+        sum = sum + term / (float)(k * k);
+    }
+    return sum;
+}
+
+GPS_DEV float dilog(float x) {
+    if (x >= 1.0f) return DILOG_ONE;
+    if (x <= 0.0f) return 0.0f;
+    if (x <= 0.5f) return dilog_series(x);
+    return DILOG_ONE - gpsm::log(x) * gpsm::log(1.0f - x) - dilog_series(1.0f - x);
+}
+
+GPS_DEV float inv_dilog(float tgt, float hi_in) {
+    if (tgt <= 0.0f) return 0.0f;
+    float lo = 0.0f, hi = hi_in, y = hi * 0.5f;
+    for (uint32_t i = 0; i < 24u; i++) {
+        float f = dilog(y) - tgt;
+        if (f > 0.0f) hi = y; else lo = y;
+        float dfdy = 1.0f;
+        if (y > 1e-6f && y < 0.9999999f) dfdy = -gpsm::log(1.0f - y) / y;
+        float ny = y - f / dfdy;
+        if (!(ny > lo && ny < hi)) ny = 0.5f * (lo + hi);
+        float dd = ny - y;
+        if (dd < 0.0f) dd = -dd;
+        if (dd <= 1e-6f * (1.0f + y)) return ny;
+        y = ny;
+    }
+    return y;
+}
+
+GPS_DEV float radius_lut(const float *__restrict__ lut, float o, float u) {
+    if (o <= 0.0f) return 0.0f;
+    float v = -gpsm::log(fmax(1.0f - u, 5.96e-8f));
+    float fo = (1.0f - gpsm::sqrt(fmax(1.0f - fmin(o, 1.0f), 0.0f))) * 64.0f;
+    float fv = fmin(gpsm::sqrt(v) * (float)(128.0 / 4.123105625617661), 128.0f);
+    uint32_t i0 = f2u(fo);
+    if (i0 > 63u) i0 = 63u;
+    uint32_t j0 = f2u(fv);
+    if (j0 > 127u) j0 = 127u;
+    float a = fo - (float)i0, b = fv - (float)j0;
+    uint32_t k = i0 * 129u + j0;
+    float r0 = mix(lut[k], lut[k + 1u], b);
+    float r1 = mix(lut[k + 129u], lut[k + 130u], b);
+    return mix(r0, r1, a);
+}
+
+GPS_DEV float corrected_radius(float o, float u) {
+    if (o <= 0.0f) return 0.0f;
+    float d = dilog(o);
+    float tgt = (1.0f - u) * d;
+    float y = fmin(inv_dilog(tgt, o), o);
+    if (y <= 0.0f) return 40.0f;
+    return gpsm::sqrt(fmax(-2.0f * gpsm::log(y / o), 0.0f));
+}
+
+// gps_splat.wgsl:142-160
+GPS_DEV uint32_t poisson(Rng &r, float lambda) {
+    if (lambda <= 0.0f) return 0u;
+    if (lambda < 30.0f) {
+        float l = gpsm::exp(-lambda);
+        uint32_t k = 0u;
+        float p = 1.0f;
+        for (;;) {
+            k = k + 1u;
+            p = p * r.next();
+            if (p <= l) break;
+        }
+        return k - 1u;
+    }
+    float u1 = fmax(r.next(), 5.96e-8f);
+    float u2 = r.next();
+    float z = gpsm::sqrt(-2.0f * gpsm::log(u1)) * gpsm::cos(2.0f * PI_F * u2);
+    return f2u(fmax(gpsm::rint(lambda + gpsm::sqrt(lambda) * z), 0.0f));
+}
+
+// ------------------------------------------------------------------ projection
+struct Proj {
+    bool ok;
+    float mx, my, a, b, c, depth;
+};
+
+GPS_DEV float dot3(const float *r, float x, float y, float z) { return r[0] * x + r[1] * y + r[2] * z; }
+
+// gps_splat.wgsl:165-206
+GPS_DEV Proj project(const GpsFrame &f, float4 g0, float4 g1, float4 g2) {
+    Proj pr;
+    pr.ok = false;
+    pr.mx = pr.my = pr.a = pr.b = pr.c = pr.depth = 0.0f;
+    float px = g0.x - f.pos[0], py = g0.y - f.pos[1], pz = g0.z - f.pos[2];
+    float camx = dot3(f.rot[0], px, py, pz), camy = dot3(f.rot[1], px, py, pz), camz = dot3(f.rot[2], px, py, pz);
+    if (camz <= fmax(f.near_z, 1e-4f)) return pr;
+    const float lim = 5.2f;
+    float tx = clamp(camx / camz, -lim, lim) * camz;
+    float ty = clamp(camy / camz, -lim, lim) * camz;
+    float fx = f.intr[0], fy = f.intr[1];
+    float iz = 1.0f / camz;
+    float sx = fx * iz, sxz = -fx * tx * iz * iz;
+    float sy = fy * iz, syz = -fy * ty * iz * iz;
+    float m0[3], m1[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        m0[j] = f.rot[0][j] * sx + f.rot[2][j] * sxz;
+        m1[j] = f.rot[1][j] * sy + f.rot[2][j] * syz;
+    }
+    float ql = gpsm::sqrt(g2.x * g2.x + g2.y * g2.y + g2.z * g2.z + g2.w * g2.w);
+    float w = g2.x / ql, x = g2.y / ql, y = g2.z / ql, z = g2.w / ql;
+    float c0x = 1.0f - 2.0f * (y * y + z * z), c0y = 2.0f * (x * y + w * z), c0z = 2.0f * (x * z - w * y);
+    float c1x = 2.0f * (x * y - w * z), c1y = 1.0f - 2.0f * (x * x + z * z), c1z = 2.0f * (y * z + w * x);
+    float c2x = 2.0f * (x * z + w * y), c2y = 2.0f * (y * z - w * x), c2z = 1.0f - 2.0f * (x * x + y * y);
+    float s2x = gpsm::exp(2.0f * g1.x), s2y = gpsm::exp(2.0f * g1.y), s2z = gpsm::exp(2.0f * g1.z);
+    float ux = dot3(m0, c0x, c0y, c0z), uy = dot3(m0, c1x, c1y, c1z), uz = dot3(m0, c2x, c2y, c2z);
+    float vx = dot3(m1, c0x, c0y, c0z), vy = dot3(m1, c1x, c1y, c1z), vz = dot3(m1, c2x, c2y, c2z);
+    float a = s2x * (ux * ux) + s2y * (uy * uy) + s2z * (uz * uz) + f.pos[3];
+    float b = s2x * (ux * vx) + s2y * (uy * vy) + s2z * (uz * vz);
+    float c = s2x * (vx * vx) + s2y * (vy * vy) + s2z * (vz * vz) + f.pos[3];
+    float det = a * c - b * b;
+    if (!(det > 0.0f)) return pr;
+    pr.ok = true;
+    pr.mx = fx * camx / camz + f.intr[2];
+    pr.my = fy * camy / camz + f.intr[3];
+    pr.a = a;
+    pr.b = b;
+    pr.c = c;
+    pr.depth = camz;
+    return pr;
+}
+
+// gps_splat.wgsl:210-251 (evaluated once per splat per view)
+GPS_DEV float4 splat_colour(const GpsFrame &f, uint32_t id, float4 g0, float4 g3, const float *__restrict__ sh) {
+    float res[3] = {g3.x, g3.y, g3.z};
+    uint32_t deg = f.sh_degree;
+    if (deg == 0u) return make_float4(fmax(res[0], 0.0f), fmax(res[1], 0.0f), fmax(res[2], 0.0f), 0.0f);
+    float px = g0.x - f.pos[0], py = g0.y - f.pos[1], pz = g0.z - f.pos[2];
+    float len = gpsm::sqrt(px * px + py * py + pz * pz);
+    float x = px / len, y = py / len, z = pz / len;
+    uint32_t r = f.sh_stride;
+    uint32_t o = id * 3u * r;
+    float out[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        const float *S = sh + o + (uint32_t)ch * r;
+        float v = res[ch];
+        v = v - 0.4886025119029199f * y * S[0] + 0.4886025119029199f * z * S[1] - 0.4886025119029199f * x * S[2];
+        if (deg >= 2u) {
+            float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            v = v + 1.0925484305920792f * xy * S[3] - 1.0925484305920792f * yz * S[4] +
+                0.31539156525252005f * (2.0f * zz - xx - yy) * S[5] - 1.0925484305920792f * xz * S[6] +
+                0.5462742152960396f * (xx - yy) * S[7];
+            if (deg >= 3u) {
+                v = v - 0.5900435899266435f * y * (3.0f * xx - yy) * S[8] + 2.890611442640554f * xy * z * S[9] -
+                    0.4570457994644658f * y * (4.0f * zz - xx - yy) * S[10] +
+                    0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * S[11] -
+                    0.4570457994644658f * x * (4.0f * zz - xx - yy) * S[12] +
+                    1.445305721320277f * z * (xx - yy) * S[13] - 0.5900435899266435f * x * (xx - 3.0f * yy) * S[14];
+            }
+        }
+        out[ch] = fmax(v, 0.0f);
+    }
+    return make_float4(out[0], out[1], out[2], 0.0f);
+}
+
+// gps_splat.wgsl:288-300
+GPS_DEV bool offscreen(const GpsFrame &f, const Proj &pr) {
+    float ex = 5.9f * gpsm::sqrt(pr.a), ey = 5.9f * gpsm::sqrt(pr.c);
+    return pr.mx + ex < 0.0f || pr.mx - ex >= (float)f.W || pr.my + ey < 0.0f || pr.my - ey >= (float)f.H;
+}
+
+GPS_DEV float expected_count(const GpsFrame &f, const Proj &pr, float o) {
+    uint32_t spp = f.spp_side * f.spp_side;
+    float det = pr.a * pr.c - pr.b * pr.b;
+    return f.density * (float)spp * 2.0f * PI_F * gpsm::sqrt(det) * dilog(o);
+}
+
+// gps_splat.wgsl:417-460
+GPS_DEV float extinction_term(float o) {
+    float c = fmin(o, 1.0f - 1e-6f);
+    if (c < 1e-3f) return c * (1.0f + c * (0.5f + c * 0.33333334f));
+    return -gpsm::log(1.0f - c);
+}
+
+GPS_DEV float pixel_density_scale(const GpsFrame &f, float depth) {
+    if (!(depth > fmax(f.near_z, 0.0f))) return 0.0f;
+    float ratio = f.ref_len / depth;
+    return fmin(ratio * ratio * f.intr[0] * f.intr[1], 1.0e12f);
+}
+
+GPS_DEV float extinction_over_linear(float x) {
+    if (x < 1e-4f) return 1.0f + 0.5f * x;
+    return -gpsm::log(1.0f - fmin(x, 1.0f - 1e-6f)) / x;
+}
+
+GPS_DEV float pbvr_lambda(const GpsFrame &f, const Proj &pr, float o) {
+    float g = extinction_term(o);
+    if (f.method == M_PROPORTIONAL) g = fmin(o, 1.0f - 1e-6f);
+    float scale = f.base_k;
+    if (f.method != M_VIEW_CONDITIONED) {
+        if (f.calib == 1u) {
+            scale = scale * pixel_density_scale(f, f.object_depth);
+        } else if (f.calib == 2u) {
+            scale = scale * pixel_density_scale(f, pr.depth);
+        } else if (f.calib == 3u) {
+            float spp = (float)(f.spp_side * f.spp_side);
+            scale = spp * 2.0f * PI_F * gpsm::sqrt(pr.a * pr.c - pr.b * pr.b);
+        }
+    }
+    float n = f.density * g * scale;
+    if (!(n > 0.0f) || n > 3.0e38f) return 0.0f;
+    return n;
+}
+
+// gps_splat.wgsl:464-485
+GPS_DEV void chol_of(float4 g1, float4 g2, float4 &ca, float4 &cb) {
+    float ql = gpsm::sqrt(g2.x * g2.x + g2.y * g2.y + g2.z * g2.z + g2.w * g2.w);
+    float w = g2.x / ql, x = g2.y / ql, y = g2.z / ql, z = g2.w / ql;
+    float c0[3] = {1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + w * z), 2.0f * (x * z - w * y)};
+    float c1[3] = {2.0f * (x * y - w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + w * x)};
+    float c2[3] = {2.0f * (x * z + w * y), 2.0f * (y * z - w * x), 1.0f - 2.0f * (x * x + y * y)};
+    float sx = gpsm::exp(2.0f * g1.x), sy = gpsm::exp(2.0f * g1.y), sz = gpsm::exp(2.0f * g1.z);
+    float a00 = sx * c0[0] * c0[0] + sy * c1[0] * c1[0] + sz * c2[0] * c2[0];
+    float a10 = sx * c0[1] * c0[0] + sy * c1[1] * c1[0] + sz * c2[1] * c2[0];
+    float a20 = sx * c0[2] * c0[0] + sy * c1[2] * c1[0] + sz * c2[2] * c2[0];
+    float a11 = sx * c0[1] * c0[1] + sy * c1[1] * c1[1] + sz * c2[1] * c2[1];
+    float a21 = sx * c0[2] * c0[1] + sy * c1[2] * c1[1] + sz * c2[2] * c2[1];
+    float a22 = sx * c0[2] * c0[2] + sy * c1[2] * c1[2] + sz * c2[2] * c2[2];
+    float l00 = gpsm::sqrt(fmax(a00, 1e-30f));
+    float l10 = a10 / l00;
+    float l20 = a20 / l00;
+    float l11 = gpsm::sqrt(fmax(a11 - l10 * l10, 1e-30f));
+    float l21 = (a21 - l20 * l10) / l11;
+    float l22 = gpsm::sqrt(fmax(a22 - l20 * l20 - l21 * l21, 1e-30f));
+    ca = make_float4(l00, l10, l11, 0.0f);
+    cb = make_float4(l20, l21, l22, 0.0f);
+}
+
+// ------------------------------------------------------------------ depth|id minimum
+GPS_DEV void write_min(uint64_t *__restrict__ d, uint64_t v) {
+    // the buffer only decreases, so a stale cached value >= the current one: skipping when it is already
+    // <= v is exact and saves the L2 atomic for occluded particles
+    uint64_t seen = __hip_atomic_load(d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (v < seen) __hip_atomic_fetch_min(d, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+GPS_DEV uint64_t pack(float depth, uint32_t id) {
+    return ((uint64_t)(gpsm::bits(depth) | 0x80000000u) << 32) | (uint64_t)id;
+}
+
+// ------------------------------------------------------------------ block reductions
+template <typename T>
+GPS_DEV T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ================================================================== k_view
+__global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restrict__ splats,
+                                              const float *__restrict__ sh, float4 *__restrict__ view,
+                                              uint2 *__restrict__ cinfo, float4 *__restrict__ colour) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.N) return;
+    float4 g0 = splats[4u * i], g1 = splats[4u * i + 1u], g2 = splats[4u * i + 2u], g3 = splats[4u * i + 3u];
+    colour[i] = splat_colour(f, i, g0, g3, sh);
+    uint2 ci = make_uint2(0u, KIND_NONE);
+    Proj pr = project(f, g0, g1, g2);
+    float o = clamp(g0.w, 0.0f, 1.0f);
+    if (pr.ok && o > 0.0f) {
+        float l00 = gpsm::sqrt(fmax(pr.a, 1e-30f));
+        float l10 = pr.b / l00;
+        float l11 = gpsm::sqrt(fmax(pr.c - l10 * l10, 1e-30f));
+        float4 v0 = make_float4(pr.mx, pr.my, pr.depth, o);
+        float4 v1 = make_float4(l00, l10, l11, 1.0f);
+        float4 v2 = make_float4(0, 0, 0, 0), v3 = v2, v4 = v2, v5 = v2;
+        if (f.method == M_GPS) {
+            // prepare_gps, gps_splat.wgsl:535-550
+            if (!offscreen(f, pr)) {
+                float en = expected_count(f, pr, o);
+                if (en > 0.0f) ci = make_uint2(gpsm::bits(en), KIND_POISSON);
+            }
+        } else if (f.method == M_OCCUPANCY) {
+            // prepare_pbvr occupancy branch, gps_splat.wgsl:553-581
+            float density = fmax(f.density, 1e-8f);
+            float amin = fmax(1.0f - gpsm::exp(gpsm::log(1.0f - 1.0f / 255.0f) / density), 1e-8f);
+            if (o > amin) {
+                float radius = gpsm::sqrt(2.0f * gpsm::log(o / amin));
+                float ex = radius * gpsm::sqrt(pr.a), ey = radius * gpsm::sqrt(pr.c);
+                float Wf = (float)f.W, Hf = (float)f.H;
+                float lox = clamp(floorf(pr.mx - ex), 0.0f, Wf), loy = clamp(floorf(pr.my - ey), 0.0f, Hf);
+                float hix = clamp(ceilf(pr.mx + ex), 0.0f, Wf), hiy = clamp(ceilf(pr.my + ey), 0.0f, Hf);
+                float szx = fmax(hix - lox, 0.0f), szy = fmax(hiy - loy, 0.0f);
+                uint32_t count = f2u(szx) * f2u(szy) * f.spp_side * f.spp_side;
+                if (count != 0u) {
+                    float invdet = 1.0f / fmax(pr.a * pr.c - pr.b * pr.b, 1e-30f);
+                    v1 = make_float4(lox, loy, szx, szy);
+                    float peak_tau = -gpsm::log(1.0f - fmin(o, 1.0f - 1e-6f));
+                    bool sparse = density * peak_tau < 1.0f;
+                    v2 = make_float4(pr.c * invdet, -pr.b * invdet, pr.a * invdet, peak_tau);
+                    v3 = make_float4(0.0f, 0.0f, 0.0f, sparse ? 1.0f : 0.0f);
+                    if (sparse) ci = make_uint2(gpsm::bits((float)count * density * peak_tau), KIND_POISSON);
+                    else ci = make_uint2(count, KIND_FIXED);
+                }
+            }
+        } else {
+            // prepare_pbvr, gps_splat.wgsl:582-611
+            float lambda = pbvr_lambda(f, pr, o);
+            if (lambda > 0.0f) {
+                float det = pr.a * pr.c - pr.b * pr.b;
+                float keep = 1.0f;
+                uint32_t kind = KIND_POISSON;
+                if (f.method == M_VIEW_CONDITIONED) {
+                    float spp = (float)(f.spp_side * f.spp_side);
+                    float target = f.density * spp * 2.0f * PI_F * gpsm::sqrt(det) * dilog(o);
+                    if (lambda > 0.5f) keep = clamp(target / lambda, 0.0f, 1.0f); else keep = 0.0f;
+                    if (target > lambda) kind |= KIND_UNDERCOVERED;
+                }
+                float idet = 1.0f / fmax(det, 1e-30f);
+                v1.w = keep;
+                v2 = make_float4(g0.x, g0.y, g0.z, pr.c * idet);
+                float oo = fmin(o, 1.0f - 1e-6f);
+                v3 = make_float4(-pr.b * idet, pr.a * idet, extinction_over_linear(oo), 0.0f);
+                chol_of(g1, g2, v4, v5);
+                ci = make_uint2(gpsm::bits(lambda), kind);
+            }
+        }
+        float4 *v = view + (size_t)VIEW_VEC4 * i;
+        v[0] = v0;
+        v[1] = v1;
+        v[2] = v2;
+        v[3] = v3;
+        v[4] = v4;
+        v[5] = v5;
+    }
+    cinfo[i] = ci;
+}
+
+// ================================================================== decoupled look-back
+// status word: [63:38] epoch, [37:36] flag (1 aggregate, 2 inclusive prefix), [35:0] saturated value
+constexpr uint64_t ST_VAL_MASK = (1ull << 36) - 1ull;
+constexpr uint32_t ST_EPOCH_MASK = (1u << 26) - 1u;
+
+GPS_DEV uint64_t sat_add(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    return s > ST_VAL_MASK ? ST_VAL_MASK : s;
+}
+
+// Called by one thread per tile: publishes the aggregate and returns the exclusive prefix of `tile`.
+GPS_DEV uint64_t lookback(uint64_t *__restrict__ status, uint32_t tile, uint64_t agg, uint32_t epoch) {
+    const uint64_t tag = (uint64_t)(epoch & ST_EPOCH_MASK) << 38;
+    agg = agg > ST_VAL_MASK ? ST_VAL_MASK : agg;
+    if (tile == 0u) {
+        __hip_atomic_store(&status[0], tag | (2ull << 36) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0ull;
+    }
+    __hip_atomic_store(&status[tile], tag | (1ull << 36) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t excl = 0ull;
+    int32_t t = (int32_t)tile - 1;
+    for (;;) {
+        uint64_t w;
+        for (;;) {
+            w = __hip_atomic_load(&status[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint32_t)(w >> 38) == (epoch & ST_EPOCH_MASK) && ((w >> 36) & 3ull) != 0ull) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        excl = sat_add(excl, w & ST_VAL_MASK);
+        if (((w >> 36) & 3ull) == 2ull) break;
+        t--;
+    }
+    __hip_atomic_store(&status[tile], tag | (2ull << 36) | sat_add(excl, agg), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
+
+// Block-wide exclusive scan of one u64 per thread (256 threads = 4 waves); returns the block total.
+GPS_DEV uint64_t block_exclusive_scan(uint64_t v, uint64_t &excl, uint64_t *s_wave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint64_t t = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    uint64_t wave_base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; w++) {
+        uint64_t s = s_wave[w];
+        if (w < wave) wave_base += s;
+        total += s;
+    }
+    excl = wave_base + inc - v;
+    return total;
+}
+
+// ================================================================== k_count_scan
+// grid = (tiles, ensembles of the batch)
+__global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan(GpsFrame f, const uint2 *__restrict__ cinfo,
+                                                           uint32_t first_seed, uint32_t *__restrict__ offsets,
+                                                           BatchState bs, uint32_t tiles, uint32_t epoch,
+                                                           uint64_t *__restrict__ stats) {
+    const uint32_t e = blockIdx.y;
+    const uint32_t seed = first_seed + e;
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_wave[SCAN_BLOCK / 64];
+    __shared__ uint64_t s_prefix;
+    __shared__ uint64_t s_stats[6][SCAN_BLOCK / 64];
+    if (threadIdx.x == 0) s_tile = atomicAdd(&bs.tickets[e], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t N = f.N;
+    const uint32_t base = tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const bool gps = f.method == M_GPS, occ = f.method == M_OCCUPANCY;
+    uint32_t cnt[SCAN_ITEMS];
+    uint64_t tsum = 0, st_pts = 0, st_trunc = 0, st_drop = 0, st_vis = 0, st_cand = 0, st_under = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        uint32_t i = base + (uint32_t)j;
+        uint32_t c = 0;
+        if (i < N) {
+            uint2 ci = cinfo[i];
+            uint32_t kind = ci.y & 3u;
+            if (kind != KIND_NONE) {
+                st_vis += 1;
+                if (ci.y & KIND_UNDERCOVERED) st_under += 1;
+                uint32_t n;
+                if (kind == KIND_FIXED) {
+                    n = ci.x;
+                } else {
+                    Rng r(particle_seed(i, 0u, seed));
+                    n = poisson(r, gpsm::fbits(ci.x));
+                }
+                if (gps) st_pts += n; else st_cand += n;
+                if (!occ && n > f.cap) {
+                    st_trunc += 1;
+                    st_drop += n - f.cap;
+                    n = f.cap;
+                }
+                c = n;
+            }
+        }
+        cnt[j] = c;
+        tsum += c;
+    }
+    uint64_t texcl;
+    uint64_t agg = block_exclusive_scan(tsum, texcl, s_wave);
+    if (threadIdx.x == 0) s_prefix = lookback(bs.status + (size_t)e * tiles, tile, agg, epoch);
+    // statistics: one atomic per counter per tile
+    {
+        uint64_t v[6] = {st_pts, st_trunc, st_drop, st_vis, st_cand, st_under};
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            uint64_t s = wave_sum(v[k]);
+            if (lane == 0) s_stats[k][wave] = s;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        uint64_t s = 0;
+        for (int w = 0; w < SCAN_BLOCK / 64; w++) s += s_stats[threadIdx.x][w];
+        const int slot[6] = {ST_POINTS, ST_TRUNC, ST_DROPPED, ST_VISIBLE, ST_CAND, ST_UNDER};
+        if (s) atomicAdd((unsigned long long *)&stats[slot[threadIdx.x]], (unsigned long long)s);
+    }
+    const uint64_t prefix = s_prefix;
+    uint32_t *out = offsets + (size_t)e * N;
+    uint64_t run = prefix + texcl;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        uint32_t i = base + (uint32_t)j;
+        if (i < N) out[i] = (uint32_t)run;
+        run += cnt[j];
+    }
+    if (tile == tiles - 1u && threadIdx.x == 0) {
+        uint64_t total = prefix + agg;
+        bool over = total > 0xffffffffull;
+        bs.totals[e] = over ? 0u : (uint32_t)total;
+        bs.skip[e] = over ? 1u : 0u;
+        if (over) atomicAdd((unsigned long long *)&stats[ST_SKIPPED], 1ull);
+        bs.tickets[e] = 0u;   // every tile of this ensemble has drawn its ticket
+    }
+}
+
+// ================================================================== particles
+GPS_DEV uint32_t owner_search(const uint32_t *__restrict__ offs, uint32_t lo, uint32_t hi, uint32_t p) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1u) >> 1;
+        if (offs[mid] <= p) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// gps_splat.wgsl:259-282 (GPS)
+GPS_DEV void emit_gps(const GpsFrame &f, Rng &r, uint32_t id, float4 v0, float4 v1, const float *__restrict__ lut,
+                      uint64_t *__restrict__ depth) {
+    float u = r.next();
+    float rad = f.exact_radius == 0u ? radius_lut(lut, v0.w, u) : corrected_radius(v0.w, u);
+    float th = 2.0f * PI_F * r.next();
+    float sn, cs;
+    gpsm::sincos(th, sn, cs);
+    float wx = rad * cs, wy = rad * sn;
+    float pxf = v0.x + v1.x * wx, pyf = v0.y + (v1.y * wx + v1.z * wy);
+    if (pxf < 0.0f || pxf >= (float)f.W || pyf < 0.0f || pyf >= (float)f.H) return;
+    const uint32_t side = f.spp_side;
+    uint32_t px = f2u(pxf), py = f2u(pyf);
+    uint32_t sx = f2u((pxf - (float)px) * (float)side), sy = f2u((pyf - (float)py) * (float)side);
+    if (sx > side - 1u) sx = side - 1u;
+    if (sy > side - 1u) sy = side - 1u;
+    uint32_t idx = (py * f.W + px) * (side * side) + sy * side + sx;
+    write_min(depth + idx, pack(v0.z, id));
+}
+
+// gps_splat.wgsl:490-532 (PBVR); returns 1 if the particle counts as a generated point
+GPS_DEV uint32_t emit_pbvr(const GpsFrame &f, Rng &r, uint32_t id, const float4 *__restrict__ v,
+                           uint64_t *__restrict__ depth) {
+    float uz = r.normal();
+    float uy = r.normal();
+    float ux = r.normal();
+    float4 a = v[0];
+    if (f.method == M_VIEW_CONDITIONED) {
+        if (r.next() >= v[1].w) return 0u;
+    }
+    float4 c = v[2], ca = v[4], cb = v[5];
+    float wx = c.x + ca.x * ux;
+    float wy = c.y + (ca.y * ux + ca.z * uy);
+    float wz = c.z + (cb.x * ux + cb.y * uy + cb.z * uz);
+    float px = wx - f.pos[0], py = wy - f.pos[1], pz = wz - f.pos[2];
+    float camx = dot3(f.rot[0], px, py, pz), camy = dot3(f.rot[1], px, py, pz), camz = dot3(f.rot[2], px, py, pz);
+    if (camz <= f.near_z) return 0u;
+    float pixx = f.intr[0] * camx / camz + f.intr[2];
+    float pixy = f.intr[1] * camy / camz + f.intr[3];
+    if (f.flags & FLAG_JITTER) {
+        float jy = r.normal();
+        float jx = r.normal();
+        float s = gpsm::sqrt(fmax(f.pos[3], 0.0f));
+        pixx = pixx + s * jx;
+        pixy = pixy + s * jy;
+    }
+    if (f.flags & FLAG_RADIAL) {
+        float4 d = v[3];
+        float dvx = pixx - a.x, dvy = pixy - a.y;
+        float r2 = c.w * dvx * dvx + 2.0f * d.x * dvx * dvy + d.y * dvy * dvy;
+        // radial_keep, gps_splat.wgsl:434-438, with h(min(o, 1-1e-6)) cached per splat in d.z
+        float oo = fmin(a.w, 1.0f - 1e-6f);
+        float y = oo * gpsm::exp(-0.5f * fmax(r2, 0.0f));
+        float keep = clamp(extinction_over_linear(y) / d.z, 0.0f, 1.0f);
+        if (r.next() >= keep) return 0u;
+    }
+    float depth_z = (f.flags & FLAG_CENTRE) ? a.z : camz;
+    if (pixx < 0.0f || pixx >= (float)f.W || pixy < 0.0f || pixy >= (float)f.H) return 1u;
+    const uint32_t side = f.spp_side;
+    uint32_t ipx = f2u(pixx), ipy = f2u(pixy);
+    uint32_t sx = f2u((pixx - (float)ipx) * (float)side), sy = f2u((pixy - (float)ipy) * (float)side);
+    if (sx > side - 1u) sx = side - 1u;
+    if (sy > side - 1u) sy = side - 1u;
+    uint32_t idx = (ipy * f.W + ipx) * (side * side) + sy * side + sx;
+    write_min(depth + idx, pack(depth_z, id));
+    return 1u;
+}
+
+// gps_splat.wgsl:686-708 (screen occupancy); returns 1 on a hit
+GPS_DEV uint32_t emit_occupancy(const GpsFrame &f, Rng &r, uint32_t id, uint32_t k, const float4 *__restrict__ v,
+                                uint64_t *__restrict__ depth) {
+    const uint32_t side = f.spp_side, samples = side * side;
+    float4 a = v[0], b = v[1], inv = v[2];
+    bool sparse = v[3].w > 0.0f;
+    uint32_t cell = k;
+    if (sparse) {
+        uint32_t a1 = f2u(r.next() * b.z * b.w * (float)samples);
+        uint32_t a2 = f2u(b.z) * f2u(b.w) * samples - 1u;
+        cell = a1 < a2 ? a1 : a2;
+    }
+    uint32_t pixel = cell / samples, sub = cell % samples;
+    uint32_t bz = f2u(b.z);
+    uint32_t xyx = f2u(b.x) + pixel % bz, xyy = f2u(b.y) + pixel / bz;
+    float n1 = r.next();
+    float n2 = r.next();
+    float uvx = ((float)(sub % side) + n1) / (float)side;
+    float uvy = ((float)(sub / side) + n2) / (float)side;
+    float dx = (float)xyx + uvx - a.x, dy = (float)xyy + uvy - a.y;
+    float r2 = fmax(inv.x * dx * dx + 2.0f * inv.y * dx * dy + inv.z * dy * dy, 0.0f);
+    float alpha = fmin(a.w * gpsm::exp(-0.5f * r2), 1.0f - 1e-6f);
+    float coverage;
+    if (sparse) coverage = -gpsm::log(1.0f - alpha) / inv.w;
+    else coverage = 1.0f - gpsm::exp(f.density * gpsm::log(1.0f - alpha));
+    if (r.next() >= coverage) return 0u;
+    uint32_t idx = (xyy * f.W + xyx) * samples + sub;
+    write_min(depth + idx, pack(a.z, id));
+    return 1u;
+}
+
+template <int MK, int PPT>
+__global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const float4 *__restrict__ view,
+                                                          const uint32_t *__restrict__ offsets,
+                                                          const uint32_t *__restrict__ totals, uint32_t nens,
+                                                          uint32_t first_seed, uint64_t *__restrict__ depth,
+                                                          uint64_t sub_count, const float *__restrict__ lut,
+                                                          uint64_t *__restrict__ stats) {
+    constexpr uint32_t CH = PART_BLOCK * PPT;
+    __shared__ uint32_t s_cstart[MAX_BATCH + 1];
+    __shared__ uint32_t s_lo, s_hi;
+    __shared__ uint64_t s_pts[PART_BLOCK / 64];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t b = 0; b < nens; b++) {
+            s_cstart[b] = acc;
+            acc += (totals[b] + CH - 1u) / CH;
+        }
+        s_cstart[nens] = acc;
+    }
+    __syncthreads();
+    const uint32_t nchunks = s_cstart[nens];
+    const uint32_t N = f.N;
+    uint64_t pts = 0;
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        uint32_t b = 0;
+        while (b + 1u < nens && s_cstart[b + 1u] <= chunk) b++;
+        const uint32_t total = totals[b];
+        const uint32_t p0 = (chunk - s_cstart[b]) * CH;
+        const uint32_t pend = min(p0 + CH, total);
+        const uint32_t *offs = offsets + (size_t)b * N;
+        uint64_t *dep = depth + (size_t)b * sub_count;
+        const uint32_t seed = first_seed + b;
+        if (threadIdx.x < 2u) {
+            uint32_t s = owner_search(offs, 0u, N - 1u, threadIdx.x == 0 ? p0 : pend - 1u);
+            if (threadIdx.x == 0) s_lo = s; else s_hi = s;
+        }
+        __syncthreads();
+        const uint32_t pfirst = p0 + threadIdx.x * PPT;
+        if (pfirst < pend) {
+            uint32_t i = owner_search(offs, s_lo, s_hi, pfirst);
+            uint32_t cur = offs[i];
+            uint32_t nxt = (i + 1u < N) ? offs[i + 1u] : total;
+#pragma unroll 1
+            for (uint32_t j = 0; j < PPT; j++) {
+                const uint32_t p = pfirst + j;
+                if (p >= pend) break;
+                while (p >= nxt) {
+                    i++;
+                    cur = nxt;
+                    nxt = (i + 1u < N) ? offs[i + 1u] : total;
+                }
+                const uint32_t k = p - cur;
+                const float4 *v = view + (size_t)VIEW_VEC4 * i;
+                Rng r(particle_seed(i, k + 1u, seed));
+                if (MK == 0) {
+                    emit_gps(f, r, i, v[0], v[1], lut, dep);
+                } else if (MK == 1) {
+                    pts += emit_pbvr(f, r, i, v, dep);
+                } else {
+                    pts += emit_occupancy(f, r, i, k, v, dep);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (MK != 0) {
+        uint64_t s = wave_sum(pts);
+        if ((threadIdx.x & 63) == 0) s_pts[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t = 0;
+            for (int w = 0; w < PART_BLOCK / 64; w++) t += s_pts[w];
+            if (t) atomicAdd((unsigned long long *)&stats[ST_POINTS], (unsigned long long)t);
+        }
+    }
+}
+
+// ================================================================== splat-parallel GPS path
+// gps_splat.wgsl:304-333: one thread per splat, one sequential stream for its count and particles
+__global__ __launch_bounds__(64) void k_splat_pass(GpsFrame f, const float4 *__restrict__ view,
+                                                   const uint2 *__restrict__ cinfo, uint32_t seed,
+                                                   uint64_t *__restrict__ depth, const float *__restrict__ lut,
+                                                   uint64_t *__restrict__ stats) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= f.N) return;
+    uint2 ci = cinfo[i];
+    if ((ci.y & 3u) == KIND_NONE) return;
+    Rng r(particle_seed(i, 0u, seed));
+    uint32_t n = poisson(r, gpsm::fbits(ci.x));
+    atomicAdd((unsigned long long *)&stats[ST_POINTS], (unsigned long long)n);
+    atomicAdd((unsigned long long *)&stats[ST_VISIBLE], 1ull);
+    if (n > f.cap) {
+        atomicAdd((unsigned long long *)&stats[ST_TRUNC], 1ull);
+        atomicAdd((unsigned long long *)&stats[ST_DROPPED], (unsigned long long)(n - f.cap));
+        n = f.cap;
+    }
+    const float4 *v = view + (size_t)VIEW_VEC4 * i;
+    float4 v0 = v[0], v1 = v[1];
+    for (uint32_t k = 0; k < n; k++) emit_gps(f, r, i, v0, v1, lut, depth);
+}
+
+// ================================================================== resolve
+// gps_splat.wgsl:361-386, for every ensemble of the batch in ensemble order
+__global__ __launch_bounds__(256) void k_resolve(GpsFrame f, uint64_t *__restrict__ depth, uint64_t sub_count,
+                                                 const uint32_t *__restrict__ skip, uint32_t nens,
+                                                 const float4 *__restrict__ colour, float4 *__restrict__ accum,
+                                                 int clear, uint64_t *__restrict__ stats) {
+    const uint32_t X = f.W * f.H;
+    const uint32_t spp = f.spp_side * f.spp_side;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) atomicAdd((unsigned long long *)&stats[ST_ENSEMBLES], (unsigned long long)nens);
+    if (i >= X) return;
+    float4 acc = accum[i];
+    const float fs = (float)spp;
+    for (uint32_t b = 0; b < nens; b++) {
+        uint64_t *d = depth + (size_t)b * sub_count + (size_t)i * spp;
+        if (skip == nullptr || skip[b] == 0u) {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (uint32_t k = 0; k < spp; k++) {
+                uint64_t w = d[k];
+                if (w == EMPTY64) {
+                    sx = sx + f.bg[0];
+                    sy = sy + f.bg[1];
+                    sz = sz + f.bg[2];
+                } else {
+                    float4 c = colour[(uint32_t)w];
+                    sx = sx + c.x;
+                    sy = sy + c.y;
+                    sz = sz + c.z;
+                }
+            }
+            acc.x = acc.x + sx / fs;
+            acc.y = acc.y + sy / fs;
+            acc.z = acc.z + sz / fs;
+            acc.w = acc.w + 1.0f;
+        }
+        if (clear)
+            for (uint32_t k = 0; k < spp; k++) d[k] = EMPTY64;
+    }
+    accum[i] = acc;
+}
+
+__global__ void k_fill_u64(uint64_t *__restrict__ p, uint64_t n, uint64_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_zero_u32(uint32_t *__restrict__ p, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void k_stats_add(uint64_t *stats, int slot, uint64_t v) {
+    if (threadIdx.x == 0) stats[slot] += v;
+}
+
+__global__ void k_stats_clear(uint64_t *stats, int first, int count) {
+    if ((int)threadIdx.x < count) stats[first + threadIdx.x] = 0ull;
+}
+
+__global__ void k_accum_clear(float4 *__restrict__ a, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ================================================================== composite
+// gps_composite.wgsl:153-179 into packed RGBA8 (R in the low byte)
+GPS_DEV float oetf(float c) {
+    if (c <= 0.0031308f) return 12.92f * c;
+    return 1.055f * gpsm::exp((1.0f / 2.4f) * gpsm::log(c)) - 0.055f;
+}
+
+__global__ __launch_bounds__(256) void k_composite(const float4 *__restrict__ accum, uint32_t W, uint32_t H,
+                                                   float exposure, uint32_t gamma_on, uint32_t ow, uint32_t oh,
+                                                   uint32_t *__restrict__ out) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ow * oh) return;
+    uint32_t X = t % ow, Y = t / ow;
+    float uvx = ((float)X + 0.5f) / (float)ow, uvy = ((float)Y + 0.5f) / (float)oh;
+    float qx = uvx * (float)W - 0.5f, qy = uvy * (float)H - 0.5f;
+    float flx = floorf(qx), fly = floorf(qy);
+    int px = (int)flx, py = (int)fly;
+    float fx = qx - flx, fy = qy - fly;
+    float lin[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int xx = px + (k & 1), yy = py + (k >> 1);
+        xx = xx < 0 ? 0 : (xx > (int)W - 1 ? (int)W - 1 : xx);
+        yy = yy < 0 ? 0 : (yy > (int)H - 1 ? (int)H - 1 : yy);
+        float4 a = accum[(size_t)yy * W + (size_t)xx];
+        float wgt = fmax(a.w, 1.0f);
+        lin[k][0] = a.x / wgt;
+        lin[k][1] = a.y / wgt;
+        lin[k][2] = a.z / wgt;
+    }
+    uint32_t packed = 0xff000000u;
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        float top = mix(lin[0][ch], lin[1][ch], fx);
+        float bot = mix(lin[2][ch], lin[3][ch], fx);
+        float c = mix(top, bot, fy) * exposure;
+        c = clamp(c, 0.0f, 1.0f);
+        if (gamma_on) c = oetf(c);
+        packed |= (f2u(gpsm::rint(c * 255.0f)) & 0xffu) << (8 * ch);
+    }
+    out[t] = packed;
+}
+
+// ================================================================== phase-0 unit kernels
+// shaders/p0_sample.wgsl:205-243 with the two passes collapsed into one packed 64-bit minimum:
+// (depth key << 32) | colour.  The colour tie rule (smallest colour among equal keys) is the
+// low word's order.
+__global__ __launch_bounds__(256) void k_phase0(uint32_t seed, uint32_t pixels, uint32_t samples,
+                                                uint64_t *__restrict__ buf) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= samples) return;
+    uint32_t h0 = pcg(i ^ pcg(seed)), h1 = pcg(h0), h2 = pcg(h1);
+    uint32_t pix = h0 % pixels;
+    uint32_t key = ((0x3F800000u | (h1 & 0x007FFFFFu)) | 0x80000000u) & 0xFFFF0000u;
+    uint32_t rgb = h2 & 0x00FFFFFFu;
+    __hip_atomic_fetch_min(&buf[pix], ((uint64_t)key << 32) | rgb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// exclusive u32 scan of a plain array with the same tile / look-back machinery as k_count_scan
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                         uint64_t n, uint32_t *ticket, uint64_t *status,
+                                                         uint32_t tiles, uint32_t epoch) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_wave[SCAN_BLOCK / 64];
+    __shared__ uint64_t s_prefix;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint64_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        v[j] = base + j < n ? in[base + j] : 0u;
+        tsum += v[j];
+    }
+    uint64_t texcl;
+    uint64_t agg = block_exclusive_scan(tsum, texcl, s_wave);
+    if (threadIdx.x == 0) s_prefix = lookback(status, tile, agg, epoch);
+    __syncthreads();
+    uint64_t run = s_prefix + texcl;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        if (base + j < n) out[base + j] = (uint32_t)run;
+        run += v[j];
+    }
+    if (tile == tiles - 1u && threadIdx.x == 0) *ticket = 0u;
+}
+
+__global__ void k_math(int fn, const float *__restrict__ x, const float *__restrict__ y, float *__restrict__ out,
+                       uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a = x[i], r;
+    switch (fn) {
+    case 0: r = gpsm::exp(a); break;
+    case 1: r = gpsm::log(a); break;
+    case 2: r = gpsm::sin(a); break;
+    case 3: r = gpsm::cos(a); break;
+    case 4: r = gpsm::sqrt(a); break;
+    default: r = a / y[i]; break;
+    }
+    out[i] = r;
+}
+
+}  // namespace
+
+// ================================================================== launchers
+static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_view(hipStream_t s, const GpsFrame &f, const float4 *splats, const float *sh, float4 *view, uint2 *cinfo,
+                 float4 *colour) {
+    if (f.N == 0) return;
+    hipLaunchKernelGGL(k_view, dim3(blocks_for(f.N, 256)), dim3(256), 0, s, f, splats, sh, view, cinfo, colour);
+}
+
+void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uint32_t first_seed, uint32_t nens,
+                       uint32_t *offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch, uint64_t *stats) {
+    hipLaunchKernelGGL(k_count_scan, dim3(tiles, nens), dim3(SCAN_BLOCK), 0, s, f, cinfo, first_seed, offsets, bs,
+                       tiles, epoch, stats);
+}
+
+void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
+                      const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
+                      const float *lut, uint64_t *stats, int grid) {
+    constexpr int PPT = 4;
+    if (f.method == M_GPS)
+        hipLaunchKernelGGL((k_particles<0, PPT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals,
+                           nens, first_seed, depth, sub_count, lut, stats);
+    else if (f.method == M_OCCUPANCY)
+        hipLaunchKernelGGL((k_particles<2, PPT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals,
+                           nens, first_seed, depth, sub_count, lut, stats);
+    else
+        hipLaunchKernelGGL((k_particles<1, PPT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals,
+                           nens, first_seed, depth, sub_count, lut, stats);
+}
+
+void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
+                    uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats) {
+    uint64_t X = (uint64_t)f.W * f.H;
+    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(X, 256)), dim3(256), 0, s, f, depth, sub_count, skip, nens, colour,
+                       accum, clear, stats);
+}
+
+void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v) {
+    if (n == 0) return;
+    unsigned grid = blocks_for(n, 256);
+    if (grid > 8192) grid = 8192;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, s, p, n, v);
+}
+
+void launch_splat_pass(hipStream_t s, const GpsFrame &f, const float4 *view, const uint2 *cinfo, uint32_t seed,
+                       uint64_t *depth, const float *lut, uint64_t *stats) {
+    if (f.N == 0) return;
+    hipLaunchKernelGGL(k_splat_pass, dim3(blocks_for(f.N, 64)), dim3(64), 0, s, f, view, cinfo, seed, depth, lut,
+                       stats);
+}
+
+void launch_composite(hipStream_t s, const float4 *accum, uint32_t W, uint32_t H, float exposure, uint32_t gamma_on,
+                      uint32_t ow, uint32_t oh, uint32_t *out) {
+    hipLaunchKernelGGL(k_composite, dim3(blocks_for((uint64_t)ow * oh, 256)), dim3(256), 0, s, accum, W, H, exposure,
+                       gamma_on, ow, oh, out);
+}
+
+void launch_zero_u32(hipStream_t s, uint32_t *p, uint64_t n) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+}
+
+void launch_stats_add(hipStream_t s, uint64_t *stats, int slot, uint64_t v) {
+    hipLaunchKernelGGL(k_stats_add, dim3(1), dim3(64), 0, s, stats, slot, v);
+}
+
+void launch_stats_clear(hipStream_t s, uint64_t *stats, int first, int count) {
+    hipLaunchKernelGGL(k_stats_clear, dim3(1), dim3(64), 0, s, stats, first, count);
+}
+
+void launch_accum_clear(hipStream_t s, float4 *accum, uint64_t n) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_accum_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, accum, n);
+}
+
+void launch_phase0(hipStream_t s, uint32_t seed, uint32_t pixels, uint32_t samples, uint64_t *buf) {
+    hipLaunchKernelGGL(k_phase0, dim3(blocks_for(samples, 256)), dim3(256), 0, s, seed, pixels, samples, buf);
+}
+
+void launch_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ticket, uint64_t *status,
+                     uint32_t tiles, uint32_t epoch) {
+    hipLaunchKernelGGL(k_scan_u32, dim3(tiles), dim3(SCAN_BLOCK), 0, s, in, out, n, ticket, status, tiles, epoch);
+}
+
+void launch_math(hipStream_t s, int fn, const float *x, const float *y, float *out, uint64_t n) {
+    hipLaunchKernelGGL(k_math, dim3(blocks_for(n, 256)), dim3(256), 0, s, fn, x, y, out, n);
+}

@@ -1,0 +1,752 @@
+// Host side of the renderer: the C ABI of include/gps_hip.h over the kernels of gps_kernels.hip.
+//
+// Mirrors gps_render::gps::GpsRenderer (crates/gps-render/src/gps.rs:302-1065): it owns every device
+// buffer, packs the per-view constants like upload_frame (gps.rs:812-856), and runs ensembles in
+// batches (one depth|id buffer per ensemble of a batch) on one in-order HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gps_hip.h"
+#include "gps_internal.h"
+
+// sorted path (gps_sorted.hip)
+struct SortedState;
+SortedState *sorted_create();
+void sorted_destroy(SortedState *s);
+int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float4 *splats, const float *sh,
+                  float4 *accum, uint64_t *stats, int probe_only, std::string &err,
+                  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> *prof);
+int sorted_read(SortedState *s, hipStream_t st, uint32_t n, uint32_t *keys, uint32_t *vals, std::string &err);
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr uint64_t MAX_SPLATS = 1ull << 30;
+constexpr uint32_t LUT_O = 65, LUT_V = 129;
+
+// ------------------------------------------------------------------ radius table (f64, lib.rs:178-248, 522-636)
+double dilog_series(double x) {
+    double term = x, sum = x;
+    for (int k = 2; k < 1000; k++) {
+        term *= x;
+        double add = term / ((double)k * (double)k);
+        sum += add;
+        if (std::fabs(add) < 1e-18 * std::fabs(sum)) break;
+    }
+    return sum;
+}
+
+constexpr double DILOG_AT_ONE = 1.6449340668482264;
+
+double dilog64(double x) {
+    if (x >= 1.0) return DILOG_AT_ONE;
+    if (x <= -1.0) return -DILOG_AT_ONE / 2.0;
+    if (x == 0.0) return 0.0;
+    if (x < 0.0) {
+        double y = x / (x - 1.0);
+        double l = std::log(1.0 - x);
+        return -dilog_series(y) - 0.5 * l * l;
+    }
+    if (x <= 0.5) return dilog_series(x);
+    return DILOG_AT_ONE - std::log(x) * std::log(1.0 - x) - dilog_series(1.0 - x);
+}
+
+double inv_dilog64(double target) {
+    double below_one = std::nextafter(DILOG_AT_ONE, 0.0);
+    target = target < 0.0 ? 0.0 : (target > below_one ? below_one : target);
+    if (target <= 0.0) return 0.0;
+    double lo = 0.0, hi = 1.0, y = target / DILOG_AT_ONE;
+    for (int i = 0; i < 40; i++) {
+        double f = dilog64(y) - target;
+        if (f > 0.0) hi = y; else lo = y;
+        double dfdy = (y > 1e-12 && y < 1.0 - 1e-15) ? -std::log(1.0 - y) / y : 1.0;
+        double ny = y - f / dfdy;
+        if (!(ny > lo && ny < hi)) ny = 0.5 * (lo + hi);
+        if (std::fabs(ny - y) <= 1e-14 * (1.0 + y)) return ny;
+        y = ny;
+    }
+    return y;
+}
+
+double corrected_radius64(double opacity, double u) {
+    double o = opacity < 0.0 ? 0.0 : (opacity > 1.0 ? 1.0 : opacity);
+    if (o <= 0.0) return 0.0;
+    u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+    double d = dilog64(o);
+    double y = std::fmin(inv_dilog64((1.0 - u) * d), o);
+    if (y <= 0.0) return 40.0;
+    return std::sqrt(std::fmax(-2.0 * std::log(y / o), 0.0));
+}
+
+void make_lut(float *out) {
+    for (uint32_t i = 0; i < LUT_O; i++) {
+        double x = (double)i / (double)(LUT_O - 1);
+        double o = 1.0 - (1.0 - x) * (1.0 - x);
+        o = o < 1e-6 ? 1e-6 : (o > 1.0 ? 1.0 : o);
+        for (uint32_t j = 0; j < LUT_V; j++) {
+            double w = (double)j * std::sqrt(17.0) / (double)(LUT_V - 1);
+            double u = -std::expm1(-w * w);
+            out[i * LUT_V + j] = (float)corrected_radius64(o, u);
+        }
+    }
+}
+
+}  // namespace
+
+struct GpsRenderer {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t W = 0, H = 0, spp_side = 1;
+    uint64_t n = 0;
+    uint32_t sh_max_degree = 0, sh_stride = 0;
+    double centroid[3] = {0, 0, 0};
+    std::string err;
+
+    // scene
+    float4 *splats = nullptr;
+    float *sh = nullptr;
+    float *lut = nullptr;
+    float4 *view = nullptr;
+    uint2 *cinfo = nullptr;
+    float4 *colour = nullptr;
+    // batch
+    uint32_t *offsets = nullptr;   // [batch_cap][n]
+    uint64_t *depth = nullptr;     // [batch_cap][S]
+    uint32_t batch_cap = 0;
+    uint32_t tiles = 0;
+    BatchState bs{};
+    uint32_t epoch = 1;
+    bool depth_dirty = false;
+    // frame
+    float4 *accum = nullptr;
+    uint64_t *stats = nullptr;
+    uint32_t *composite_buf = nullptr;
+    uint64_t composite_cap = 0;
+    uint32_t accumulated = 0;
+    uint32_t debug = 0;
+    // last particle-path batch slot for the debug hooks
+    bool have_last = false;
+    SortedState *sorted = nullptr;
+    // profiling
+    bool profiling = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::map<std::string, std::pair<uint64_t, double>> times;
+};
+
+namespace {
+
+int fail(GpsRenderer *r, const std::string &m) {
+    r->err = m;
+    return -1;
+}
+
+#define HIP_TRY(r, call)                                                                     \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail((r), std::string(#call) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+uint64_t sub_count(const GpsRenderer *r) { return (uint64_t)r->W * r->H * r->spp_side * r->spp_side; }
+
+// Per-launch HIP events on the renderer's stream when profiling is on.
+struct Prof {
+    GpsRenderer *r;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Prof(GpsRenderer *r_, const char *n_) : r(r_), name(n_) {
+        if (r->profiling) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, r->stream);
+        }
+    }
+    ~Prof() {
+        if (r->profiling) {
+            (void)hipEventRecord(b, r->stream);
+            r->pending.push_back({name, {a, b}});
+        }
+    }
+};
+
+void drain_profile(GpsRenderer *r) {
+    for (auto &p : r->pending) {
+        float ms = 0.0f;
+        (void)hipEventSynchronize(p.second.second);
+        (void)hipEventElapsedTime(&ms, p.second.first, p.second.second);
+        auto &t = r->times[p.first];
+        t.first += 1;
+        t.second += ms;
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    r->pending.clear();
+}
+
+int alloc_frame(GpsRenderer *r) {
+    dfree(r->accum);
+    dfree(r->depth);
+    uint64_t X = (uint64_t)r->W * r->H;
+    HIP_TRY(r, hipMalloc(&r->accum, X * sizeof(float4)));
+    launch_accum_clear(r->stream, r->accum, X);
+    r->accumulated = 0;
+    return 0;
+}
+
+// depth|id buffers and per-ensemble offsets sized for a batch of ensembles
+int alloc_batch(GpsRenderer *r) {
+    dfree(r->depth);
+    dfree(r->offsets);
+    dfree(r->bs.status);
+    uint64_t S = sub_count(r);
+    const char *env = std::getenv("GPS_HIP_BATCH_BYTES");
+    uint64_t budget = env ? std::strtoull(env, nullptr, 10) : (8ull << 30);
+    uint64_t per = 8 * S + 4 * (r->n ? r->n : 1);
+    uint64_t cap = budget / per;
+    if (cap < 1) cap = 1;
+    if (cap > MAX_BATCH) cap = MAX_BATCH;
+    r->batch_cap = (uint32_t)cap;
+    HIP_TRY(r, hipMalloc(&r->depth, cap * S * sizeof(uint64_t)));
+    launch_fill_u64(r->stream, r->depth, cap * S, EMPTY64);
+    HIP_TRY(r, hipMalloc(&r->offsets, cap * (r->n ? r->n : 1) * sizeof(uint32_t)));
+    r->tiles = (uint32_t)((r->n + SCAN_TILE - 1) / SCAN_TILE);
+    if (r->tiles == 0) r->tiles = 1;
+    HIP_TRY(r, hipMalloc(&r->bs.status, (uint64_t)MAX_BATCH * r->tiles * sizeof(uint64_t)));
+    HIP_TRY(r, hipMemsetAsync(r->bs.status, 0, (uint64_t)MAX_BATCH * r->tiles * sizeof(uint64_t), r->stream));
+    r->depth_dirty = false;
+    r->have_last = false;
+    return 0;
+}
+
+void pack_frame(const GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p, GpsFrame &f) {
+    std::memset(&f, 0, sizeof f);
+    for (int row = 0; row < 3; row++) {
+        for (int k = 0; k < 3; k++) f.rot[row][k] = (float)cam->view_rot[3 * row + k];
+        f.rot[row][3] = 0.0f;
+    }
+    for (int k = 0; k < 3; k++) f.pos[k] = (float)cam->view_pos[k];
+    f.pos[3] = (float)cam->low_pass;
+    double cx = cam->cx, cy = cam->cy;
+    if (cx == 0.0 && cy == 0.0) {
+        cx = 0.5 * (double)r->W;
+        cy = 0.5 * (double)r->H;
+    }
+    f.intr[0] = (float)cam->focal_x;
+    f.intr[1] = (float)cam->focal_y;
+    f.intr[2] = (float)cx;
+    f.intr[3] = (float)cy;
+    f.W = r->W;
+    f.H = r->H;
+    f.spp_side = r->spp_side;
+    f.N = (uint32_t)r->n;
+    f.cap = p->max_points_per_splat > 1u ? p->max_points_per_splat : 1u;
+    f.sh_stride = r->sh_stride;
+    f.exact_radius = p->exact_radius ? 1u : 0u;
+    f.sh_degree = p->sh_degree < r->sh_max_degree ? p->sh_degree : r->sh_max_degree;
+    f.method = p->method;
+    f.calib = p->pbvr.calibration;
+    uint32_t flags = 0;
+    if (p->pbvr.radial_correction) flags |= FLAG_RADIAL;
+    if (p->pbvr.centre_depth) flags |= FLAG_CENTRE;
+    if (p->pbvr.low_pass_jitter) flags |= FLAG_JITTER;
+    f.flags = flags;
+    f.density = p->density_scale;
+    f.near_z = p->near;
+    f.base_k = p->pbvr.base_k;
+    f.ref_len = p->pbvr.reference_pixel_length;
+    double d[3] = {r->centroid[0] - cam->view_pos[0], r->centroid[1] - cam->view_pos[1],
+                   r->centroid[2] - cam->view_pos[2]};
+    f.object_depth = (float)(cam->view_rot[6] * d[0] + cam->view_rot[7] * d[1] + cam->view_rot[8] * d[2]);
+    for (int k = 0; k < 3; k++) f.bg[k] = p->background[k];
+}
+
+int check_params(GpsRenderer *r, const GpsRenderParams *p) {
+    uint32_t s = p->spp_side < 1 ? 1 : (p->spp_side > 4 ? 4 : p->spp_side);
+    if (s != r->spp_side) return fail(r, "params.spp_side differs from the renderer's; call resize() first");
+    if (p->method > 4) return fail(r, "unknown method");
+    return 0;
+}
+
+int grid_for_particles(GpsRenderer *r) {
+    static int cus = 0;
+    if (!cus) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, r->device) == hipSuccess) cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    const char *env = std::getenv("GPS_HIP_PARTICLE_WAVES");
+    int per_cu = env ? std::atoi(env) : 8;
+    if (per_cu < 1) per_cu = 1;
+    return cus * per_cu;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+void gps_render_params_default(GpsRenderParams *o) {
+    std::memset(o, 0, sizeof *o);
+    o->spp_side = 2;
+    o->density_scale = 1.0f;
+    o->max_points_per_splat = 1u << 16;
+    o->near = 0.05f;
+    o->background[0] = 0.05f;
+    o->background[1] = 0.05f;
+    o->background[2] = 0.08f;
+    o->exposure = 1.0f;
+    o->srgb_output = 1;
+    o->exact_radius = 0;
+    o->sh_degree = 3;
+    o->path = GPS_PATH_PARTICLE;
+    o->method = GPS_METHOD_GPS;
+    o->pbvr.calibration = GPS_CAL_PER_SPLAT_FOOTPRINT;
+    o->pbvr.radial_correction = 0;
+    o->pbvr.centre_depth = 0;
+    o->pbvr.low_pass_jitter = 1;
+    o->pbvr.base_k = 512.0f;
+    o->pbvr.reference_pixel_length = 0.01f;
+}
+
+const char *gps_renderer_last_error(const GpsRenderer *r) { return r ? r->err.c_str() : g_create_error.c_str(); }
+
+int gps_renderer_create(uint32_t width, uint32_t height, uint32_t spp_side, int device, GpsRenderer **out) {
+    *out = nullptr;
+    if (width == 0 || height == 0) {
+        g_create_error = "width/height must be > 0";
+        return -1;
+    }
+    GpsRenderer *r = new GpsRenderer();
+    r->device = device;
+    r->W = width;
+    r->H = height;
+    r->spp_side = spp_side < 1 ? 1 : (spp_side > 4 ? 4 : spp_side);
+    auto bail = [&](const std::string &m) {
+        g_create_error = m;
+        gps_renderer_destroy(r);
+        return -1;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail("hipSetDevice failed (no GPU?)");
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) return bail("hipStreamCreate failed");
+    std::vector<float> lut(LUT_O * LUT_V);
+    make_lut(lut.data());
+    if (hipMalloc(&r->lut, lut.size() * sizeof(float)) != hipSuccess) return bail("hipMalloc(lut) failed");
+    if (hipMemcpy(r->lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return bail("lut upload failed");
+    if (hipMalloc(&r->stats, ST_WORDS * sizeof(uint64_t)) != hipSuccess) return bail("hipMalloc(stats) failed");
+    if (hipMemsetAsync(r->stats, 0, ST_WORDS * sizeof(uint64_t), r->stream) != hipSuccess) return bail("stats clear");
+    if (hipMalloc(&r->bs.tickets, MAX_BATCH * sizeof(uint32_t)) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMalloc(&r->bs.totals, MAX_BATCH * sizeof(uint32_t)) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMalloc(&r->bs.skip, MAX_BATCH * sizeof(uint32_t)) != hipSuccess) return bail("hipMalloc failed");
+    if (hipMemsetAsync(r->bs.tickets, 0, MAX_BATCH * sizeof(uint32_t), r->stream) != hipSuccess) return bail("memset");
+    if (hipMemsetAsync(r->bs.totals, 0, MAX_BATCH * sizeof(uint32_t), r->stream) != hipSuccess) return bail("memset");
+    if (hipMemsetAsync(r->bs.skip, 0, MAX_BATCH * sizeof(uint32_t), r->stream) != hipSuccess) return bail("memset");
+    if (alloc_frame(r) != 0) return bail(r->err);
+    if (gps_renderer_set_gaussians(r, nullptr, 0, nullptr, 0) != 0) return bail(r->err);
+    r->sorted = sorted_create();
+    if (hipStreamSynchronize(r->stream) != hipSuccess) return bail("stream sync failed");
+    *out = r;
+    return 0;
+}
+
+void gps_renderer_destroy(GpsRenderer *r) {
+    if (!r) return;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    drain_profile(r);
+    if (r->sorted) sorted_destroy(r->sorted);
+    dfree(r->splats);
+    dfree(r->sh);
+    dfree(r->lut);
+    dfree(r->view);
+    dfree(r->cinfo);
+    dfree(r->colour);
+    dfree(r->offsets);
+    dfree(r->depth);
+    dfree(r->bs.tickets);
+    dfree(r->bs.status);
+    dfree(r->bs.totals);
+    dfree(r->bs.skip);
+    dfree(r->accum);
+    dfree(r->stats);
+    dfree(r->composite_buf);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+}
+
+int gps_renderer_set_gaussians(GpsRenderer *r, const GpsGaussian *g, uint64_t n, const float *sh_rest,
+                               uint32_t degree) {
+    if (n > 0 && !g) return fail(r, "null Gaussian pointer");
+    if (n >= MAX_SPLATS) return fail(r, "too many Gaussians for one dispatch");
+    uint32_t deg = 0, stride = 0;
+    if (sh_rest && degree >= 1 && n > 0) {
+        deg = degree > 3 ? 3 : degree;
+        stride = (deg + 1) * (deg + 1) - 1;
+    }
+    HIP_TRY(r, hipSetDevice(r->device));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    // allocate the new scene first so that a failure leaves the old one in place (gps.rs:682)
+    float4 *splats = nullptr, *view = nullptr, *colour = nullptr;
+    float *sh = nullptr;
+    uint2 *cinfo = nullptr;
+    uint64_t nn = n ? n : 1;
+    auto cleanup = [&]() {
+        dfree(splats);
+        dfree(view);
+        dfree(colour);
+        dfree(sh);
+        dfree(cinfo);
+    };
+    if (hipMalloc(&splats, nn * 64) != hipSuccess || hipMalloc(&view, nn * VIEW_VEC4 * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&colour, nn * sizeof(float4)) != hipSuccess || hipMalloc(&cinfo, nn * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&sh, (stride ? nn * 3 * stride : 16) * sizeof(float)) != hipSuccess) {
+        cleanup();
+        return fail(r, "out of device memory for the scene");
+    }
+    if (n) {
+        if (hipMemcpyAsync(splats, g, n * 64, hipMemcpyHostToDevice, r->stream) != hipSuccess ||
+            (stride && hipMemcpyAsync(sh, sh_rest, n * 3 * stride * sizeof(float), hipMemcpyHostToDevice,
+                                      r->stream) != hipSuccess)) {
+            cleanup();
+            return fail(r, "scene upload failed");
+        }
+    }
+    dfree(r->splats);
+    dfree(r->view);
+    dfree(r->colour);
+    dfree(r->sh);
+    dfree(r->cinfo);
+    r->splats = splats;
+    r->view = view;
+    r->colour = colour;
+    r->sh = sh;
+    r->cinfo = cinfo;
+    r->n = n;
+    r->sh_max_degree = deg;
+    r->sh_stride = stride;
+    double c[3] = {0, 0, 0};
+    for (uint64_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) c[k] = c[k] + (double)g[i].pos_opacity[k];
+    for (int k = 0; k < 3; k++) r->centroid[k] = n ? c[k] / (double)n : 0.0;
+    if (alloc_batch(r) != 0) return -1;
+    if (gps_renderer_reset_accum(r) != 0) return -1;
+    HIP_TRY(r, hipStreamSynchronize(r->stream));   // the caller may free the host arrays after return
+    return 0;
+}
+
+int gps_renderer_resize(GpsRenderer *r, uint32_t width, uint32_t height, uint32_t spp_side) {
+    spp_side = spp_side < 1 ? 1 : (spp_side > 4 ? 4 : spp_side);
+    if (width == r->W && height == r->H && spp_side == r->spp_side) return 0;
+    if (width == 0 || height == 0) return fail(r, "width/height must be > 0");
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    r->W = width;
+    r->H = height;
+    r->spp_side = spp_side;
+    if (alloc_frame(r) != 0) return -1;
+    if (alloc_batch(r) != 0) return -1;
+    return gps_renderer_reset_accum(r);
+}
+
+int gps_renderer_reset_accum(GpsRenderer *r) {
+    launch_accum_clear(r->stream, r->accum, (uint64_t)r->W * r->H);
+    launch_stats_clear(r->stream, r->stats, ST_SKIPPED, 1);
+    r->accumulated = 0;
+    HIP_TRY(r, hipGetLastError());
+    return 0;
+}
+
+int gps_renderer_reset_stats(GpsRenderer *r) {
+    launch_stats_clear(r->stream, r->stats, 0, ST_WORDS);
+    HIP_TRY(r, hipGetLastError());
+    return 0;
+}
+
+int gps_renderer_render_ensembles(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p,
+                                  uint32_t first_seed, uint32_t count) {
+    if (count == 0) return 0;
+    if (check_params(r, p) != 0) return -1;
+    HIP_TRY(r, hipSetDevice(r->device));
+    GpsFrame f;
+    pack_frame(r, cam, p, f);
+    const uint64_t S = sub_count(r);
+    const bool use_particles = (p->path == GPS_PATH_PARTICLE || p->method != GPS_METHOD_GPS) && r->n > 0;
+    if (r->n > 0) {
+        Prof pr(r, "view");
+        launch_view(r->stream, f, r->splats, r->sh, r->view, r->cinfo, r->colour);
+    }
+    const bool keep = (r->debug & GPS_DEBUG_KEEP_BUFFERS) != 0;
+    const int grid = grid_for_particles(r);
+    uint32_t done = 0;
+    while (done < count) {
+        uint32_t B = count - done;
+        if (B > r->batch_cap) B = r->batch_cap;
+        if (keep) B = 1;
+        const uint32_t seed0 = first_seed + done;
+        if (r->depth_dirty) {
+            Prof pr(r, "clear");
+            launch_fill_u64(r->stream, r->depth, (uint64_t)B * S, EMPTY64);
+            r->depth_dirty = false;
+        }
+        if (use_particles) {
+            {
+                Prof pr(r, "count_scan");
+                launch_count_scan(r->stream, f, r->cinfo, seed0, B, r->offsets, r->bs, r->tiles, r->epoch, r->stats);
+            }
+            r->epoch = (r->epoch + 1u) & ((1u << 26) - 1u);
+            if (r->epoch == 0) r->epoch = 1;
+            {
+                Prof pr(r, "particles");
+                launch_particles(r->stream, f, r->view, r->offsets, r->bs, B, seed0, r->depth, S, r->lut, r->stats,
+                                 grid);
+            }
+        } else if (r->n > 0) {
+            Prof pr(r, "splat_pass");
+            for (uint32_t b = 0; b < B; b++)
+                launch_splat_pass(r->stream, f, r->view, r->cinfo, seed0 + b, r->depth + (uint64_t)b * S, r->lut,
+                                  r->stats);
+        }
+        {
+            Prof pr(r, "resolve");
+            launch_resolve(r->stream, f, r->depth, S, use_particles ? r->bs.skip : nullptr, B, r->colour, r->accum,
+                           keep ? 0 : 1, r->stats);
+        }
+        if (keep) r->depth_dirty = true;
+        r->have_last = keep && use_particles;
+        done += B;
+    }
+    HIP_TRY(r, hipGetLastError());
+    r->accumulated += count;
+    return 0;
+}
+
+int gps_renderer_render_sorted(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p, uint32_t as_ensembles) {
+    if (r->n == 0) return fail(r, "empty scene");
+    if (check_params(r, p) != 0) return -1;
+    GpsRenderParams q = *p;
+    q.method = GPS_METHOD_GPS;
+    GpsFrame f;
+    pack_frame(r, cam, &q, f);
+    if (sorted_render(r->sorted, r->stream, f, r->splats, r->sh, r->accum, r->stats, 0, r->err,
+                      r->profiling ? &r->pending : nullptr) != 0)
+        return -1;
+    r->accumulated = as_ensembles > 1 ? as_ensembles : 1;
+    return 0;
+}
+
+int gps_renderer_probe_demand(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p) {
+    if (r->n == 0) return fail(r, "empty scene");
+    GpsFrame f;
+    pack_frame(r, cam, p, f);
+    f.method = GPS_METHOD_GPS;
+    return sorted_render(r->sorted, r->stream, f, r->splats, r->sh, r->accum, r->stats, 1, r->err,
+                         r->profiling ? &r->pending : nullptr);
+}
+
+int gps_renderer_read_sorted(GpsRenderer *r, uint32_t *keys, uint32_t *vals) {
+    return sorted_read(r->sorted, r->stream, (uint32_t)r->n, keys, vals, r->err);
+}
+
+int gps_renderer_composite(GpsRenderer *r, const GpsRenderParams *p, uint32_t out_w, uint32_t out_h, uint8_t *rgba8) {
+    if (out_w == 0 || out_h == 0) return fail(r, "empty composite target");
+    uint64_t px = (uint64_t)out_w * out_h;
+    if (px > r->composite_cap) {
+        dfree(r->composite_buf);
+        HIP_TRY(r, hipMalloc(&r->composite_buf, px * 4));
+        r->composite_cap = px;
+    }
+    {
+        Prof pr(r, "composite");
+        launch_composite(r->stream, r->accum, r->W, r->H, p->exposure, p->srgb_output ? 1u : 0u, out_w, out_h,
+                         r->composite_buf);
+    }
+    HIP_TRY(r, hipMemcpyAsync(rgba8, r->composite_buf, px * 4, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    return 0;
+}
+
+int gps_renderer_read_accum(GpsRenderer *r, float *out) {
+    HIP_TRY(r, hipMemcpyAsync(out, r->accum, (uint64_t)r->W * r->H * sizeof(float4), hipMemcpyDeviceToHost,
+                              r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    return 0;
+}
+
+int gps_renderer_read_stats(GpsRenderer *r, GpsStats *out) {
+    uint64_t w[ST_WORDS];
+    HIP_TRY(r, hipMemcpyAsync(w, r->stats, sizeof w, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    std::memset(out, 0, sizeof *out);
+    out->points = w[ST_POINTS];
+    out->truncated_splats = w[ST_TRUNC];
+    out->dropped_points = w[ST_DROPPED];
+    out->visible_splats = w[ST_VISIBLE];
+    out->orphan_subpixels = w[ST_ORPHAN];
+    out->candidates = w[ST_CAND];
+    out->undercovered_splats = w[ST_UNDER];
+    out->skipped_ensembles = w[ST_SKIPPED];
+    out->ensembles = w[ST_ENSEMBLES];
+    out->probe_demand = w[ST_PROBE];
+    return 0;
+}
+
+int gps_renderer_wait_idle(GpsRenderer *r) {
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    return 0;
+}
+
+uint32_t gps_renderer_submitted(const GpsRenderer *r) { return r->accumulated; }
+
+uint32_t gps_renderer_accumulated(GpsRenderer *r) {
+    GpsStats s;
+    if (gps_renderer_read_stats(r, &s) != 0) return r->accumulated;
+    uint64_t sk = s.skipped_ensembles;
+    return sk >= r->accumulated ? 0u : r->accumulated - (uint32_t)sk;
+}
+
+void gps_renderer_info(const GpsRenderer *r, uint32_t *w, uint32_t *h, uint32_t *spp, uint64_t *n, uint32_t *deg) {
+    if (w) *w = r->W;
+    if (h) *h = r->H;
+    if (spp) *spp = r->spp_side;
+    if (n) *n = r->n;
+    if (deg) *deg = r->sh_max_degree;
+}
+
+void gps_estimate_memory(uint32_t width, uint32_t height, uint32_t spp_side, uint64_t n_splats, uint64_t *dw,
+                         uint64_t *ac, uint64_t *sp) {
+    uint64_t sub = (uint64_t)width * height * (uint64_t)(spp_side * spp_side);
+    if (dw) *dw = 8 * sub;
+    if (ac) *ac = 16ull * width * height;
+    if (sp) *sp = 164ull * n_splats;
+}
+
+void gps_radius_lut(float *out) { make_lut(out); }
+
+int gps_renderer_set_debug(GpsRenderer *r, uint32_t flags) {
+    r->debug = flags;
+    return 0;
+}
+
+int gps_renderer_read_counts(GpsRenderer *r, uint32_t *counts, uint32_t *offsets, uint32_t *total, uint32_t *skipped) {
+    if (!r->have_last) return fail(r, "no kept ensemble: enable GPS_DEBUG_KEEP_BUFFERS and render the particle path");
+    uint64_t n = r->n;
+    std::vector<uint32_t> off(n ? n : 1);
+    uint32_t t = 0, sk = 0;
+    HIP_TRY(r, hipMemcpyAsync(off.data(), r->offsets, n * 4, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipMemcpyAsync(&t, r->bs.totals, 4, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipMemcpyAsync(&sk, r->bs.skip, 4, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    if (offsets) std::memcpy(offsets, off.data(), n * 4);
+    if (counts && !sk)
+        for (uint64_t i = 0; i < n; i++) counts[i] = (i + 1 < n ? off[i + 1] : t) - off[i];
+    if (total) *total = t;
+    if (skipped) *skipped = sk;
+    return 0;
+}
+
+int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out) {
+    if (!(r->debug & GPS_DEBUG_KEEP_BUFFERS)) return fail(r, "enable GPS_DEBUG_KEEP_BUFFERS first");
+    HIP_TRY(r, hipMemcpyAsync(out, r->depth, sub_count(r) * 8, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    return 0;
+}
+
+int gps_renderer_profile(GpsRenderer *r, int enable) {
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    drain_profile(r);
+    r->profiling = enable != 0;
+    if (enable) r->times.clear();
+    return 0;
+}
+
+int gps_renderer_profile_read(GpsRenderer *r, GpsKernelTime *out, int max) {
+    if (hipStreamSynchronize(r->stream) != hipSuccess) return -1;
+    drain_profile(r);
+    int i = 0;
+    for (auto &t : r->times) {
+        if (i < max && out) {
+            std::memset(&out[i], 0, sizeof out[i]);
+            std::snprintf(out[i].name, sizeof out[i].name, "%s", t.first.c_str());
+            out[i].launches = t.second.first;
+            out[i].total_ms = t.second.second;
+        }
+        i++;
+    }
+    return i;
+}
+
+// -------------------------------------------------------------- phase-0 unit entry points
+int gps_phase0_atomic(int device, uint32_t seed, uint32_t pixels, uint32_t samples, uint32_t *depth, uint32_t *color) {
+    if (pixels == 0) return -1;
+    if (hipSetDevice(device) != hipSuccess) return -1;
+    uint64_t *buf = nullptr;
+    if (hipMalloc(&buf, (uint64_t)pixels * 8) != hipSuccess) return -1;
+    launch_fill_u64(nullptr, buf, pixels, EMPTY64);
+    launch_phase0(nullptr, seed, pixels, samples, buf);
+    std::vector<uint64_t> h(pixels);
+    int rc = hipMemcpy(h.data(), buf, (uint64_t)pixels * 8, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+    (void)hipFree(buf);
+    for (uint32_t i = 0; i < pixels && rc == 0; i++) {
+        depth[i] = (uint32_t)(h[i] >> 32);
+        color[i] = (uint32_t)h[i];
+    }
+    return rc;
+}
+
+int gps_scan_exclusive_u32(int device, const uint32_t *in, uint32_t *out, uint64_t n) {
+    if (n == 0) return 0;
+    if (hipSetDevice(device) != hipSuccess) return -1;
+    uint32_t tiles = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
+    uint32_t *din = nullptr, *dout = nullptr, *ticket = nullptr;
+    uint64_t *status = nullptr;
+    int rc = 0;
+    if (hipMalloc(&din, n * 4) != hipSuccess || hipMalloc(&dout, n * 4) != hipSuccess ||
+        hipMalloc(&ticket, 4) != hipSuccess || hipMalloc(&status, (uint64_t)tiles * 8) != hipSuccess)
+        rc = -1;
+    if (rc == 0 && (hipMemcpy(din, in, n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemset(ticket, 0, 4) != hipSuccess || hipMemset(status, 0, (uint64_t)tiles * 8) != hipSuccess))
+        rc = -1;
+    if (rc == 0) {
+        launch_scan_u32(nullptr, din, dout, n, ticket, status, tiles, 1u);
+        if (hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    }
+    dfree(din);
+    dfree(dout);
+    dfree(ticket);
+    dfree(status);
+    return rc;
+}
+
+int gps_math_eval(int device, int fn, const float *x, const float *y, float *out, uint64_t n) {
+    if (n == 0) return 0;
+    if (hipSetDevice(device) != hipSuccess) return -1;
+    float *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    int rc = 0;
+    if (hipMalloc(&dx, n * 4) != hipSuccess || hipMalloc(&dy, n * 4) != hipSuccess ||
+        hipMalloc(&dout, n * 4) != hipSuccess)
+        rc = -1;
+    if (rc == 0 && hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    if (rc == 0 && y && hipMemcpy(dy, y, n * 4, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    if (rc == 0) {
+        launch_math(nullptr, fn, dx, dy, dout, n);
+        if (hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    }
+    dfree(dx);
+    dfree(dy);
+    dfree(dout);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,689 @@
+// Depth-sorted alpha path (crates/gps-render/src/sorted.rs; shaders/gps_splat.wgsl:736-786,
+// gps_sort.wgsl, gps_sorted_draw.wgsl).
+//
+// CDNA has no rasterizer, so the reference's instanced-quad draw with hardware SrcAlpha /
+// OneMinusSrcAlpha blending becomes compute:
+//   k_sp_project   per splat: cull, GPS demand probe, quad axes, SH colour, 24-bit far->near key
+//   radix sort     stable LSD, 8-bit digits: per-tile digit histograms (digit-major), a decoupled
+//                  look-back scan over them, then a stable scatter ranked with wave64 ballots
+//   binning        each visible splat (in sorted order) emits one (tile, rank) pair per 16x16 tile its
+//                  quad touches; a stable sort by tile keeps the back-to-front order inside each tile
+//   k_sp_blend     one workgroup per tile: back-to-front blend of the tile's splats at pixel centres in
+//                  f32, written to the accumulation buffer with count 1 (sp_resolve)
+#include <string>
+#include <vector>
+
+#include "gps_devmath.h"
+#include "gps_internal.h"
+
+namespace {
+
+using gpsm::clamp;
+using gpsm::f2u;
+using gpsm::fmax;
+using gpsm::fmin;
+
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+constexpr int BIN = 16;                      // screen tile side in pixels
+constexpr uint32_t SP_KEY_INVALID = 0xFFFFFFu;
+constexpr float SP_MIN_ALPHA = 0.00392156862745098f;
+constexpr float PI_F = 3.14159265358979f;
+constexpr float DILOG_ONE = 1.64493406684823f;
+
+// ------------------------------------------------------------------ arithmetic shared with gps_kernels.hip
+GPS_DEV float dilog_series(float x) {
+    float term = x, sum = x;
+    for (uint32_t k = 2u; k < 32u; k++) {
+        term = term * x;
+        sum = sum + term / (float)(k * k);
+    }
+    return sum;
+}
+
+GPS_DEV float dilog(float x) {
+    if (x >= 1.0f) return DILOG_ONE;
+    if (x <= 0.0f) return 0.0f;
+    if (x <= 0.5f) return dilog_series(x);
+    return DILOG_ONE - gpsm::log(x) * gpsm::log(1.0f - x) - dilog_series(1.0f - x);
+}
+
+GPS_DEV float dot3(const float *r, float x, float y, float z) { return r[0] * x + r[1] * y + r[2] * z; }
+
+struct Proj {
+    bool ok;
+    float mx, my, a, b, c, depth;
+};
+
+// gps_splat.wgsl:165-206
+GPS_DEV Proj project(const GpsFrame &f, float4 g0, float4 g1, float4 g2) {
+    Proj pr;
+    pr.ok = false;
+    pr.mx = pr.my = pr.a = pr.b = pr.c = pr.depth = 0.0f;
+    float px = g0.x - f.pos[0], py = g0.y - f.pos[1], pz = g0.z - f.pos[2];
+    float camx = dot3(f.rot[0], px, py, pz), camy = dot3(f.rot[1], px, py, pz), camz = dot3(f.rot[2], px, py, pz);
+    if (camz <= fmax(f.near_z, 1e-4f)) return pr;
+    const float lim = 5.2f;
+    float tx = clamp(camx / camz, -lim, lim) * camz;
+    float ty = clamp(camy / camz, -lim, lim) * camz;
+    float fx = f.intr[0], fy = f.intr[1];
+    float iz = 1.0f / camz;
+    float sx = fx * iz, sxz = -fx * tx * iz * iz;
+    float sy = fy * iz, syz = -fy * ty * iz * iz;
+    float m0[3], m1[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        m0[j] = f.rot[0][j] * sx + f.rot[2][j] * sxz;
+        m1[j] = f.rot[1][j] * sy + f.rot[2][j] * syz;
+    }
+    float ql = gpsm::sqrt(g2.x * g2.x + g2.y * g2.y + g2.z * g2.z + g2.w * g2.w);
+    float w = g2.x / ql, x = g2.y / ql, y = g2.z / ql, z = g2.w / ql;
+    float c0x = 1.0f - 2.0f * (y * y + z * z), c0y = 2.0f * (x * y + w * z), c0z = 2.0f * (x * z - w * y);
+    float c1x = 2.0f * (x * y - w * z), c1y = 1.0f - 2.0f * (x * x + z * z), c1z = 2.0f * (y * z + w * x);
+    float c2x = 2.0f * (x * z + w * y), c2y = 2.0f * (y * z - w * x), c2z = 1.0f - 2.0f * (x * x + y * y);
+    float s2x = gpsm::exp(2.0f * g1.x), s2y = gpsm::exp(2.0f * g1.y), s2z = gpsm::exp(2.0f * g1.z);
+    float ux = dot3(m0, c0x, c0y, c0z), uy = dot3(m0, c1x, c1y, c1z), uz = dot3(m0, c2x, c2y, c2z);
+    float vx = dot3(m1, c0x, c0y, c0z), vy = dot3(m1, c1x, c1y, c1z), vz = dot3(m1, c2x, c2y, c2z);
+    float a = s2x * (ux * ux) + s2y * (uy * uy) + s2z * (uz * uz) + f.pos[3];
+    float b = s2x * (ux * vx) + s2y * (uy * vy) + s2z * (uz * vz);
+    float c = s2x * (vx * vx) + s2y * (vy * vy) + s2z * (vz * vz) + f.pos[3];
+    float det = a * c - b * b;
+    if (!(det > 0.0f)) return pr;
+    pr.ok = true;
+    pr.mx = fx * camx / camz + f.intr[2];
+    pr.my = fy * camy / camz + f.intr[3];
+    pr.a = a;
+    pr.b = b;
+    pr.c = c;
+    pr.depth = camz;
+    return pr;
+}
+
+// gps_splat.wgsl:210-251
+GPS_DEV float4 splat_colour(const GpsFrame &f, uint32_t id, float4 g0, float4 g3, const float *__restrict__ sh) {
+    float res[3] = {g3.x, g3.y, g3.z};
+    uint32_t deg = f.sh_degree;
+    if (deg == 0u) return make_float4(fmax(res[0], 0.0f), fmax(res[1], 0.0f), fmax(res[2], 0.0f), 0.0f);
+    float px = g0.x - f.pos[0], py = g0.y - f.pos[1], pz = g0.z - f.pos[2];
+    float len = gpsm::sqrt(px * px + py * py + pz * pz);
+    float x = px / len, y = py / len, z = pz / len;
+    uint32_t r = f.sh_stride;
+    uint32_t o = id * 3u * r;
+    float out[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        const float *S = sh + o + (uint32_t)ch * r;
+        float v = res[ch];
+        v = v - 0.4886025119029199f * y * S[0] + 0.4886025119029199f * z * S[1] - 0.4886025119029199f * x * S[2];
+        if (deg >= 2u) {
+            float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            v = v + 1.0925484305920792f * xy * S[3] - 1.0925484305920792f * yz * S[4] +
+                0.31539156525252005f * (2.0f * zz - xx - yy) * S[5] - 1.0925484305920792f * xz * S[6] +
+                0.5462742152960396f * (xx - yy) * S[7];
+            if (deg >= 3u) {
+                v = v - 0.5900435899266435f * y * (3.0f * xx - yy) * S[8] + 2.890611442640554f * xy * z * S[9] -
+                    0.4570457994644658f * y * (4.0f * zz - xx - yy) * S[10] +
+                    0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * S[11] -
+                    0.4570457994644658f * x * (4.0f * zz - xx - yy) * S[12] +
+                    1.445305721320277f * z * (xx - yy) * S[13] - 0.5900435899266435f * x * (xx - 3.0f * yy) * S[14];
+            }
+        }
+        out[ch] = fmax(v, 0.0f);
+    }
+    return make_float4(out[0], out[1], out[2], 0.0f);
+}
+
+// ------------------------------------------------------------------ k_sp_project (gps_splat.wgsl:736-777)
+__global__ __launch_bounds__(256) void k_sp_project(GpsFrame f, const float4 *__restrict__ splats,
+                                                    const float *__restrict__ sh, float4 *__restrict__ sp,
+                                                    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                    uint32_t *__restrict__ visible, uint64_t *__restrict__ stats,
+                                                    int probe_only) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t demand = 0, vis = 0;
+    if (i < f.N) {
+        if (!probe_only) {
+            keys[i] = SP_KEY_INVALID;
+            vals[i] = i;
+        }
+        float4 g0 = splats[4u * i], g1 = splats[4u * i + 1u], g2 = splats[4u * i + 2u];
+        Proj pr = project(f, g0, g1, g2);
+        float o = clamp(g0.w, 0.0f, 1.0f);
+        float ex = 5.9f * gpsm::sqrt(pr.a), ey = 5.9f * gpsm::sqrt(pr.c);
+        bool off = pr.mx + ex < 0.0f || pr.mx - ex >= (float)f.W || pr.my + ey < 0.0f || pr.my - ey >= (float)f.H;
+        if (pr.ok && o > 0.0f && !off) {
+            uint32_t spp = f.spp_side * f.spp_side;
+            float det0 = pr.a * pr.c - pr.b * pr.b;
+            float en = f.density * (float)spp * 2.0f * PI_F * gpsm::sqrt(det0) * dilog(o);
+            if (en > 0.0f) demand = f2u(fmin(en, 4.0e9f));
+            if (!probe_only && !(o < SP_MIN_ALPHA)) {
+                float k = gpsm::sqrt(2.0f * gpsm::log(o / SP_MIN_ALPHA));
+                float kx = k * gpsm::sqrt(pr.a), ky = k * gpsm::sqrt(pr.c);
+                if (!(pr.mx + kx < 0.0f || pr.mx - kx >= (float)f.W || pr.my + ky < 0.0f ||
+                      pr.my - ky >= (float)f.H)) {
+                    float tr2 = 0.5f * (pr.a + pr.c);
+                    float disc = gpsm::sqrt(fmax(tr2 * tr2 - (pr.a * pr.c - pr.b * pr.b), 0.0f));
+                    float l1 = tr2 + disc, l2 = fmax(tr2 - disc, 1e-12f);
+                    float v1x = 1.0f, v1y = 0.0f;
+                    float absb = pr.b < 0.0f ? -pr.b : pr.b;
+                    if (absb > 1e-12f) {
+                        float ax = l1 - pr.c, ay = pr.b;
+                        float len = gpsm::sqrt(ax * ax + ay * ay);
+                        v1x = ax / len;
+                        v1y = ay / len;
+                    } else if (pr.a < pr.c) {
+                        v1x = 0.0f;
+                        v1y = 1.0f;
+                    }
+                    float v2x = -v1y, v2y = v1x;
+                    float4 col = splat_colour(f, i, g0, splats[4u * i + 3u], sh);
+                    float s1 = gpsm::sqrt(l1), s2 = gpsm::sqrt(l2);
+                    sp[3u * i] = make_float4(pr.mx, pr.my, o, k);
+                    sp[3u * i + 1u] = make_float4(v1x * s1, v1y * s1, v2x * s2, v2y * s2);
+                    sp[3u * i + 2u] = make_float4(col.x, col.y, col.z, 0.0f);
+                    keys[i] = 0xFFFFFEu - (gpsm::bits(pr.depth) >> 8u);
+                    vis = 1;
+                }
+            }
+        }
+    }
+    // one atomic per wave for each counter
+    uint64_t d64 = demand;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        d64 += __shfl_xor(d64, off, 64);
+        vis += __shfl_xor(vis, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (d64) atomicAdd((unsigned long long *)&stats[ST_PROBE], (unsigned long long)d64);
+        if (vis && !probe_only) atomicAdd(visible, vis);
+    }
+}
+
+// ------------------------------------------------------------------ radix sort
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
+                                                      uint32_t ntiles, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        uint32_t i = tile * RS_TILE + j * RS_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
+}
+
+// Stable scatter.  Wave w owns items [tile*TILE + w*64*ITEMS, +64*ITEMS) in (j, lane) order, so ranks
+// follow the input order: earlier waves, then earlier j, then lower lanes.
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint32_t *__restrict__ kin,
+                                                         const uint32_t *__restrict__ vin,
+                                                         uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                         uint32_t n, uint32_t shift, uint32_t ntiles,
+                                                         const uint32_t *__restrict__ hist) {
+    constexpr int NW = RS_BLOCK / 64;
+    __shared__ uint32_t wh[NW][257];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t d = lane; d < 257; d += 64) wh[w][d] = 0;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t key[RS_ITEMS], val[RS_ITEMS], dig[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        uint32_t i = tile * RS_TILE + w * (64 * RS_ITEMS) + j * 64 + lane;
+        bool valid = i < n;
+        key[j] = valid ? kin[i] : 0u;
+        val[j] = valid && vin ? vin[i] : 0u;
+        uint32_t d = valid ? ((key[j] >> shift) & 255u) : 256u;
+        dig[j] = d;
+        uint64_t peers = ~0ull;
+#pragma unroll
+        for (int b = 0; b < 9; b++) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t before = wh[w][d];
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = before + (uint32_t)__popcll(peers & lt);
+        if ((peers & lt) == 0ull) wh[w][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        uint32_t run = 0;
+        for (int ww = 0; ww < NW; ww++) {
+            uint32_t c = wh[ww][threadIdx.x];
+            wh[ww][threadIdx.x] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        uint32_t d = dig[j];
+        if (d < 256u) {
+            uint32_t pos = hist[d * ntiles + tile] + wh[w][d] + rank[j];
+            kout[pos] = key[j];
+            if (vout) vout[pos] = val[j];
+        }
+    }
+}
+
+// exclusive scan of the digit-major histogram: same look-back scheme as k_count_scan
+GPS_DEV uint64_t sat_add(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    const uint64_t m = (1ull << 36) - 1ull;
+    return s > m ? m : s;
+}
+
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t *__restrict__ data, uint32_t n, uint32_t *ticket,
+                                                 uint64_t *status, uint32_t tiles, uint32_t epoch) {
+    constexpr int ITEMS = 8, TILE = 256 * ITEMS;
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_wave[4];
+    __shared__ uint64_t s_prefix;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * TILE + threadIdx.x * ITEMS;
+    uint32_t v[ITEMS];
+    uint64_t tsum = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        v[j] = base + j < n ? data[base + j] : 0u;
+        tsum += v[j];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t inc = tsum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint64_t t = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    uint64_t wave_base = 0, agg = 0;
+    for (int ww = 0; ww < 4; ww++) {
+        if (ww < wave) wave_base += s_wave[ww];
+        agg += s_wave[ww];
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t m = (1ull << 36) - 1ull;
+        const uint64_t tag = (uint64_t)(epoch & ((1u << 26) - 1u)) << 38;
+        uint64_t a = agg > m ? m : agg, excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&status[0], tag | (2ull << 36) | a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&status[tile], tag | (1ull << 36) | a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int32_t t = (int32_t)tile - 1;
+            for (;;) {
+                uint64_t wv;
+                for (;;) {
+                    wv = __hip_atomic_load(&status[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((uint32_t)(wv >> 38) == (epoch & ((1u << 26) - 1u)) && ((wv >> 36) & 3ull) != 0ull) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                excl = sat_add(excl, wv & m);
+                if (((wv >> 36) & 3ull) == 2ull) break;
+                t--;
+            }
+            __hip_atomic_store(&status[tile], tag | (2ull << 36) | sat_add(excl, a), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = excl;
+    }
+    __syncthreads();
+    uint64_t run = s_prefix + wave_base + inc - tsum;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        if (base + j < n) data[base + j] = (uint32_t)run;
+        run += v[j];
+    }
+    if (tile == tiles - 1u && threadIdx.x == 0) *ticket = 0u;
+}
+
+// ------------------------------------------------------------------ binning
+GPS_DEV void quad_tiles(const float4 *sp, uint32_t id, uint32_t W, uint32_t H, uint32_t &tx0, uint32_t &ty0,
+                        uint32_t &tx1, uint32_t &ty1) {
+    float4 a = sp[3u * id], b = sp[3u * id + 1u];
+    float ex = a.w * (fabsf(b.x) + fabsf(b.z)) + 1.0f, ey = a.w * (fabsf(b.y) + fabsf(b.w)) + 1.0f;
+    float x0 = floorf(a.x - ex), x1 = ceilf(a.x + ex), y0 = floorf(a.y - ey), y1 = ceilf(a.y + ey);
+    x0 = clamp(x0, 0.0f, (float)(W - 1));
+    x1 = clamp(x1, 0.0f, (float)(W - 1));
+    y0 = clamp(y0, 0.0f, (float)(H - 1));
+    y1 = clamp(y1, 0.0f, (float)(H - 1));
+    tx0 = (uint32_t)x0 / BIN;
+    tx1 = (uint32_t)x1 / BIN;
+    ty0 = (uint32_t)y0 / BIN;
+    ty1 = (uint32_t)y1 / BIN;
+}
+
+__global__ __launch_bounds__(256) void k_sp_tilecount(const float4 *__restrict__ sp, const uint32_t *__restrict__ vals,
+                                                      const uint32_t *__restrict__ visible, uint32_t n, uint32_t W,
+                                                      uint32_t H, uint32_t *__restrict__ cnt) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t c = 0;
+    if (j < *visible) {
+        uint32_t tx0, ty0, tx1, ty1;
+        quad_tiles(sp, vals[j], W, H, tx0, ty0, tx1, ty1);
+        c = (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u);
+    }
+    cnt[j] = c;
+}
+
+__global__ __launch_bounds__(256) void k_sp_emit(const float4 *__restrict__ sp, const uint32_t *__restrict__ vals,
+                                                 const uint32_t *__restrict__ visible, uint32_t n, uint32_t W,
+                                                 uint32_t H, const uint32_t *__restrict__ off,
+                                                 uint32_t *__restrict__ pkeys, uint32_t *__restrict__ pvals) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || j >= *visible) return;
+    uint32_t tx0, ty0, tx1, ty1;
+    quad_tiles(sp, vals[j], W, H, tx0, ty0, tx1, ty1);
+    const uint32_t tw = (W + BIN - 1) / BIN;
+    uint32_t o = off[j];
+    for (uint32_t ty = ty0; ty <= ty1; ty++)
+        for (uint32_t tx = tx0; tx <= tx1; tx++) {
+            pkeys[o] = ty * tw + tx;
+            pvals[o] = j;
+            o++;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict__ pkeys, uint32_t npairs,
+                                                     uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    uint32_t k = pkeys[p];
+    if (p == 0 || pkeys[p - 1] != k) start[k] = p;
+    if (p + 1 == npairs || pkeys[p + 1] != k) end[k] = p + 1;
+}
+
+// back-to-front blend of a tile's splats (gps_sorted_draw.wgsl:100-124 evaluated at pixel centres)
+__global__ __launch_bounds__(256) void k_sp_blend(GpsFrame f, const float4 *__restrict__ sp,
+                                                  const uint32_t *__restrict__ vals,
+                                                  const uint32_t *__restrict__ pvals,
+                                                  const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
+                                                  float4 *__restrict__ accum) {
+    __shared__ float4 s_a[256], s_b[256], s_c[256];
+    const uint32_t tw = (f.W + BIN - 1) / BIN;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t x = (tile % tw) * BIN + (threadIdx.x % BIN), y = (tile / tw) * BIN + (threadIdx.x / BIN);
+    const bool inside = x < f.W && y < f.H;
+    float cr = f.bg[0], cg = f.bg[1], cb = f.bg[2];
+    const float qx = (float)x + 0.5f, qy = (float)y + 0.5f;
+    const uint32_t s0 = start[tile], s1 = end[tile];
+    for (uint32_t base = s0; base < s1; base += 256) {
+        uint32_t p = base + threadIdx.x;
+        if (p < s1) {
+            uint32_t id = vals[pvals[p]];
+            s_a[threadIdx.x] = sp[3u * id];
+            s_b[threadIdx.x] = sp[3u * id + 1u];
+            s_c[threadIdx.x] = sp[3u * id + 2u];
+        }
+        __syncthreads();
+        uint32_t cnt = min(256u, s1 - base);
+        if (inside) {
+            for (uint32_t q = 0; q < cnt; q++) {
+                float4 a = s_a[q], b = s_b[q];
+                float dx = qx - a.x, dy = qy - a.y;
+                float u = (dx * b.x + dy * b.y) / (b.x * b.x + b.y * b.y);
+                float v = (dx * b.z + dy * b.w) / (b.z * b.z + b.w * b.w);
+                float r2 = u * u + v * v;
+                if (r2 > a.w * a.w) continue;
+                float alpha = fmin(a.z * gpsm::exp(-0.5f * r2), 0.99f);
+                if (alpha < SP_MIN_ALPHA) continue;
+                float4 c = s_c[q];
+                cr = c.x * alpha + cr * (1.0f - alpha);
+                cg = c.y * alpha + cg * (1.0f - alpha);
+                cb = c.z * alpha + cb * (1.0f - alpha);
+            }
+        }
+        __syncthreads();
+    }
+    if (inside) accum[(size_t)y * f.W + x] = make_float4(cr, cg, cb, 1.0f);
+}
+
+__global__ void k_u32_fill(uint32_t *p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_probe_reset(uint64_t *stats, uint32_t *visible) {
+    stats[ST_PROBE] = 0ull;
+    *visible = 0u;
+}
+
+unsigned nblk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ====================================================================== host
+struct SortedState {
+    uint32_t n = 0, W = 0, H = 0;
+    float4 *sp = nullptr;
+    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr, *ticket = nullptr, *visible = nullptr;
+    uint64_t *status = nullptr;
+    uint64_t status_cap = 0, hist_cap = 0;
+    uint32_t *cnt = nullptr;          // per sorted rank: tiles touched, then exclusive offsets
+    uint32_t *pk[2] = {nullptr, nullptr}, *pv[2] = {nullptr, nullptr};
+    uint64_t pair_cap = 0;
+    uint32_t *tstart = nullptr, *tend = nullptr;
+    uint32_t tiles_cap = 0;
+    uint32_t epoch = 1;
+    int result = 0;                   // which keys/vals buffer holds the last sort
+};
+
+SortedState *sorted_create() { return new SortedState(); }
+
+template <typename T>
+static void sfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+void sorted_destroy(SortedState *s) {
+    if (!s) return;
+    sfree(s->sp);
+    for (int k = 0; k < 2; k++) {
+        sfree(s->keys[k]);
+        sfree(s->vals[k]);
+        sfree(s->pk[k]);
+        sfree(s->pv[k]);
+    }
+    sfree(s->hist);
+    sfree(s->ticket);
+    sfree(s->visible);
+    sfree(s->status);
+    sfree(s->cnt);
+    sfree(s->tstart);
+    sfree(s->tend);
+    delete s;
+}
+
+#define S_TRY(call)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return -1;                                                               \
+        }                                                                            \
+    } while (0)
+
+using ProfList = std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>>;
+
+struct SProf {
+    ProfList *l;
+    hipStream_t st;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    SProf(ProfList *l_, hipStream_t s_, const char *n_) : l(l_), st(s_), name(n_) {
+        if (l) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~SProf() {
+        if (l) {
+            (void)hipEventRecord(b, st);
+            l->push_back({name, {a, b}});
+        }
+    }
+};
+
+static int ensure_status(SortedState *s, uint64_t tiles, std::string &err) {
+    if (tiles > s->status_cap) {
+        sfree(s->status);
+        S_TRY(hipMalloc(&s->status, tiles * 8));
+        S_TRY(hipMemset(s->status, 0, tiles * 8));
+        s->status_cap = tiles;
+    }
+    return 0;
+}
+
+// Stable LSD radix sort of n (key, value) pairs over bits [0, bits); result in buffers[result].
+static int radix_sort(SortedState *s, hipStream_t st, uint32_t **kb, uint32_t **vb, uint32_t n, uint32_t bits,
+                      int &result, std::string &err, ProfList *prof) {
+    result = 0;
+    if (n <= 1) return 0;
+    uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    uint64_t hn = 256ull * ntiles;
+    if (hn > s->hist_cap) {
+        sfree(s->hist);
+        S_TRY(hipMalloc(&s->hist, hn * 4));
+        s->hist_cap = hn;
+    }
+    uint32_t stiles = (uint32_t)((hn + 2047) / 2048);
+    if (ensure_status(s, stiles, err) != 0) return -1;
+    int cur = 0;
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        SProf p(prof, st, "radix_pass");
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], n, shift, ntiles, s->hist);
+        hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->hist, (uint32_t)hn, s->ticket, s->status,
+                           stiles, s->epoch);
+        s->epoch = (s->epoch + 1u) & ((1u << 26) - 1u);
+        if (!s->epoch) s->epoch = 1;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], vb[cur], kb[cur ^ 1],
+                           vb[cur ^ 1], n, shift, ntiles, s->hist);
+        cur ^= 1;
+    }
+    result = cur;
+    S_TRY(hipGetLastError());
+    return 0;
+}
+
+int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float4 *splats, const float *sh,
+                  float4 *accum, uint64_t *stats, int probe_only, std::string &err, ProfList *prof) {
+    const uint32_t n = f.N;
+    if (n != s->n || f.W != s->W || f.H != s->H) {
+        S_TRY(hipStreamSynchronize(st));
+        sfree(s->sp);
+        for (int k = 0; k < 2; k++) {
+            sfree(s->keys[k]);
+            sfree(s->vals[k]);
+        }
+        sfree(s->cnt);
+        sfree(s->tstart);
+        sfree(s->tend);
+        S_TRY(hipMalloc(&s->sp, (uint64_t)n * 48));
+        for (int k = 0; k < 2; k++) {
+            S_TRY(hipMalloc(&s->keys[k], (uint64_t)n * 4));
+            S_TRY(hipMalloc(&s->vals[k], (uint64_t)n * 4));
+        }
+        S_TRY(hipMalloc(&s->cnt, (uint64_t)n * 4));
+        if (!s->ticket) {
+            S_TRY(hipMalloc(&s->ticket, 4));
+            S_TRY(hipMemset(s->ticket, 0, 4));
+            S_TRY(hipMalloc(&s->visible, 4));
+        }
+        s->tiles_cap = ((f.W + BIN - 1) / BIN) * ((f.H + BIN - 1) / BIN);
+        S_TRY(hipMalloc(&s->tstart, (uint64_t)s->tiles_cap * 4));
+        S_TRY(hipMalloc(&s->tend, (uint64_t)s->tiles_cap * 4));
+        s->n = n;
+        s->W = f.W;
+        s->H = f.H;
+    }
+    hipLaunchKernelGGL(k_probe_reset, dim3(1), dim3(1), 0, st, stats, s->visible);
+    {
+        SProf p(prof, st, "sp_project");
+        hipLaunchKernelGGL(k_sp_project, dim3(nblk(n, 256)), dim3(256), 0, st, f, splats, sh, s->sp, s->keys[0],
+                           s->vals[0], s->visible, stats, probe_only);
+    }
+    if (probe_only) {
+        S_TRY(hipGetLastError());
+        return 0;
+    }
+    int res = 0;
+    if (radix_sort(s, st, s->keys, s->vals, n, 24, res, err, prof) != 0) return -1;
+    s->result = res;
+    uint32_t *skeys = s->keys[res], *svals = s->vals[res];
+    (void)skeys;
+    // binning: tiles per visible splat (sorted order) -> exclusive offsets -> pairs -> stable sort by tile
+    hipLaunchKernelGGL(k_sp_tilecount, dim3(nblk(n, 256)), dim3(256), 0, st, s->sp, svals, s->visible, n, f.W, f.H,
+                       s->cnt);
+    uint32_t last_cnt = 0;
+    S_TRY(hipMemcpyAsync(&last_cnt, s->cnt + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    S_TRY(hipStreamSynchronize(st));
+    uint32_t stiles = (n + 2047) / 2048;
+    if (ensure_status(s, stiles, err) != 0) return -1;
+    hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->cnt, n, s->ticket, s->status, stiles, s->epoch);
+    s->epoch = (s->epoch + 1u) & ((1u << 26) - 1u);
+    if (!s->epoch) s->epoch = 1;
+    uint32_t last_off = 0;
+    S_TRY(hipMemcpyAsync(&last_off, s->cnt + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    S_TRY(hipStreamSynchronize(st));
+    uint64_t npairs = (uint64_t)last_off + last_cnt;
+    if (npairs > 0xFFFFFFFFull) {
+        err = "sorted path: too many tile pairs";
+        return -1;
+    }
+    if (npairs > s->pair_cap) {
+        for (int k = 0; k < 2; k++) {
+            sfree(s->pk[k]);
+            sfree(s->pv[k]);
+            S_TRY(hipMalloc(&s->pk[k], npairs * 4));
+            S_TRY(hipMalloc(&s->pv[k], npairs * 4));
+        }
+        s->pair_cap = npairs;
+    }
+    hipLaunchKernelGGL(k_u32_fill, dim3(nblk(s->tiles_cap, 256)), dim3(256), 0, st, s->tstart, s->tiles_cap, 0u);
+    hipLaunchKernelGGL(k_u32_fill, dim3(nblk(s->tiles_cap, 256)), dim3(256), 0, st, s->tend, s->tiles_cap, 0u);
+    if (npairs) {
+        hipLaunchKernelGGL(k_sp_emit, dim3(nblk(n, 256)), dim3(256), 0, st, s->sp, svals, s->visible, n, f.W, f.H,
+                           s->cnt, s->pk[0], s->pv[0]);
+        uint32_t bits = 0;
+        while ((1u << bits) < s->tiles_cap) bits++;
+        bits = (bits + 7) / 8 * 8;
+        if (bits == 0) bits = 8;
+        int pres = 0;
+        if (radix_sort(s, st, s->pk, s->pv, (uint32_t)npairs, bits, pres, err, prof) != 0) return -1;
+        hipLaunchKernelGGL(k_tile_ranges, dim3(nblk(npairs, 256)), dim3(256), 0, st, s->pk[pres], (uint32_t)npairs,
+                           s->tstart, s->tend);
+        SProf p(prof, st, "sp_blend");
+        hipLaunchKernelGGL(k_sp_blend, dim3(s->tiles_cap), dim3(256), 0, st, f, s->sp, svals, s->pv[pres], s->tstart,
+                           s->tend, accum);
+    } else {
+        SProf p(prof, st, "sp_blend");
+        hipLaunchKernelGGL(k_sp_blend, dim3(s->tiles_cap), dim3(256), 0, st, f, s->sp, svals, s->pv[0], s->tstart,
+                           s->tend, accum);
+    }
+    S_TRY(hipGetLastError());
+    return 0;
+}
+
+int sorted_read(SortedState *s, hipStream_t st, uint32_t n, uint32_t *keys, uint32_t *vals, std::string &err) {
+    if (!s->keys[0] || s->n != n) {
+        err = "no sorted render yet";
+        return -1;
+    }
+    S_TRY(hipMemcpyAsync(keys, s->keys[s->result], (uint64_t)n * 4, hipMemcpyDeviceToHost, st));
+    S_TRY(hipMemcpyAsync(vals, s->vals[s->result], (uint64_t)n * 4, hipMemcpyDeviceToHost, st));
+    S_TRY(hipStreamSynchronize(st));
+    return 0;
+}

@@ -1,0 +1,424 @@
+"""Python binding of libgps_hip.so (the C ABI of include/gps_hip.h).
+
+``GpsRenderer`` mirrors gps_render::gps::GpsRenderer (crates/gps-render/src/gps.rs:302-1065): the
+same method names, argument meaning and error behaviour (a failed call raises ``GpsError`` carrying the
+library's message, the reference's ``Err(String)``).  The renderer runs on the MI355X through the HIP
+kernels only; there is no CPU fallback: loading fails loudly when the library is missing.
+"""
+import ctypes as C
+import enum
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libgps_hip.so")
+
+
+class GpsError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise GpsError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                       f"(make -C gps-pbvr-web_amd/csrc)")
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+u8, u32, u64, f32, f64 = C.c_uint8, C.c_uint32, C.c_uint64, C.c_float, C.c_double
+P = C.POINTER
+
+# GpuGaussian, gps.rs:26-38 (64 bytes)
+GAUSSIAN_DTYPE = np.dtype([("pos_opacity", "<f4", 4), ("log_scale", "<f4", 4), ("rot_wxyz", "<f4", 4),
+                           ("color", "<f4", 4)])
+SH_C0 = 0.28209479177387814
+
+
+class Method(enum.IntEnum):
+    """gps.rs:104-140"""
+    Gps = 0
+    Proportional = 1
+    Extinction = 2
+    ViewConditioned = 3
+    ScreenOccupancy = 4
+
+
+class Calibration(enum.IntEnum):
+    """gps.rs:144-154"""
+    None_ = 0
+    ObjectZoom = 1
+    PerSplatDepth = 2
+    PerSplatFootprint = 3
+
+
+class RenderPath(enum.IntEnum):
+    """gps.rs:93-100"""
+    Primitive = 0
+    Particle = 1
+
+
+class GpsCamera(C.Structure):
+    """OracleCamera, oracle.rs:33-42 (view_rot row-major)."""
+    _fields_ = [("view_rot", f64 * 9), ("view_pos", f64 * 3), ("focal_x", f64), ("focal_y", f64), ("cx", f64),
+                ("cy", f64), ("low_pass", f64)]
+
+
+class PbvrParams(C.Structure):
+    """gps.rs:169-179"""
+    _fields_ = [("calibration", u32), ("radial_correction", u32), ("centre_depth", u32), ("low_pass_jitter", u32),
+                ("base_k", f32), ("reference_pixel_length", f32)]
+
+
+class RenderParams(C.Structure):
+    """gps.rs:189-229; RenderParams() gives RenderParams::default()."""
+    _fields_ = [("spp_side", u32), ("density_scale", f32), ("max_points_per_splat", u32), ("near", f32),
+                ("background", f32 * 3), ("exposure", f32), ("srgb_output", u32), ("exact_radius", u32),
+                ("sh_degree", u32), ("path", u32), ("method", u32), ("pbvr", PbvrParams)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib.gps_render_params_default(C.byref(self))
+        for k, v in kw.items():
+            if k == "pbvr" and isinstance(v, dict):
+                for kk, vv in v.items():
+                    setattr(self.pbvr, kk, int(vv) if isinstance(vv, (bool, enum.IntEnum)) else vv)
+            elif k == "background":
+                self.background[:] = list(v)
+            else:
+                setattr(self, k, int(v) if isinstance(v, (bool, enum.IntEnum)) else v)
+
+    def copy(self, **kw):
+        q = RenderParams()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(RenderParams))
+        for k, v in kw.items():
+            if k == "pbvr" and isinstance(v, dict):
+                for kk, vv in v.items():
+                    setattr(q.pbvr, kk, int(vv) if isinstance(vv, (bool, enum.IntEnum)) else vv)
+            elif k == "background":
+                q.background[:] = list(v)
+            else:
+                setattr(q, k, int(v) if isinstance(v, (bool, enum.IntEnum)) else v)
+        return q
+
+
+class GpuStats(C.Structure):
+    """gps.rs:233-252"""
+    _fields_ = [("points", u64), ("truncated_splats", u64), ("dropped_points", u64), ("visible_splats", u64),
+                ("orphan_subpixels", u64), ("candidates", u64), ("undercovered_splats", u64),
+                ("skipped_ensembles", u64), ("ensembles", u64), ("probe_demand", u64), ("issue", u32),
+                ("reserved", u32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+    def __eq__(self, other):
+        return self.as_dict() == other.as_dict()
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", u64), ("total_ms", f64)]
+
+
+def _sig(name, res, args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+
+
+_R = C.c_void_p
+_sig("gps_render_params_default", None, [P(RenderParams)])
+_sig("gps_renderer_create", C.c_int, [u32, u32, u32, C.c_int, P(C.c_void_p)])
+_sig("gps_renderer_destroy", None, [_R])
+_sig("gps_renderer_last_error", C.c_char_p, [_R])
+_sig("gps_renderer_set_gaussians", C.c_int, [_R, C.c_void_p, u64, P(f32), u32])
+_sig("gps_renderer_resize", C.c_int, [_R, u32, u32, u32])
+_sig("gps_renderer_reset_accum", C.c_int, [_R])
+_sig("gps_renderer_reset_stats", C.c_int, [_R])
+_sig("gps_renderer_render_ensembles", C.c_int, [_R, P(GpsCamera), P(RenderParams), u32, u32])
+_sig("gps_renderer_render_sorted", C.c_int, [_R, P(GpsCamera), P(RenderParams), u32])
+_sig("gps_renderer_probe_demand", C.c_int, [_R, P(GpsCamera), P(RenderParams)])
+_sig("gps_renderer_composite", C.c_int, [_R, P(RenderParams), u32, u32, P(u8)])
+_sig("gps_renderer_read_accum", C.c_int, [_R, P(f32)])
+_sig("gps_renderer_read_stats", C.c_int, [_R, P(GpuStats)])
+_sig("gps_renderer_read_sorted", C.c_int, [_R, P(u32), P(u32)])
+_sig("gps_renderer_wait_idle", C.c_int, [_R])
+_sig("gps_renderer_submitted", u32, [_R])
+_sig("gps_renderer_accumulated", u32, [_R])
+_sig("gps_renderer_info", None, [_R, P(u32), P(u32), P(u32), P(u64), P(u32)])
+_sig("gps_estimate_memory", None, [u32, u32, u32, u64, P(u64), P(u64), P(u64)])
+_sig("gps_radius_lut", None, [P(f32)])
+_sig("gps_renderer_set_debug", C.c_int, [_R, u32])
+_sig("gps_renderer_read_counts", C.c_int, [_R, P(u32), P(u32), P(u32), P(u32)])
+_sig("gps_renderer_read_depth_id", C.c_int, [_R, P(u64)])
+_sig("gps_phase0_atomic", C.c_int, [C.c_int, u32, u32, u32, P(u32), P(u32)])
+_sig("gps_scan_exclusive_u32", C.c_int, [C.c_int, P(u32), P(u32), u64])
+_sig("gps_math_eval", C.c_int, [C.c_int, C.c_int, P(f32), P(f32), P(f32), u64])
+_sig("gps_renderer_profile", C.c_int, [_R, C.c_int])
+_sig("gps_renderer_profile_read", C.c_int, [_R, P(KernelTime), C.c_int])
+
+# exported names the header declares (checked by the CPU test suite)
+EXPORTED = [
+    "gps_render_params_default", "gps_renderer_create", "gps_renderer_destroy", "gps_renderer_last_error",
+    "gps_renderer_set_gaussians", "gps_renderer_resize", "gps_renderer_reset_accum", "gps_renderer_reset_stats",
+    "gps_renderer_render_ensembles", "gps_renderer_render_sorted", "gps_renderer_probe_demand",
+    "gps_renderer_composite", "gps_renderer_read_accum", "gps_renderer_read_stats", "gps_renderer_read_sorted",
+    "gps_renderer_wait_idle", "gps_renderer_submitted", "gps_renderer_accumulated", "gps_renderer_info",
+    "gps_estimate_memory", "gps_radius_lut", "gps_renderer_set_debug", "gps_renderer_read_counts",
+    "gps_renderer_read_depth_id", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
+    "gps_renderer_profile", "gps_renderer_profile_read",
+]
+
+DEBUG_KEEP_BUFFERS = 1
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(P(t))
+
+
+def radius_lut():
+    """crates/gps-core/src/lib.rs:624-636, computed by the library (f64, rounded to f32)."""
+    t = np.zeros(65 * 129, np.float32)
+    lib.gps_radius_lut(_ptr(t, f32))
+    return t
+
+
+def estimate_memory(w, h, spp_side, n):
+    a, b, c = u64(), u64(), u64()
+    lib.gps_estimate_memory(w, h, spp_side, n, C.byref(a), C.byref(b), C.byref(c))
+    return dict(depth_winner_bytes=a.value, accum_bytes=b.value, splat_bytes=c.value)
+
+
+class GpsRenderer:
+    """gps.rs:302-1065 on one HIP device."""
+
+    def __init__(self, width, height, spp_side, device=0):
+        h = C.c_void_p()
+        if lib.gps_renderer_create(width, height, spp_side, device, C.byref(h)) != 0:
+            raise GpsError(lib.gps_renderer_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.gps_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GpsError(lib.gps_renderer_last_error(self._h).decode())
+
+    # ---- scene / buffers
+    def set_gaussians(self, data):
+        return self.set_gaussians_sh(data, None, 0)
+
+    def set_gaussians_sh(self, data, sh_rest, degree):
+        data = np.ascontiguousarray(data)
+        if data.dtype != GAUSSIAN_DTYPE:
+            data = np.ascontiguousarray(data, dtype=np.float32).reshape(-1, 16).view(GAUSSIAN_DTYPE).reshape(-1)
+        n = data.shape[0]
+        sh = None
+        if sh_rest is not None and degree >= 1 and n > 0:
+            stride = (min(degree, 3) + 1) ** 2 - 1
+            sh = np.ascontiguousarray(sh_rest, np.float32).reshape(-1)
+            if sh.size != n * 3 * stride:
+                raise GpsError(f"SH data has {sh.size} values, expected {n * 3 * stride} for {n} splats at degree "
+                               f"{degree}")
+        self._check(lib.gps_renderer_set_gaussians(self._h, data.ctypes.data if n else None, n,
+                                                   _ptr(sh, f32) if sh is not None else None,
+                                                   degree if sh is not None else 0))
+
+    def resize(self, width, height, spp_side):
+        self._check(lib.gps_renderer_resize(self._h, width, height, spp_side))
+
+    def reset_accum(self):
+        self._check(lib.gps_renderer_reset_accum(self._h))
+
+    def reset_stats(self):
+        self._check(lib.gps_renderer_reset_stats(self._h))
+
+    # ---- rendering
+    def render_ensembles(self, cam, params, first_seed, count):
+        self._check(lib.gps_renderer_render_ensembles(self._h, C.byref(cam), C.byref(params), first_seed & 0xFFFFFFFF,
+                                                      count))
+
+    def render_sorted(self, cam, params, as_ensembles=1):
+        self._check(lib.gps_renderer_render_sorted(self._h, C.byref(cam), C.byref(params), as_ensembles))
+
+    def probe_demand(self, cam, params):
+        self._check(lib.gps_renderer_probe_demand(self._h, C.byref(cam), C.byref(params)))
+
+    def composite(self, params, out_w=None, out_h=None):
+        w, h = self.size()
+        ow, oh = out_w or w, out_h or h
+        out = np.zeros((oh, ow, 4), np.uint8)
+        self._check(lib.gps_renderer_composite(self._h, C.byref(params), ow, oh, _ptr(out, u8)))
+        return out
+
+    # ---- readbacks
+    def read_accum(self):
+        w, h = self.size()
+        out = np.zeros((h, w, 4), np.float32)
+        self._check(lib.gps_renderer_read_accum(self._h, _ptr(out, f32)))
+        return out
+
+    def read_stats(self):
+        s = GpuStats()
+        self._check(lib.gps_renderer_read_stats(self._h, C.byref(s)))
+        return s
+
+    def read_sorted(self):
+        n = self.gaussian_count()
+        k = np.zeros(max(n, 1), np.uint32)
+        v = np.zeros(max(n, 1), np.uint32)
+        self._check(lib.gps_renderer_read_sorted(self._h, _ptr(k, u32), _ptr(v, u32)))
+        return k[:n], v[:n]
+
+    def wait_idle(self):
+        self._check(lib.gps_renderer_wait_idle(self._h))
+
+    def submitted(self):
+        return lib.gps_renderer_submitted(self._h)
+
+    def accumulated(self):
+        return lib.gps_renderer_accumulated(self._h)
+
+    def _info(self):
+        w, h, s, d = u32(), u32(), u32(), u32()
+        n = u64()
+        lib.gps_renderer_info(self._h, C.byref(w), C.byref(h), C.byref(s), C.byref(n), C.byref(d))
+        return w.value, h.value, s.value, n.value, d.value
+
+    def size(self):
+        w, h, _, _, _ = self._info()
+        return w, h
+
+    def spp_side(self):
+        return self._info()[2]
+
+    def gaussian_count(self):
+        return self._info()[3]
+
+    def sh_max_degree(self):
+        return self._info()[4]
+
+    # ---- bit-exact test hooks
+    def set_debug(self, flags):
+        self._check(lib.gps_renderer_set_debug(self._h, flags))
+
+    def read_counts(self):
+        n = self.gaussian_count()
+        c = np.zeros(max(n, 1), np.uint32)
+        o = np.zeros(max(n, 1), np.uint32)
+        t, sk = u32(), u32()
+        self._check(lib.gps_renderer_read_counts(self._h, _ptr(c, u32), _ptr(o, u32), C.byref(t), C.byref(sk)))
+        return dict(counts=c[:n], offsets=o[:n], total=t.value, skipped=sk.value)
+
+    def read_depth_id(self):
+        w, h, s, _, _ = self._info()
+        out = np.zeros(w * h * s * s, np.uint64)
+        self._check(lib.gps_renderer_read_depth_id(self._h, _ptr(out, u64)))
+        return out
+
+    # ---- timing
+    def profile(self, enable=True):
+        self._check(lib.gps_renderer_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        buf = (KernelTime * 64)()
+        n = lib.gps_renderer_profile_read(self._h, buf, 64)
+        if n < 0:
+            raise GpsError("profile read failed")
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(min(n, 64))}
+
+
+# ------------------------------------------------------------------ unit entry points
+def phase0_atomic(seed, pixels, samples, device=0):
+    d = np.zeros(pixels, np.uint32)
+    c = np.zeros(pixels, np.uint32)
+    if lib.gps_phase0_atomic(device, seed, pixels, samples, _ptr(d, u32), _ptr(c, u32)) != 0:
+        raise GpsError("phase-0 atomic kernel failed")
+    return d, c
+
+
+def scan_exclusive_u32(x, device=0):
+    x = np.ascontiguousarray(x, np.uint32)
+    out = np.zeros_like(x)
+    if lib.gps_scan_exclusive_u32(device, _ptr(x, u32), _ptr(out, u32), x.size) != 0:
+        raise GpsError("scan kernel failed")
+    return out
+
+
+MATH_FN = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sqrt": 4, "div": 5}
+
+
+def math_eval(fn, x, y=None, device=0):
+    x = np.ascontiguousarray(x, np.float32)
+    yy = np.ascontiguousarray(y, np.float32) if y is not None else None
+    out = np.zeros_like(x)
+    if lib.gps_math_eval(device, MATH_FN[fn], _ptr(x, f32), _ptr(yy, f32) if yy is not None else None,
+                         _ptr(out, f32), x.size) != 0:
+        raise GpsError("math kernel failed")
+    return out
+
+
+# ------------------------------------------------------------------ cameras (oracle.rs:53-70, camera.rs:31-40)
+def _normalize(v):
+    v = np.asarray(v, np.float64)
+    return v * (1.0 / math.sqrt(float(v @ v)))
+
+
+def look_at(eye, target, up, fov_y, w, h):
+    """OracleCamera::look_at: +Z forward, image +Y down; rows of view_rot are (right, down, forward)."""
+    eye = np.asarray(eye, np.float64)
+    f = _normalize(np.asarray(target, np.float64) - eye)
+    right = _normalize(np.cross(f, np.asarray(up, np.float64)))
+    down = np.cross(f, right)
+    c = GpsCamera()
+    c.view_rot[:] = [*right, *down, *f]
+    c.view_pos[:] = list(eye)
+    focal = 0.5 * h / math.tan(0.5 * fov_y)
+    c.focal_x = c.focal_y = focal
+    c.cx, c.cy = 0.5 * w, 0.5 * h
+    c.low_pass = 0.3
+    return c
+
+
+def orbit_eye(target, distance, yaw, pitch, up):
+    """OrbitCamera::eye, camera.rs:31-40."""
+    up = _normalize(up)
+    helper = np.array([1.0, 0.0, 0.0]) if abs(up[0]) < 0.9 else np.array([0.0, 0.0, 1.0])
+    a = _normalize(np.cross(up, helper))
+    b = np.cross(up, a)
+    horiz = a * math.cos(yaw) + b * math.sin(yaw)
+    d = horiz * math.cos(pitch) + up * math.sin(pitch)
+    return np.asarray(target, np.float64) + d * distance
+
+
+def orbit_camera(target, distance, yaw, pitch, up, fov_y, w, h):
+    return look_at(orbit_eye(target, distance, yaw, pitch, up), target, up, fov_y, w, h)
+
+
+def default_camera(focal=200.0, low_pass=0.3):
+    """OracleCamera::default() (oracle.rs:44-48) with a chosen focal length."""
+    c = GpsCamera()
+    c.view_rot[:] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    c.view_pos[:] = [0, 0, 0]
+    c.focal_x = c.focal_y = focal
+    c.cx = c.cy = 0.0
+    c.low_pass = low_pass
+    return c

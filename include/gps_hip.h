@@ -1,0 +1,185 @@
+/*
+ * gps_hip: C ABI of the MI355X (gfx950) particle renderer.
+ *
+ * This ABI replaces the reference renderer gps_render::gps::GpsRenderer
+ * (crates/gps-render/src/gps.rs:302-1065) for callers that bind C: the
+ * gps-verify CLI (crates/gps-verify/src/main.rs), the phase gates
+ * (crates/gps-render/src/verify.rs, lib.rs) and the browser viewer
+ * (crates/gps-web/src/lib.rs).  Every method keeps the reference's name,
+ * argument meaning and error behaviour:
+ *
+ *   - every entry point returns an int status: 0 = ok, nonzero = error, and
+ *     gps_renderer_last_error() returns the message (the reference's
+ *     Result<_, String>); a failed call leaves the renderer's state as it was
+ *     (gps.rs:682);
+ *   - all GPU work of one renderer runs in order on one HIP stream owned by
+ *     the renderer; read functions synchronise that stream;
+ *   - one renderer per host thread (the reference's &mut self).
+ *
+ * Struct layouts are the reference's (gps.rs:26-59, 167-272; oracle.rs:33-42).
+ */
+#ifndef GPS_HIP_H
+#define GPS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One Gaussian, 64 bytes: GpuGaussian, gps.rs:26-38 */
+typedef struct GpsGaussian {
+    float pos_opacity[4]; /* xyz position, w = activated opacity */
+    float log_scale[4];   /* xyz log standard deviations */
+    float rot_wxyz[4];    /* quaternion (w, x, y, z), normalised by the kernels */
+    float color[4];       /* xyz = f_dc * SH_C0 + 0.5 (linear) */
+} GpsGaussian;
+
+/* OracleCamera, oracle.rs:33-42.  cam = view_rot * (world - view_pos), +Z forward,
+ * pixel y down; view_rot is row-major.  cx = cy = 0 means the image centre. */
+typedef struct GpsCamera {
+    double view_rot[9];
+    double view_pos[3];
+    double focal_x, focal_y, cx, cy, low_pass;
+} GpsCamera;
+
+/* Method, gps.rs:104-140 */
+enum { GPS_METHOD_GPS = 0, GPS_METHOD_PROPORTIONAL = 1, GPS_METHOD_EXTINCTION = 2,
+       GPS_METHOD_VIEW_CONDITIONED = 3, GPS_METHOD_SCREEN_OCCUPANCY = 4 };
+/* Calibration, gps.rs:144-154 */
+enum { GPS_CAL_NONE = 0, GPS_CAL_OBJECT_ZOOM = 1, GPS_CAL_PER_SPLAT_DEPTH = 2, GPS_CAL_PER_SPLAT_FOOTPRINT = 3 };
+/* RenderPath, gps.rs:93-100 */
+enum { GPS_PATH_PRIMITIVE = 0, GPS_PATH_PARTICLE = 1 };
+
+/* PbvrParams, gps.rs:169-179 */
+typedef struct GpsPbvrParams {
+    uint32_t calibration;
+    uint32_t radial_correction;
+    uint32_t centre_depth;
+    uint32_t low_pass_jitter;
+    float base_k;
+    float reference_pixel_length;
+} GpsPbvrParams;
+
+/* RenderParams, gps.rs:189-210 */
+typedef struct GpsRenderParams {
+    uint32_t spp_side;             /* subpixels per pixel = spp_side^2, 1..4 */
+    float density_scale;
+    uint32_t max_points_per_splat;
+    float near;
+    float background[3];
+    float exposure;
+    uint32_t srgb_output;
+    uint32_t exact_radius;
+    uint32_t sh_degree;
+    uint32_t path;
+    uint32_t method;
+    GpsPbvrParams pbvr;
+} GpsRenderParams;
+
+/* GpuStats, gps.rs:233-252 (cumulative since the last gps_renderer_reset_stats) */
+typedef struct GpsStats {
+    uint64_t points;
+    uint64_t truncated_splats;
+    uint64_t dropped_points;
+    uint64_t visible_splats;
+    uint64_t orphan_subpixels; /* always 0: one packed 64-bit minimum per subpixel */
+    uint64_t candidates;
+    uint64_t undercovered_splats;
+    uint64_t skipped_ensembles;
+    uint64_t ensembles;
+    uint64_t probe_demand;
+    uint32_t issue;
+    uint32_t reserved;
+} GpsStats;
+
+typedef struct GpsRenderer GpsRenderer;
+
+/* RenderParams::default(), gps.rs:212-229 */
+void gps_render_params_default(GpsRenderParams *out);
+
+/* GpsRenderer::new, gps.rs:380-570: w x h pixels, spp_side clamped to 1..4, on HIP device `device`. */
+int gps_renderer_create(uint32_t width, uint32_t height, uint32_t spp_side, int device, GpsRenderer **out);
+void gps_renderer_destroy(GpsRenderer *r);
+/* message of the last failed call on r (empty if none); with r == NULL, of the last failed create */
+const char *gps_renderer_last_error(const GpsRenderer *r);
+
+/* set_gaussians / set_gaussians_sh, gps.rs:683-740.  sh_rest: NULL or 3 * ((degree+1)^2 - 1) floats per
+ * splat, channel-major; degree 1..3. */
+int gps_renderer_set_gaussians(GpsRenderer *r, const GpsGaussian *g, uint64_t n, const float *sh_rest,
+                               uint32_t degree);
+/* resize, gps.rs:781-789 */
+int gps_renderer_resize(GpsRenderer *r, uint32_t width, uint32_t height, uint32_t spp_side);
+/* reset_accum, gps.rs:792-801 (also restarts the skipped-ensemble counter) */
+int gps_renderer_reset_accum(GpsRenderer *r);
+/* reset_stats, gps.rs:803-809 */
+int gps_renderer_reset_stats(GpsRenderer *r);
+/* render_ensembles, gps.rs:861-925: ensembles first_seed .. first_seed+count-1 into the accumulation
+ * buffer.  Any count is accepted (the reference caps one submission at 4). */
+int gps_renderer_render_ensembles(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p,
+                                  uint32_t first_seed, uint32_t count);
+/* render_sorted, gps.rs:930-944 */
+int gps_renderer_render_sorted(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p,
+                               uint32_t as_ensembles);
+/* probe_demand, gps.rs:948-955 */
+int gps_renderer_probe_demand(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p);
+/* composite, gps.rs:967-986, into an out_w x out_h RGBA8 image (the wgpu target view's replacement) */
+int gps_renderer_composite(GpsRenderer *r, const GpsRenderParams *p, uint32_t out_w, uint32_t out_h,
+                           uint8_t *rgba8);
+/* read_accum, gps.rs:989-993: w*h*4 floats (RGB sum, ensemble count) */
+int gps_renderer_read_accum(GpsRenderer *r, float *out);
+/* read_stats, gps.rs:996-1000 */
+int gps_renderer_read_stats(GpsRenderer *r, GpsStats *out);
+/* read_sorted, gps.rs:958-963: n keys and n values of the last render_sorted */
+int gps_renderer_read_sorted(GpsRenderer *r, uint32_t *keys, uint32_t *vals);
+/* wait_idle, gps.rs:1055-1057 */
+int gps_renderer_wait_idle(GpsRenderer *r);
+/* submitted / accumulated, gps.rs:669-677 */
+uint32_t gps_renderer_submitted(const GpsRenderer *r);
+uint32_t gps_renderer_accumulated(GpsRenderer *r);
+/* size / spp_side / gaussian_count / sh_max_degree accessors, gps.rs:651-666 */
+void gps_renderer_info(const GpsRenderer *r, uint32_t *width, uint32_t *height, uint32_t *spp_side,
+                       uint64_t *n_splats, uint32_t *sh_max_degree);
+
+/* estimate_memory, gps.rs:288-291 (bytes of depth+winner, accum, splats) */
+void gps_estimate_memory(uint32_t width, uint32_t height, uint32_t spp_side, uint64_t n_splats,
+                         uint64_t *depth_winner, uint64_t *accum, uint64_t *splats);
+/* radius_lut, crates/gps-core/src/lib.rs:624-636: 65 x 129 floats */
+void gps_radius_lut(float *out);
+
+/* ---- bit-exact test hooks (the reference reads these buffers in its phase-0 and verify gates) ----
+ * GPS_DEBUG_KEEP_BUFFERS keeps the per-ensemble buffers of the last ensemble of the last
+ * gps_renderer_render_ensembles call readable (one ensemble per batch; slower). */
+enum { GPS_DEBUG_KEEP_BUFFERS = 1 };
+int gps_renderer_set_debug(GpsRenderer *r, uint32_t flags);
+/* per-splat particle counts and exclusive offsets (n u32 each), particle total and skip flag */
+int gps_renderer_read_counts(GpsRenderer *r, uint32_t *counts, uint32_t *offsets, uint32_t *total,
+                             uint32_t *skipped);
+/* packed depth|id per subpixel (w*h*spp u64): (ordered depth key << 32) | splat id, all ones when empty;
+ * the high and low halves are the reference's `depth` and `winner` buffers (gps_splat.wgsl:28-29) */
+int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out);
+
+/* Phase-0 unit kernels (crates/gps-render/src/lib.rs:99-275; shaders/p0_sample.wgsl, gps_scan.wgsl).
+ * gps_phase0_atomic: samples of p0_sample.wgsl:205-218 for one ensemble with one packed 64-bit
+ * minimum; writes pixels depth keys and colours.  gps_scan_exclusive_u32: exclusive prefix sum. */
+int gps_phase0_atomic(int device, uint32_t seed, uint32_t pixels, uint32_t samples, uint32_t *depth,
+                      uint32_t *color);
+int gps_scan_exclusive_u32(int device, const uint32_t *in, uint32_t *out, uint64_t n);
+/* Device evaluation of the kernels' f32 functions (0 exp, 1 log, 2 sin, 3 cos, 4 sqrt, 5 div x/y). */
+int gps_math_eval(int device, int fn, const float *x, const float *y, float *out, uint64_t n);
+
+/* ---- timing: HIP events around every kernel launch on the renderer's stream ---- */
+typedef struct GpsKernelTime {
+    char name[32];
+    uint64_t launches;
+    double total_ms;
+} GpsKernelTime;
+int gps_renderer_profile(GpsRenderer *r, int enable);
+/* fills up to max entries, returns the number of kernels recorded (or -1) */
+int gps_renderer_profile_read(GpsRenderer *r, GpsKernelTime *out, int max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -180,7 +180,7 @@ EXPORT void orc_eval_sh(uint32_t degree, const double dc[3], const double *rest,
 }
 
 /* lib.rs:98-106: Sigma = R S^2 R^T */
-static m3 covariance_3d(const double ls[3], const double q[4]) {
+static m3 covariance_3d(const double *ls, const double *q) {
     m3 r = quat_to_rotation(q[0], q[1], q[2], q[3]);
     m3 s2;
     memset(&s2, 0, sizeof s2);

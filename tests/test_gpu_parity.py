@@ -1,0 +1,408 @@
+"""HIP path vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Bit-exact: device f32 functions, phase-0 packed atomics (crates/gps-render/src/lib.rs:99-192), the scan
+(lib.rs:196-275), per-splat particle counts, exclusive offsets, particle totals, depth|id buffers,
+accumulation buffers and every statistics counter of render_ensembles, against oracle/gps_mirror.c on the
+same seeded inputs.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from scenes import BG, crowd, many_splats, phase_scenes, to_gpu
+
+pytestmark = pytest.mark.gpu
+
+gh = pytest.importorskip("gps_hip")
+from gps_hip import scenes as S  # noqa: E402
+
+STAT_KEYS = ("points", "truncated_splats", "dropped_points", "visible_splats", "orphan_subpixels", "candidates",
+             "undercovered_splats", "skipped_ensembles", "ensembles")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mparams(p):
+    """gps_hip.RenderParams -> oracle MParams (same field order)."""
+    m = O.MParams()
+    import ctypes
+    ctypes.memmove(ctypes.byref(m), ctypes.byref(p), ctypes.sizeof(O.MParams))
+    return m
+
+
+def ocam(c):
+    o = O.OrcCamera()
+    import ctypes
+    ctypes.memmove(ctypes.byref(o), ctypes.byref(c), ctypes.sizeof(O.OrcCamera))
+    return o
+
+
+def stats_tuple(st):
+    return tuple(int(getattr(st, k)) for k in STAT_KEYS)
+
+
+def mirror_stats_tuple(m):
+    return tuple(int(m[i]) for i in range(9))
+
+
+# ------------------------------------------------------------------ elementary functions
+@pytest.mark.parametrize("fn,lo,hi", [("exp", -104.0, 89.0), ("log", -30.0, 30.0), ("sin", 0.0, 6.2832),
+                                      ("cos", 0.0, 6.2832)])
+def test_device_math_bit_exact(fn, lo, hi):
+    say(f"[math] {fn}")
+    rng = np.random.default_rng(3)
+    if fn == "log":
+        x = np.exp(rng.uniform(lo, hi, 1_000_000)).astype(np.float32)
+        x = np.concatenate([x, rng.uniform(0.0, 2.0, 1_000_000).astype(np.float32),
+                            np.array([0.0, 1e-45, 1.0, np.inf], np.float32)])
+    else:
+        x = rng.uniform(lo, hi, 2_000_000).astype(np.float32)
+    dev = gh.math_eval(fn, x)
+    ref = O.mirror_fn(f"mir_{fn}f_n", x)
+    assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
+
+
+def test_device_sqrt_div_correctly_rounded():
+    say("[math] sqrt/div")
+    rng = np.random.default_rng(4)
+    x = np.exp(rng.uniform(-80, 80, 1_000_000)).astype(np.float32)
+    y = np.exp(rng.uniform(-80, 80, 1_000_000)).astype(np.float32)
+    assert np.array_equal(gh.math_eval("sqrt", x).view(np.uint32), np.sqrt(x).view(np.uint32))
+    assert np.array_equal(gh.math_eval("div", x, y).view(np.uint32), (x / y).view(np.uint32))
+
+
+# ------------------------------------------------------------------ phase 0
+def test_phase0_packed_atomic_equals_two_pass():
+    """p0_sample.wgsl two-pass depth/colour atomics == one packed 64-bit minimum (lib.rs:169-191)."""
+    PIX, SAMPLES = 4096, 200_000
+    for e in range(3):
+        seed = 1000 + e * 7
+        say(f"[phase0 atomic] seed={seed}")
+        d, c = gh.phase0_atomic(seed, PIX, SAMPLES)
+        rd = np.zeros(PIX, np.uint32)
+        rc = np.zeros(PIX, np.uint32)
+        O.lib.mir_phase0(seed, 0, PIX, SAMPLES, O.ptr(rd, O.u32), O.ptr(rc, O.u32))
+        assert np.array_equal(d, rd) and np.array_equal(c, rc)
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 65_536, 300_000, 1_000_003, 40_000_000])
+def test_scan_exact(n):
+    """lib.rs:196-275 with input pcg_hash(i) % 7."""
+    say(f"[scan] n={n}")
+    i = np.arange(n, dtype=np.uint64)
+    m = np.uint64(0xFFFFFFFF)
+    s = (i * np.uint64(747796405) + np.uint64(2891336453)) & m
+    w = (((s >> ((s >> np.uint64(28)) + np.uint64(4))) ^ s) * np.uint64(277803737)) & m
+    x = (((w >> np.uint64(22)) ^ w) % np.uint64(7)).astype(np.uint32)
+    got = gh.scan_exclusive_u32(x)
+    ref = np.concatenate([[0], np.cumsum(x[:-1], dtype=np.uint64)]).astype(np.uint32)
+    assert np.array_equal(got, ref)
+
+
+# ------------------------------------------------------------------ render_ensembles vs the mirror
+W = H = 96
+
+
+def run_pair(scene_arr, sh, deg, cam, p, w, h, first_seed, count, keep=True):
+    """Render `count` ensembles on the GPU and in the mirror; return both results."""
+    r = gh.GpsRenderer(w, h, p.spp_side)
+    try:
+        if sh is not None:
+            r.set_gaussians_sh(scene_arr, sh, deg)
+        else:
+            r.set_gaussians(scene_arr)
+        if keep:
+            r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+        r.reset_stats()
+        r.render_ensembles(cam, p, first_seed, count)
+        acc = r.read_accum()
+        st = r.read_stats()
+        bufs = None
+        if keep and (p.path == 1 or p.method != 0):
+            bufs = r.read_counts()
+            bufs["depth_id"] = r.read_depth_id()
+    finally:
+        r.close()
+    m = O.mirror_render(scene_arr, sh, deg, ocam(cam), mparams(p), w, h, first_seed, count, want_buffers=keep)
+    return dict(accum=acc, stats=st, bufs=bufs), m
+
+
+def assert_exact(gpu, m, check_buffers=True):
+    assert stats_tuple(gpu["stats"]) == mirror_stats_tuple(m["stats"])
+    assert np.array_equal(gpu["accum"].view(np.uint32), m["accum"].view(np.uint32))
+    if check_buffers and gpu["bufs"] is not None:
+        b = gpu["bufs"]
+        assert b["skipped"] == m["skipped"] and b["total"] == m["total"]
+        assert np.array_equal(b["offsets"], m["offsets"])
+        if not m["skipped"]:
+            assert np.array_equal(b["counts"], m["counts"])
+        assert np.array_equal(b["depth_id"], m["depth_id"])
+
+
+CAM = gh.default_camera(200.0)
+
+
+@pytest.mark.parametrize("idx", range(5))
+@pytest.mark.parametrize("path", [0, 1])
+def test_gps_bit_exact(idx, path):
+    name, scene = phase_scenes()[idx]
+    say(f"[gps] {name} path={path}")
+    p = gh.RenderParams(spp_side=2, background=BG, path=path)
+    gpu, m = run_pair(to_gpu(scene), None, 0, CAM, p, W, H, 1, 6)
+    assert_exact(gpu, m)
+    assert gpu["stats"].orphan_subpixels == 0
+
+
+PBVR_CASES = [(1, 0, 0, 0, 1), (1, 3, 0, 0, 1), (2, 0, 0, 0, 1), (2, 1, 0, 0, 1), (2, 2, 0, 0, 1), (2, 3, 0, 0, 1),
+              (2, 3, 0, 0, 0), (2, 3, 0, 1, 1), (2, 3, 1, 0, 1), (2, 3, 1, 1, 1), (3, 0, 0, 0, 1)]
+
+
+@pytest.mark.parametrize("case", PBVR_CASES)
+@pytest.mark.parametrize("si", [1, 3, 4])
+def test_pbvr_bit_exact(case, si):
+    method, cal, radial, centre, jitter = case
+    name, scene = phase_scenes()[si]
+    say(f"[pbvr] {name} case={case}")
+    p = gh.RenderParams(spp_side=2, background=BG, method=method, max_points_per_splat=1 << 24,
+                        pbvr=dict(calibration=cal, radial_correction=radial, centre_depth=centre,
+                                  low_pass_jitter=jitter))
+    gpu, m = run_pair(to_gpu(scene), None, 0, CAM, p, W, H, 1, 4)
+    assert_exact(gpu, m)
+
+
+@pytest.mark.parametrize("density", [0.25, 2.0])
+@pytest.mark.parametrize("si", [1, 3, 4])
+def test_occupancy_bit_exact(density, si):
+    name, scene = phase_scenes()[si]
+    say(f"[occupancy] {name} density={density}")
+    p = gh.RenderParams(spp_side=2, background=BG, method=4, density_scale=density, max_points_per_splat=1)
+    gpu, m = run_pair(to_gpu(scene), None, 0, CAM, p, W, H, 500, 4)
+    assert_exact(gpu, m)
+
+
+def test_cap_truncation_and_exact_radius():
+    scene = phase_scenes()[2][1]
+    for kw in (dict(max_points_per_splat=50), dict(exact_radius=1), dict(spp_side=1), dict(spp_side=3)):
+        say(f"[gps variant] {kw}")
+        p = gh.RenderParams(spp_side=2, background=BG).copy(**kw)
+        gpu, m = run_pair(to_gpu(scene), None, 0, CAM, p, W, H, 7, 3)
+        assert_exact(gpu, m)
+
+
+def test_sh_degrees_bit_exact():
+    rng = np.random.default_rng(5)
+    scene = S.small_uniform(512, seed=3)
+    sh = rng.normal(0, 0.3, 512 * 3 * 15).astype(np.float32)
+    cam = S.small_uniform_camera(64, 64)
+    for deg in range(4):
+        say(f"[sh] degree {deg}")
+        p = gh.RenderParams(spp_side=1, sh_degree=deg, method=2)
+        gpu, m = run_pair(scene, sh, 3, cam, p, 64, 64, 3, 3)
+        assert_exact(gpu, m)
+
+
+def test_many_splats_three_levels():
+    """verify.rs:173-186 and 541-557: 70,000 splats (several scan tiles)."""
+    scene = to_gpu(many_splats())
+    for p in (gh.RenderParams(spp_side=2, background=BG),
+              gh.RenderParams(spp_side=2, background=BG, method=2, max_points_per_splat=1 << 24,
+                              pbvr=dict(radial_correction=1))):
+        say(f"[70k] method={p.method}")
+        gpu, m = run_pair(scene, None, 0, CAM, p, W, H, 1, 3)
+        assert_exact(gpu, m)
+
+
+def test_small_uniform_extinction_c3_16_ensembles():
+    """4,096-Gaussian 256x256 PBVR Extinction C3 workload: 16 ensembles, seeds 1..16."""
+    say("[small uniform] 16 ensembles")
+    scene = S.small_uniform()
+    cam = S.small_uniform_camera()
+    p = gh.RenderParams(spp_side=1, method=2, max_points_per_splat=16384)
+    gpu, m = run_pair(scene, None, 0, cam, p, 256, 256, 1, 16)
+    assert_exact(gpu, m)
+    # batched (no debug hook) must equal the per-ensemble run
+    r = gh.GpsRenderer(256, 256, 1)
+    r.set_gaussians(scene)
+    r.render_ensembles(cam, p, 1, 16)
+    acc = r.read_accum()
+    r.close()
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+
+
+def test_overflow_skip_exact():
+    scene = phase_scenes()[3][1]
+    p = gh.RenderParams(spp_side=2, background=BG, density_scale=1.0e7, max_points_per_splat=0xFFFFFFFF)
+    say("[overflow]")
+    gpu, m = run_pair(to_gpu(scene), None, 0, CAM, p, 16, 16, 1, 1)
+    assert m["skipped"] == 1 and gpu["bufs"]["skipped"] == 1
+    assert_exact(gpu, m)
+
+
+def test_offscreen_camera_is_background():
+    """verify.rs:587-591"""
+    scene = to_gpu(phase_scenes()[3][1])
+    cam = gh.default_camera(200.0)
+    cam.view_pos[:] = [1000.0, 0.0, 0.0]
+    p = gh.RenderParams(spp_side=2, background=BG, method=2, pbvr=dict(radial_correction=1))
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians(scene)
+    r.render_ensembles(cam, p, 800, 2)
+    acc = r.read_accum()
+    r.close()
+    assert (acc[..., 3] == 2.0).all()
+    for k in range(3):
+        assert np.abs(acc[..., k] - 2.0 * np.float32(BG[k])).max() < 1e-6
+
+
+def test_history_and_batched_vs_separate():
+    """verify.rs:212-223, 572-605: counts, reset, batched == separate, determinism."""
+    scene = to_gpu(phase_scenes()[3][1])
+    p = gh.RenderParams(spp_side=2, background=BG, method=2, max_points_per_splat=1 << 24,
+                        pbvr=dict(radial_correction=1))
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians(scene)
+    r.render_ensembles(CAM, p, 700, 8)
+    expected = r.read_accum()
+    assert (expected[..., 3] == 8.0).all() and r.accumulated() == 8
+    r.reset_accum()
+    assert (r.read_accum() == 0).all() and r.accumulated() == 0
+    for s in range(700, 708):
+        r.render_ensembles(CAM, p, s, 1)
+    assert np.array_equal(r.read_accum(), expected)
+    r.reset_accum()
+    r.render_ensembles(CAM, p, 700, 4)
+    r.render_ensembles(CAM, p, 704, 4)
+    assert np.array_equal(r.read_accum(), expected)
+    r.reset_accum()
+    r.render_ensembles(CAM, p, 600, 8)
+    assert not np.array_equal(r.read_accum(), expected)
+    r.close()
+
+
+def test_composite_bit_exact():
+    scene = to_gpu(phase_scenes()[1][1])
+    p = gh.RenderParams(spp_side=2, background=BG)
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians(scene)
+    r.render_ensembles(CAM, p, 1, 16)
+    acc = r.read_accum()
+    for ow, oh, gamma, expo in ((W, H, 1, 1.0), (2 * W, 2 * H, 1, 1.0), (W, H, 0, 1.7), (50, 70, 1, 0.8)):
+        q = p.copy(srgb_output=gamma, exposure=expo)
+        img = r.composite(q, ow, oh)
+        ref = O.mirror_composite(acc, expo, gamma, ow, oh)
+        assert np.array_equal(img, ref), (ow, oh, gamma, expo)
+    r.close()
+
+
+def test_empty_scene_renders_background():
+    r = gh.GpsRenderer(32, 32, 2)
+    p = gh.RenderParams(spp_side=2, background=BG)
+    r.render_ensembles(CAM, p, 1, 3)
+    acc = r.read_accum()
+    r.close()
+    assert (acc[..., 3] == 3.0).all()
+    assert np.abs(acc[..., 0] - 3 * np.float32(BG[0])).max() < 1e-6
+
+
+def test_errors_are_reported_not_fatal():
+    r = gh.GpsRenderer(32, 32, 2)
+    with pytest.raises(gh.GpsError):
+        r.render_ensembles(CAM, gh.RenderParams(spp_side=1), 1, 1)
+    with pytest.raises(gh.GpsError):
+        r.set_gaussians_sh(to_gpu(phase_scenes()[1][1]), np.zeros(5, np.float32), 3)
+    r.render_ensembles(CAM, gh.RenderParams(spp_side=2), 1, 1)
+    r.close()
+    with pytest.raises(gh.GpsError):
+        gh.GpsRenderer(0, 10, 1)
+
+
+# ------------------------------------------------------------------ sorted path (verify.rs:751-890)
+def sorted_scenes():
+    out = [(n, s) for n, s in phase_scenes()]
+    out.append(("crowd of 3000", crowd()))
+    return out
+
+
+@pytest.mark.parametrize("idx", range(6))
+def test_sorted_path_vs_mirror_and_analytic(idx):
+    name, scene = sorted_scenes()[idx]
+    say(f"[sorted] {name}")
+    arr = to_gpu(scene)
+    p = gh.RenderParams(spp_side=2, background=BG)
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians(arr)
+    r.render_sorted(CAM, p, 1)
+    acc = r.read_accum()
+    keys, vals = r.read_sorted()
+    st = r.read_stats()
+    assert r.accumulated() == 1
+    r.close()
+    m = O.mirror_sorted(arr, None, 0, ocam(CAM), mparams(p), W, H)
+    assert np.array_equal(keys, m["keys"]) and np.array_equal(vals, m["vals"])
+    assert st.probe_demand == int(m["stats"][9])
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+    img = acc[..., :3].astype(np.float64)
+    ana = O.render_analytic(scene, ocam(CAM), W, H, BG)
+    assert O.psnr(img, ana) >= 40.0 and O.ssim(img, ana) >= 0.99 and (acc[..., 3] == 1.0).all()
+    assert (np.diff(keys.astype(np.int64)) >= 0).all()
+    assert np.array_equal(np.sort(vals), np.arange(len(vals)))
+
+
+def test_radix_sort_300k_multilevel():
+    """verify.rs:827-853: 300k small splats."""
+    say("[sorted] 300k")
+    rng = np.random.default_rng(11)
+    n = 300_000
+    scene = np.zeros(n, gh.GAUSSIAN_DTYPE)
+    scene["pos_opacity"][:, 0] = rng.uniform(-1.3, 1.3, n)
+    scene["pos_opacity"][:, 1] = rng.uniform(-1.3, 1.3, n)
+    scene["pos_opacity"][:, 2] = rng.uniform(4.0, 9.0, n)
+    scene["pos_opacity"][:, 3] = 0.5
+    scene["log_scale"][:, :3] = math.log(0.01)
+    scene["rot_wxyz"][:, 0] = 1.0
+    scene["color"][:, :3] = rng.uniform(0, 1, (n, 3))
+    p = gh.RenderParams(spp_side=2, background=BG)
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians(scene)
+    r.render_sorted(CAM, p, 1)
+    keys, vals = r.read_sorted()
+    acc = r.read_accum()
+    r.close()
+    m = O.mirror_sorted(scene, None, 0, ocam(CAM), mparams(p), W, H)
+    assert np.array_equal(keys, m["keys"]) and np.array_equal(vals, m["vals"])
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+
+
+def test_demand_probe_matches_gps_points():
+    """verify.rs:855-877: probe within 2 % of the GPS path's mean points per ensemble."""
+    arr = to_gpu(crowd())
+    p = gh.RenderParams(spp_side=2, background=BG, max_points_per_splat=1 << 24)
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians(arr)
+    r.reset_stats()
+    r.probe_demand(CAM, p)
+    probe = r.read_stats().probe_demand
+    r.reset_stats()
+    r.render_ensembles(CAM, p, 1, 40)
+    pts = r.read_stats().points / 40.0
+    r.close()
+    assert pts > 1000.0 and abs(probe - pts) / pts < 0.02
+
+
+def test_sorted_after_resize():
+    arr = to_gpu(crowd())
+    p = gh.RenderParams(spp_side=2, background=BG)
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians(arr)
+    r.render_sorted(CAM, p, 1)
+    r.resize(64, 64, 2)
+    r.set_gaussians(arr)
+    r.render_sorted(CAM, p, 1)
+    assert r.read_accum().shape == (64, 64, 4)
+    r.close()
