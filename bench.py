@@ -53,6 +53,9 @@ def main():
     ap.add_argument("--ensembles", type=int, default=64)
     ap.add_argument("--cpu-fraction", type=int, default=32, help="CPU baseline uses 1/F of the splats")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", default="bench", choices=["bench", "calib"],
+                    help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
@@ -63,6 +66,13 @@ def main():
 
     import gps_hip as gh
     from gps_hip import scenes as S
+
+    if a.mode == "calib":
+        x = np.ones(64 << 20, np.uint32)
+        y = gh.scan_exclusive_u32(x)
+        assert int(y[-1]) == (64 << 20) - 1
+        print(json.dumps({"calib_bytes_read": 4 * x.size, "calib_bytes_written": 4 * x.size}), flush=True)
+        return
 
     W, H = 1280, 720
     g, sh = S.clustered(560_000, seed=1, sh_degree=3)

@@ -2,9 +2,9 @@
 //
 // The WGSL builtins exp/log/sin/cos that shaders/gps_splat.wgsl relies on are
 // driver approximations with no fixed bits.  Here they are one fixed
-// algorithm built only from IEEE-754 binary32 +, -, *, / and sqrt, which the
-// device evaluates with correct rounding (compiled with -ffp-contract=off and
-// -fhip-fp32-correctly-rounded-divide-sqrt).  A CPU evaluation of the same
+// algorithm built only from IEEE-754 binary32 +, -, *, /, sqrt and fused
+// multiply-add, which the device evaluates with correct rounding (compiled
+// with -ffp-contract=off and -fhip-fp32-correctly-rounded-divide-sqrt).  A CPU evaluation of the same
 // steps therefore reproduces every result bit for bit.
 //
 //   exp: Cody-Waite reduction x = k ln2 + r (ln2 in two parts, the high part
@@ -14,6 +14,8 @@
 //        s = (m-1)/(m+1), odd series through s^9, plus e ln2 in two parts.
 //   sin/cos: reduction by pi/2 with a three-part constant (exact products for
 //        |k| < 2^11), Taylor polynomials of degree 9 / 10 on |r| <= pi/4.
+//   Every Horner step and reduction step is one explicit fused multiply-add
+//   (fmaf: a single correct rounding, the same on the CPU and on the GPU).
 //
 // Accuracy: <= 2 ulp against correctly rounded results over the argument
 // ranges the pipeline uses (sin/cos arguments are 2 pi u, u in [0, 1)).
@@ -38,6 +40,7 @@ GPS_DEV uint32_t bits(float x) { return __float_as_uint(x); }
 GPS_DEV float fbits(uint32_t u) { return __uint_as_float(u); }
 GPS_DEV float rint(float x) { return __builtin_rintf(x); }
 GPS_DEV float sqrt(float x) { return __builtin_sqrtf(x); }
+GPS_DEV float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 GPS_DEV float exp(float x) {
     if (x != x) return x;
@@ -45,15 +48,16 @@ GPS_DEV float exp(float x) {
     if (x < -103.972084045410156f) return 0.0f;
     float kf = rint(x * LOG2E);
     int k = (int)kf;
-    float r = (x - kf * LN2_HI) - kf * LN2_LO;
+    float r = fma(-kf, LN2_HI, x);
+    r = fma(-kf, LN2_LO, r);
     float p = 0x1.a01a02p-13f;
-    p = 0x1.6c16c2p-10f + r * p;
-    p = 0x1.111112p-7f + r * p;
-    p = 0x1.555556p-5f + r * p;
-    p = 0x1.555556p-3f + r * p;
-    p = 0.5f + r * p;
-    p = 1.0f + r * p;
-    p = 1.0f + r * p;
+    p = fma(p, r, 0x1.6c16c2p-10f);
+    p = fma(p, r, 0x1.111112p-7f);
+    p = fma(p, r, 0x1.555556p-5f);
+    p = fma(p, r, 0x1.555556p-3f);
+    p = fma(p, r, 0.5f);
+    p = fma(p, r, 1.0f);
+    p = fma(p, r, 1.0f);
     if (k > 127) return p * fbits(0x7f000000u) * fbits((uint32_t)(k - 127 + 127) << 23);
     if (k < -126) return p * fbits((uint32_t)(k + 64 + 127) << 23) * 0x1p-64f;
     return p * fbits((uint32_t)(k + 127) << 23);
@@ -81,37 +85,39 @@ GPS_DEV float log(float x) {
     float s = f / (2.0f + f);
     float z = s * s;
     float q = 0x1.c71c72p-4f;
-    q = 0x1.24924ap-3f + z * q;
-    q = 0x1.99999ap-3f + z * q;
-    q = 0x1.555556p-2f + z * q;
+    q = fma(q, z, 0x1.24924ap-3f);
+    q = fma(q, z, 0x1.99999ap-3f);
+    q = fma(q, z, 0x1.555556p-2f);
     float s2 = s + s;
-    float lm = s2 + s2 * (z * q);
+    float lm = fma(s2, z * q, s2);
     float ef = (float)e;
-    return ef * LN2_HI + (ef * LN2_LO + lm);
+    return fma(ef, LN2_HI, fma(ef, LN2_LO, lm));
 }
 
 GPS_DEV float sin_poly(float r) {
     float z = r * r;
     float p = 0x1.71de3ap-19f;
-    p = -0x1.a01a02p-13f + z * p;
-    p = 0x1.111112p-7f + z * p;
-    p = -0x1.555556p-3f + z * p;
-    return r + r * (z * p);
+    p = fma(p, z, -0x1.a01a02p-13f);
+    p = fma(p, z, 0x1.111112p-7f);
+    p = fma(p, z, -0x1.555556p-3f);
+    return fma(r, z * p, r);
 }
 
 GPS_DEV float cos_poly(float r) {
     float z = r * r;
     float p = -0x1.27e4fcp-22f;
-    p = 0x1.a01a02p-16f + z * p;
-    p = -0x1.6c16c2p-10f + z * p;
-    p = 0x1.555556p-5f + z * p;
-    p = -0.5f + z * p;
-    return 1.0f + z * p;
+    p = fma(p, z, 0x1.a01a02p-16f);
+    p = fma(p, z, -0x1.6c16c2p-10f);
+    p = fma(p, z, 0x1.555556p-5f);
+    p = fma(p, z, -0.5f);
+    return fma(z, p, 1.0f);
 }
 
 GPS_DEV float reduce(float x, int &q) {
     float kf = rint(x * TWO_OVER_PI);
-    float r = ((x - kf * PIO2_1) - kf * PIO2_2) - kf * PIO2_3;
+    float r = fma(-kf, PIO2_1, x);
+    r = fma(-kf, PIO2_2, r);
+    r = fma(-kf, PIO2_3, r);
     q = ((int)kf) & 3;
     return r;
 }

@@ -14,7 +14,7 @@ struct GpsFrame {
     uint32_t cap, sh_stride, exact_radius, sh_degree;
     uint32_t method, calib, flags, pad0;
     float density, near_z, base_k, ref_len;
-    float object_depth, pad1, pad2, pad3;
+    float object_depth, jitter, pad2, pad3;   // jitter = sqrt(max(pos.w, 0)), the same for every particle
     float bg[4];
 };
 

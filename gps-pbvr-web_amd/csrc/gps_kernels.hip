@@ -47,6 +47,13 @@ GPS_DEV uint32_t particle_seed(uint32_t splat, uint32_t k, uint32_t frame) {
     return s ? s : 1u;
 }
 
+// particle_seed with the ensemble term frame * 2654435761 + 1 computed once per ensemble
+GPS_DEV uint32_t particle_seed_mixed(uint32_t splat, uint32_t k, uint32_t frame_mix) {
+    uint32_t s = pcg(splat * 747796405u + k + 1u);
+    s = pcg(s ^ frame_mix);
+    return s ? s : 1u;
+}
+
 struct Rng {
     uint32_t s;
     float spare;
@@ -612,9 +619,8 @@ GPS_DEV uint32_t emit_pbvr(const GpsFrame &f, Rng &r, uint32_t id, const float4 
     if (f.flags & FLAG_JITTER) {
         float jy = r.normal();
         float jx = r.normal();
-        float s = gpsm::sqrt(fmax(f.pos[3], 0.0f));
-        pixx = pixx + s * jx;
-        pixy = pixy + s * jy;
+        pixx = pixx + f.jitter * jx;
+        pixy = pixy + f.jitter * jy;
     }
     if (f.flags & FLAG_RADIAL) {
         float4 d = v[3];
@@ -700,7 +706,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
         const uint32_t pend = min(p0 + CH, total);
         const uint32_t *offs = offsets + (size_t)b * N;
         uint64_t *dep = depth + (size_t)b * sub_count;
-        const uint32_t seed = first_seed + b;
+        const uint32_t seed_mix = (first_seed + b) * 2654435761u + 1u;
         if (threadIdx.x < 2u) {
             uint32_t s = owner_search(offs, 0u, N - 1u, threadIdx.x == 0 ? p0 : pend - 1u);
             if (threadIdx.x == 0) s_lo = s; else s_hi = s;
@@ -722,7 +728,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
                 }
                 const uint32_t k = p - cur;
                 const float4 *v = view + (size_t)VIEW_VEC4 * i;
-                Rng r(particle_seed(i, k + 1u, seed));
+                Rng r(particle_seed_mixed(i, k + 1u, seed_mix));
                 if (MK == 0) {
                     emit_gps(f, r, i, v[0], v[1], lut, dep);
                 } else if (MK == 1) {

@@ -240,6 +240,7 @@ void pack_frame(const GpsRenderer *r, const GpsCamera *cam, const GpsRenderParam
     }
     for (int k = 0; k < 3; k++) f.pos[k] = (float)cam->view_pos[k];
     f.pos[3] = (float)cam->low_pass;
+    f.jitter = std::sqrt(f.pos[3] > 0.0f ? f.pos[3] : 0.0f);   // IEEE f32 sqrt, the kernels' value bit for bit
     double cx = cam->cx, cy = cam->cy;
     if (cx == 0.0 && cy == 0.0) {
         cx = 0.5 * (double)r->W;

@@ -4,9 +4,9 @@
  * The WGSL builtins exp/log/sin/cos (shaders/gps_splat.wgsl) have no bit-exact
  * definition: each WebGPU driver supplies its own approximation.  The HIP
  * kernels therefore evaluate these functions with one fixed algorithm built
- * only from IEEE-754 binary32 +, -, *, / and sqrt (all correctly rounded on
- * x86-64 SSE and on gfx950 with -ffp-contract=off), so that the oracle can
- * reproduce every bit.  This file is the oracle's own statement of that
+ * only from IEEE-754 binary32 +, -, *, /, sqrt and fused multiply-add (all
+ * correctly rounded on x86-64 and on gfx950 with -ffp-contract=off), so that
+ * the oracle can reproduce every bit.  This file is the oracle's own statement of that
  * algorithm:
  *
  *   exp: Cody & Waite argument reduction x = k ln2 + r with ln2 split in two
@@ -21,13 +21,17 @@
  *        quadrant selection.  Accurate for |x| < 2^14, which covers every
  *        argument the pipeline produces (2 pi u with u in [0,1)).
  *
- * Accuracy against libm is pinned by tests/test_oracle_math.py.
+ * Every Horner step and reduction step is one explicit fused multiply-add
+ * (fmaf, correctly rounded on both the CPU and the GPU).
+ *
+ * Accuracy against libm is pinned by tests/test_oracle_mirror.py.
  * Compile with -ffp-contract=off (no FMA contraction) and without -ffast-math.
  */
 #ifndef GPS_ORACLE_F32MATH_H
 #define GPS_ORACLE_F32MATH_H
 
 #include <stdint.h>
+#include <math.h>
 #include <string.h>
 
 static inline uint32_t om_bits(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
@@ -43,6 +47,7 @@ static inline float om_float(uint32_t u) { float x; memcpy(&x, &u, 4); return x;
 #define OM_SQRT2 0x1.6a09e6p+0f
 
 static inline float om_rint(float x) { return __builtin_rintf(x); }
+static inline float om_fma(float a, float b, float c) { return fmaf(a, b, c); }
 
 static inline float om_expf(float x) {
     if (x != x) return x;
@@ -50,15 +55,16 @@ static inline float om_expf(float x) {
     if (x < -103.972084045410156f) return 0.0f;
     float kf = om_rint(x * OM_LOG2E);
     int k = (int)kf;
-    float r = (x - kf * OM_LN2_HI) - kf * OM_LN2_LO;
+    float r = om_fma(-kf, OM_LN2_HI, x);
+    r = om_fma(-kf, OM_LN2_LO, r);
     float p = 0x1.a01a02p-13f;                  /* 1/7! */
-    p = 0x1.6c16c2p-10f + r * p;                /* 1/6! */
-    p = 0x1.111112p-7f + r * p;                 /* 1/5! */
-    p = 0x1.555556p-5f + r * p;                 /* 1/4! */
-    p = 0x1.555556p-3f + r * p;                 /* 1/3! */
-    p = 0.5f + r * p;
-    p = 1.0f + r * p;
-    p = 1.0f + r * p;
+    p = om_fma(p, r, 0x1.6c16c2p-10f);          /* 1/6! */
+    p = om_fma(p, r, 0x1.111112p-7f);           /* 1/5! */
+    p = om_fma(p, r, 0x1.555556p-5f);           /* 1/4! */
+    p = om_fma(p, r, 0x1.555556p-3f);           /* 1/3! */
+    p = om_fma(p, r, 0.5f);
+    p = om_fma(p, r, 1.0f);
+    p = om_fma(p, r, 1.0f);
     if (k > 127) return p * om_float(0x7f000000u) * om_float((uint32_t)(k - 127 + 127) << 23);
     if (k < -126) return p * om_float((uint32_t)(k + 64 + 127) << 23) * 0x1p-64f;
     return p * om_float((uint32_t)(k + 127) << 23);
@@ -83,38 +89,40 @@ static inline float om_logf(float x) {
     float s = f / (2.0f + f);
     float z = s * s;
     float q = 0x1.c71c72p-4f;                   /* 1/9 */
-    q = 0x1.24924ap-3f + z * q;                 /* 1/7 */
-    q = 0x1.99999ap-3f + z * q;                 /* 1/5 */
-    q = 0x1.555556p-2f + z * q;                 /* 1/3 */
+    q = om_fma(q, z, 0x1.24924ap-3f);           /* 1/7 */
+    q = om_fma(q, z, 0x1.99999ap-3f);           /* 1/5 */
+    q = om_fma(q, z, 0x1.555556p-2f);           /* 1/3 */
     float s2 = s + s;
-    float lm = s2 + s2 * (z * q);
+    float lm = om_fma(s2, z * q, s2);
     float ef = (float)e;
-    return ef * OM_LN2_HI + (ef * OM_LN2_LO + lm);
+    return om_fma(ef, OM_LN2_HI, om_fma(ef, OM_LN2_LO, lm));
 }
 
 static inline float om_sin_poly(float r) {
     float z = r * r;
     float p = 0x1.71de3ap-19f;                  /* 1/9! */
-    p = -0x1.a01a02p-13f + z * p;               /* -1/7! */
-    p = 0x1.111112p-7f + z * p;                 /* 1/5! */
-    p = -0x1.555556p-3f + z * p;                /* -1/3! */
-    return r + r * (z * p);
+    p = om_fma(p, z, -0x1.a01a02p-13f);         /* -1/7! */
+    p = om_fma(p, z, 0x1.111112p-7f);           /* 1/5! */
+    p = om_fma(p, z, -0x1.555556p-3f);          /* -1/3! */
+    return om_fma(r, z * p, r);
 }
 
 static inline float om_cos_poly(float r) {
     float z = r * r;
     float p = -0x1.27e4fcp-22f;                 /* -1/10! */
-    p = 0x1.a01a02p-16f + z * p;                /* 1/8! */
-    p = -0x1.6c16c2p-10f + z * p;               /* -1/6! */
-    p = 0x1.555556p-5f + z * p;                 /* 1/4! */
-    p = -0.5f + z * p;
-    return 1.0f + z * p;
+    p = om_fma(p, z, 0x1.a01a02p-16f);          /* 1/8! */
+    p = om_fma(p, z, -0x1.6c16c2p-10f);         /* -1/6! */
+    p = om_fma(p, z, 0x1.555556p-5f);           /* 1/4! */
+    p = om_fma(p, z, -0.5f);
+    return om_fma(z, p, 1.0f);
 }
 
 /* quadrant k (mod 4) and reduced argument r of x = k pi/2 + r */
 static inline float om_reduce(float x, int *quadrant) {
     float kf = om_rint(x * OM_TWO_OVER_PI);
-    float r = ((x - kf * OM_PIO2_1) - kf * OM_PIO2_2) - kf * OM_PIO2_3;
+    float r = om_fma(-kf, OM_PIO2_1, x);
+    r = om_fma(-kf, OM_PIO2_2, r);
+    r = om_fma(-kf, OM_PIO2_3, r);
     *quadrant = ((int)kf) & 3;
     return r;
 }
