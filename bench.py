@@ -13,10 +13,13 @@ times the CPU oracle's f32 restatement of the same pipeline (single thread) on a
 import argparse
 import ctypes
 import json
+import math
 import os
 import subprocess
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -43,6 +46,154 @@ def kernel_bytes(name, n, n_vis, sh_floats, particles, S, X, ens):
     return 0
 
 
+def psnr(a, b):
+    """crates/gps-core/src/oracle.rs:503-514: MSE over all channels, peak 1, capped at 120 dB."""
+    d = (a.astype(np.float64) - b.astype(np.float64)) ** 2
+    mse = float(d.mean())
+    return 120.0 if mse <= 1e-20 else 10.0 * math.log10(1.0 / mse)
+
+
+def time_to_psnr(r, cam, p, ms_per_ens, target_db=40.0, ref_ensembles=1024, ref_seed=1_000_000):
+    """Plain ensemble averaging: smallest ensemble count whose mean reaches target_db against an independent
+    ref_ensembles-ensemble render of the same view, and the GPU time that count costs."""
+    r.reset_accum()
+    r.render_ensembles(cam, p, ref_seed, ref_ensembles)
+    acc = r.read_accum()
+    ref = acc[..., :3] / np.maximum(acc[..., 3:4], 1.0)
+    r.reset_accum()
+    done, curve = 0, []
+    for e in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 768, 1024, 1536, 2048, 2560, 3072, 4096):
+        r.render_ensembles(cam, p, 1 + done, e - done)
+        done = e
+        acc = r.read_accum()
+        img = acc[..., :3] / np.maximum(acc[..., 3:4], 1.0)
+        q = psnr(img, ref)
+        curve.append((e, round(q, 2)))
+        if q >= target_db:
+            return {"ensembles": e, "gpu_ms": round(e * ms_per_ens, 3), "psnr_db": round(q, 2), "curve": curve}
+    return {"ensembles": None, "gpu_ms": None, "psnr_db": curve[-1][1], "curve": curve}
+
+
+def bench_sort3m(a):
+    """3,000,000 clustered Gaussians (seed 2, SH3), 1920x1080, SPP side 2, GPS, cap 65,536: the packed-minimum
+    resolve and the depth-sorted resolve (64-bit onesweep sort of every particle) per view."""
+    import gps_hip as gh
+    from gps_hip import scenes as S
+    W, H = 1920, 1080
+    g, sh = S.clustered(3_000_000, seed=2, sh_degree=3)
+    r = gh.GpsRenderer(W, H, 2)
+    r.set_gaussians_sh(g, sh, 3)
+    out = {}
+    for v in [x for x in a.views.split(",") if x]:
+        cam = S.clustered_camera(v, W, H)
+        res = {}
+        for path in (1, 2):
+            p = gh.RenderParams(spp_side=2, method=gh.Method.Gps, max_points_per_splat=65536, path=path)
+            ens = max(1, min(a.ensembles, 8))
+            r.reset_accum()
+            r.render_ensembles(cam, p, 1, 1)
+            r.wait_idle()
+            r.reset_stats()
+            r.profile(True)
+            r.render_ensembles(cam, p, 100, ens)
+            r.wait_idle()
+            prof = r.profile_read()
+            r.profile(False)
+            st = r.read_stats()
+            total = sum(t for _, t in prof.values())
+            parts = (st.points - st.dropped_points) / ens
+            d = {"ms_per_ensemble": round(total / ens, 3), "particles_per_ensemble": parts,
+                 "skipped": st.skipped_ensembles,
+                 "kernels": {k: {"launches": c, "avg_ms": round(t / c, 4)} for k, (c, t) in prof.items()}}
+            if path == 2 and "sort_pass" in prof:
+                c, t = prof["sort_pass"]
+                keys = parts * ens
+                d["sort_passes_per_ensemble"] = c / ens
+                d["sort_pass_GBps"] = round(24.0 * keys * (c / ens) / (t * 1e-3) / 1e9, 1)
+                d["sort_keys_per_s"] = round(keys / ((t + prof.get("sort_hist", (0, 0))[1]) * 1e-3), 1)
+            res["atomic" if path == 1 else "sorted"] = d
+            log(f"[sort3m {v} path={path}] {d}")
+        out[v] = res
+    print(json.dumps({"mode": "sort3m", "workload": "clustered 3M Gaussians SH3 seed 2, 1920x1080, SPP side 2, GPS, "
+                      "cap 65536", "views": out}), flush=True)
+    r.close()
+
+
+BATCH_SIZES = (20_000, 50_000, 100_000, 250_000, 560_000, 1_000_000, 1_500_000, 2_000_000)
+BATCH_RES = ((640, 360), (1280, 720), (1920, 1080))
+
+
+def bench_batch8(a):
+    """Eight clustered scenes (seeds 10-17, SH3) of uneven size; scene i renders at BATCH_RES[i % 3]; each
+    scene gets a 16-view 360-degree orbit (radius 6, elevation 15 degrees) with 32 ensembles per view of PBVR
+    Extinction C3+R (SPP 1, cap 16384).  All 128 views are recorded into one hipGraph and launched once.
+    Compared with the same items launched straight onto the stream and with one submission per ensemble
+    (one render_ensembles call per ensemble, the reference's trajectory-export pattern)."""
+    import gps_hip as gh
+    from gps_hip import scenes as S
+    views, ens = 16, 32
+    p = gh.RenderParams(spp_side=1, method=gh.Method.Extinction, max_points_per_splat=16384,
+                        pbvr=dict(calibration=3, radial_correction=1))
+    rends, items, total_g = [], [], 0
+    for i, n in enumerate(BATCH_SIZES):
+        w, h = BATCH_RES[i % 3]
+        g, sh = S.clustered(n, seed=10 + i, sh_degree=3)
+        r = gh.GpsRenderer(w, h, 1)
+        r.set_gaussians_sh(g, sh, 3)
+        rends.append(r)
+        for v in range(views):
+            cam = gh.orbit_camera((0.0, 0.0, 0.0), 6.0, 2.0 * math.pi * v / views, math.radians(15.0),
+                                  (0.0, 0.0, 1.0), S.FOV_60, w, h)
+            items.append((r, cam, p, 1 + 1000 * v, ens))
+        total_g += n * views * ens
+        log(f"[batch8] scene {i}: {n} Gaussians at {w}x{h}")
+    tuning = gh.get_tuning()
+    if not a.no_autotune:
+        tuning, tms = rends[4].autotune(items[4 * views][1], p, trial_ensembles=4)
+        log(f"[batch8 autotune] {tuning} {tms:.3f} ms/ens")
+    b = gh.Batch()
+    out = {}
+    for leg, use_graph in (("graph", True), ("stream", False)):
+        for _ in range(max(a.warmup, 1)):
+            b.render(items, use_graph=use_graph)
+        gms, rec = [], []
+        for _ in range(max(a.steps, 1)):
+            b.render(items, use_graph=use_graph)
+            t = b.timing()
+            gms.append(t["gpu_ms"])
+            rec.append(t["record_ms"])
+        out[leg] = {"gpu_ms": round(min(gms), 2), "record_ms": round(min(rec), 2), "graph_nodes": t["graph_nodes"]}
+        log(f"[batch8 {leg}] {out[leg]}")
+    last = [len(items) - 1 - k * views for k in range(len(BATCH_SIZES))][::-1]
+    snap = {i: b.read(i) for i in last}
+    for r in rends:
+        r.wait_idle()
+    t0 = time.perf_counter()
+    for r, cam, q, seed, count in items:
+        r.reset_accum()
+        for e in range(count):
+            r.render_ensembles(cam, q, seed + e, 1)
+    for r in rends:
+        r.wait_idle()
+    per_ens_ms = 1000.0 * (time.perf_counter() - t0)
+    exact = all(np.array_equal(snap[i].view(np.uint32), items[i][0].read_accum().view(np.uint32)) for i in last)
+    out["per_ensemble_submits"] = {"wall_ms": round(per_ens_ms, 2), "bit_identical_to_graph": bool(exact)}
+    log(f"[batch8 per-ensemble] {out['per_ensemble_submits']}")
+    gm = out["graph"]["gpu_ms"]
+    print(json.dumps({
+        "mode": "batch8",
+        "workload": "8 clustered scenes (seeds 10-17; 20k..2M Gaussians, SH3) x 16-view orbit (r 6, elev 15 deg), "
+                    "640x360/1280x720/1920x1080 cycling per scene, PBVR Extinction C3+R, SPP 1, cap 16384, "
+                    "32 ensembles per view",
+        "views": len(items), "ensembles": len(items) * ens, "legs": out,
+        "ms_per_view": round(gm / len(items), 3), "ensembles_per_s": round(len(items) * ens / (gm * 1e-3), 1),
+        "gaussian_ensembles_per_s": round(total_g / (gm * 1e-3), 1),
+        "graph_speedup_vs_per_ensemble_submits": round(per_ens_ms / gm, 3), "tuning": tuning}), flush=True)
+    b.close()
+    for r in rends:
+        r.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,7 +204,9 @@ def main():
     ap.add_argument("--ensembles", type=int, default=64)
     ap.add_argument("--cpu-fraction", type=int, default=32, help="CPU baseline uses 1/F of the splats")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", default="bench", choices=["bench", "calib"],
+    ap.add_argument("--no-psnr", action="store_true", help="skip the time-to-40-dB measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
+    ap.add_argument("--mode", default="bench", choices=["bench", "calib", "sort3m", "batch8"],
                     help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
@@ -61,8 +214,6 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
 
     import gps_hip as gh
     from gps_hip import scenes as S
@@ -74,6 +225,11 @@ def main():
         print(json.dumps({"calib_bytes_read": 4 * x.size, "calib_bytes_written": 4 * x.size}), flush=True)
         return
 
+    if a.mode == "sort3m":
+        return bench_sort3m(a)
+    if a.mode == "batch8":
+        return bench_batch8(a)
+
     W, H = 1280, 720
     g, sh = S.clustered(560_000, seed=1, sh_degree=3)
     n = len(g)
@@ -84,6 +240,10 @@ def main():
     views = [v for v in a.views.split(",") if v]
     if a.view not in views:
         views.append(a.view)
+    tuning = gh.get_tuning()
+    if not a.no_autotune:
+        tuning, tms = r.autotune(S.clustered_camera(a.view, W, H), p, trial_ensembles=4)
+        log(f"[autotune] {tuning} {tms:.3f} ms/ens")
     per_view = {}
     prof_view = {}
     for v in views:
@@ -135,6 +295,12 @@ def main():
         kernels[k] = {"launches": c, "avg_ms": round(t / c, 4),
                       "GBps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 and b else None}
 
+    ttp = None
+    if not a.no_psnr:
+        ttp = {"plain": time_to_psnr(r, S.clustered_camera(head, W, H), p, per_view[head]["ms_per_ensemble"]),
+               "denoised": None}
+        log(f"[psnr] {ttp}")
+
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
@@ -181,6 +347,8 @@ def main():
             "kernels": kernels,
             "gaussians_per_s": round(n * a.ensembles / (v["ms_per_ensemble"] * a.ensembles * 1e-3), 1),
             "particles_per_s": round(v["particles_per_ensemble"] / (v["ms_per_ensemble"] * 1e-3), 1),
+            "time_to_40db": ttp,
+            "tuning": tuning,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -40,6 +40,15 @@ constexpr int PART_BLOCK = 256;
 constexpr int MAX_BATCH = 64;         // ensembles per batch (one depth|id buffer each)
 constexpr uint64_t EMPTY64 = ~0ull;
 
+// process-wide launch tuning (include/gps_hip.h GpsTuning); results never depend on it
+struct GpsTuningDev {
+    uint32_t particle_waves_per_cu;   // persistent particle grid = CUs * this
+    uint32_t particles_per_thread;    // 1, 2, 4 or 8
+    uint32_t sort_items_per_thread;   // 8 or 16
+    uint32_t batch_ensembles;         // 0 = as many as memory allows (<= 64)
+};
+GpsTuningDev gps_tuning_current();
+
 struct BatchState {
     uint32_t *tickets;      // [MAX_BATCH] scan tile tickets (reset by the last tile)
     uint64_t *status;       // [MAX_BATCH * tiles] decoupled look-back words
@@ -54,7 +63,8 @@ void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uin
                        uint32_t *offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch, uint64_t *stats);
 void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
                       const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
-                      const float *lut, uint64_t *stats, int grid);
+                      const float *lut, uint64_t *stats, int grid, uint64_t *pkeys = nullptr,
+                      uint32_t *pidx = nullptr, int ppt = 4);
 void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
                     uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats);
 void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
@@ -66,6 +76,7 @@ void launch_zero_u32(hipStream_t s, uint32_t *p, uint64_t n);
 void launch_stats_add(hipStream_t s, uint64_t *stats, int slot, uint64_t v);
 void launch_stats_clear(hipStream_t s, uint64_t *stats, int first, int count);
 void launch_accum_clear(hipStream_t s, float4 *accum, uint64_t n);
+void launch_copy_f4(hipStream_t s, float4 *dst, const float4 *src, uint64_t n);
 void launch_phase0(hipStream_t s, uint32_t seed, uint32_t pixels, uint32_t samples, uint64_t *buf);
 void launch_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ticket, uint64_t *status,
                      uint32_t tiles, uint32_t epoch);

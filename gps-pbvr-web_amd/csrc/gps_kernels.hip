@@ -329,9 +329,6 @@ GPS_DEV void write_min(uint64_t *__restrict__ d, uint64_t v) {
     if (v < seen) __hip_atomic_fetch_min(d, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-GPS_DEV uint64_t pack(float depth, uint32_t id) {
-    return ((uint64_t)(gpsm::bits(depth) | 0x80000000u) << 32) | (uint64_t)id;
-}
 
 // ------------------------------------------------------------------ block reductions
 template <typename T>
@@ -577,9 +574,17 @@ GPS_DEV uint32_t owner_search(const uint32_t *__restrict__ offs, uint32_t lo, ui
     return lo;
 }
 
+// A generated particle: the subpixel it writes (NO_HIT if none) and its ordered depth key.
+constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
+struct Hit {
+    uint32_t idx;
+    uint32_t key;
+    uint32_t counted;   // 1 if the particle counts as a generated point (PBVR / occupancy statistic)
+};
+
 // gps_splat.wgsl:259-282 (GPS)
-GPS_DEV void emit_gps(const GpsFrame &f, Rng &r, uint32_t id, float4 v0, float4 v1, const float *__restrict__ lut,
-                      uint64_t *__restrict__ depth) {
+GPS_DEV Hit emit_gps(const GpsFrame &f, Rng &r, float4 v0, float4 v1, const float *__restrict__ lut) {
+    Hit h{NO_HIT, 0xFFFFFFFFu, 0u};
     float u = r.next();
     float rad = f.exact_radius == 0u ? radius_lut(lut, v0.w, u) : corrected_radius(v0.w, u);
     float th = 2.0f * PI_F * r.next();
@@ -587,25 +592,26 @@ GPS_DEV void emit_gps(const GpsFrame &f, Rng &r, uint32_t id, float4 v0, float4 
     gpsm::sincos(th, sn, cs);
     float wx = rad * cs, wy = rad * sn;
     float pxf = v0.x + v1.x * wx, pyf = v0.y + (v1.y * wx + v1.z * wy);
-    if (pxf < 0.0f || pxf >= (float)f.W || pyf < 0.0f || pyf >= (float)f.H) return;
+    h.key = gpsm::bits(v0.z) | 0x80000000u;
+    if (pxf < 0.0f || pxf >= (float)f.W || pyf < 0.0f || pyf >= (float)f.H) return h;
     const uint32_t side = f.spp_side;
     uint32_t px = f2u(pxf), py = f2u(pyf);
     uint32_t sx = f2u((pxf - (float)px) * (float)side), sy = f2u((pyf - (float)py) * (float)side);
     if (sx > side - 1u) sx = side - 1u;
     if (sy > side - 1u) sy = side - 1u;
-    uint32_t idx = (py * f.W + px) * (side * side) + sy * side + sx;
-    write_min(depth + idx, pack(v0.z, id));
+    h.idx = (py * f.W + px) * (side * side) + sy * side + sx;
+    return h;
 }
 
-// gps_splat.wgsl:490-532 (PBVR); returns 1 if the particle counts as a generated point
-GPS_DEV uint32_t emit_pbvr(const GpsFrame &f, Rng &r, uint32_t id, const float4 *__restrict__ v,
-                           uint64_t *__restrict__ depth) {
+// gps_splat.wgsl:490-532 (PBVR)
+GPS_DEV Hit emit_pbvr(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) {
+    Hit h{NO_HIT, 0xFFFFFFFFu, 0u};
     float uz = r.normal();
     float uy = r.normal();
     float ux = r.normal();
     float4 a = v[0];
     if (f.method == M_VIEW_CONDITIONED) {
-        if (r.next() >= v[1].w) return 0u;
+        if (r.next() >= v[1].w) return h;
     }
     float4 c = v[2], ca = v[4], cb = v[5];
     float wx = c.x + ca.x * ux;
@@ -613,7 +619,7 @@ GPS_DEV uint32_t emit_pbvr(const GpsFrame &f, Rng &r, uint32_t id, const float4 
     float wz = c.z + (cb.x * ux + cb.y * uy + cb.z * uz);
     float px = wx - f.pos[0], py = wy - f.pos[1], pz = wz - f.pos[2];
     float camx = dot3(f.rot[0], px, py, pz), camy = dot3(f.rot[1], px, py, pz), camz = dot3(f.rot[2], px, py, pz);
-    if (camz <= f.near_z) return 0u;
+    if (camz <= f.near_z) return h;
     float pixx = f.intr[0] * camx / camz + f.intr[2];
     float pixy = f.intr[1] * camy / camz + f.intr[3];
     if (f.flags & FLAG_JITTER) {
@@ -630,23 +636,24 @@ GPS_DEV uint32_t emit_pbvr(const GpsFrame &f, Rng &r, uint32_t id, const float4 
         float oo = fmin(a.w, 1.0f - 1e-6f);
         float y = oo * gpsm::exp(-0.5f * fmax(r2, 0.0f));
         float keep = clamp(extinction_over_linear(y) / d.z, 0.0f, 1.0f);
-        if (r.next() >= keep) return 0u;
+        if (r.next() >= keep) return h;
     }
     float depth_z = (f.flags & FLAG_CENTRE) ? a.z : camz;
-    if (pixx < 0.0f || pixx >= (float)f.W || pixy < 0.0f || pixy >= (float)f.H) return 1u;
+    h.counted = 1u;
+    h.key = gpsm::bits(depth_z) | 0x80000000u;
+    if (pixx < 0.0f || pixx >= (float)f.W || pixy < 0.0f || pixy >= (float)f.H) return h;
     const uint32_t side = f.spp_side;
     uint32_t ipx = f2u(pixx), ipy = f2u(pixy);
     uint32_t sx = f2u((pixx - (float)ipx) * (float)side), sy = f2u((pixy - (float)ipy) * (float)side);
     if (sx > side - 1u) sx = side - 1u;
     if (sy > side - 1u) sy = side - 1u;
-    uint32_t idx = (ipy * f.W + ipx) * (side * side) + sy * side + sx;
-    write_min(depth + idx, pack(depth_z, id));
-    return 1u;
+    h.idx = (ipy * f.W + ipx) * (side * side) + sy * side + sx;
+    return h;
 }
 
-// gps_splat.wgsl:686-708 (screen occupancy); returns 1 on a hit
-GPS_DEV uint32_t emit_occupancy(const GpsFrame &f, Rng &r, uint32_t id, uint32_t k, const float4 *__restrict__ v,
-                                uint64_t *__restrict__ depth) {
+// gps_splat.wgsl:686-708 (screen occupancy)
+GPS_DEV Hit emit_occupancy(const GpsFrame &f, Rng &r, uint32_t k, const float4 *__restrict__ v) {
+    Hit h{NO_HIT, 0xFFFFFFFFu, 0u};
     const uint32_t side = f.spp_side, samples = side * side;
     float4 a = v[0], b = v[1], inv = v[2];
     bool sparse = v[3].w > 0.0f;
@@ -669,23 +676,29 @@ GPS_DEV uint32_t emit_occupancy(const GpsFrame &f, Rng &r, uint32_t id, uint32_t
     float coverage;
     if (sparse) coverage = -gpsm::log(1.0f - alpha) / inv.w;
     else coverage = 1.0f - gpsm::exp(f.density * gpsm::log(1.0f - alpha));
-    if (r.next() >= coverage) return 0u;
-    uint32_t idx = (xyy * f.W + xyx) * samples + sub;
-    write_min(depth + idx, pack(a.z, id));
-    return 1u;
+    if (r.next() >= coverage) return h;
+    h.idx = (xyy * f.W + xyx) * samples + sub;
+    h.key = gpsm::bits(a.z) | 0x80000000u;
+    h.counted = 1u;
+    return h;
 }
 
-template <int MK, int PPT>
+// OUT = 0: packed 64-bit minimum into the depth|id buffer of the particle's ensemble.
+// OUT = 1: (single ensemble) per-particle key (depth key << 32 | particle index) and subpixel into pkeys /
+//          pidx at the particle's index, for the depth-sorted resolve.
+template <int MK, int PPT, int OUT>
 __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const float4 *__restrict__ view,
                                                           const uint32_t *__restrict__ offsets,
                                                           const uint32_t *__restrict__ totals, uint32_t nens,
                                                           uint32_t first_seed, uint64_t *__restrict__ depth,
                                                           uint64_t sub_count, const float *__restrict__ lut,
-                                                          uint64_t *__restrict__ stats) {
+                                                          uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys,
+                                                          uint32_t *__restrict__ pidx) {
     constexpr uint32_t CH = PART_BLOCK * PPT;
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
     __shared__ uint32_t s_lo, s_hi;
     __shared__ uint64_t s_pts[PART_BLOCK / 64];
+    static_assert(PPT == 1 || PPT == 2 || PPT == 4 || PPT == 8, "particles per thread");
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
         for (uint32_t b = 0; b < nens; b++) {
@@ -729,12 +742,16 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
                 const uint32_t k = p - cur;
                 const float4 *v = view + (size_t)VIEW_VEC4 * i;
                 Rng r(particle_seed_mixed(i, k + 1u, seed_mix));
-                if (MK == 0) {
-                    emit_gps(f, r, i, v[0], v[1], lut, dep);
-                } else if (MK == 1) {
-                    pts += emit_pbvr(f, r, i, v, dep);
+                Hit h;
+                if (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
+                else if (MK == 1) h = emit_pbvr(f, r, v);
+                else h = emit_occupancy(f, r, k, v);
+                pts += h.counted;
+                if (OUT == 0) {
+                    if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)i);
                 } else {
-                    pts += emit_occupancy(f, r, i, k, v, dep);
+                    pkeys[p] = ((uint64_t)h.key << 32) | (uint64_t)p;
+                    pidx[p] = h.idx;
                 }
             }
         }
@@ -773,7 +790,10 @@ __global__ __launch_bounds__(64) void k_splat_pass(GpsFrame f, const float4 *__r
     }
     const float4 *v = view + (size_t)VIEW_VEC4 * i;
     float4 v0 = v[0], v1 = v[1];
-    for (uint32_t k = 0; k < n; k++) emit_gps(f, r, i, v0, v1, lut, depth);
+    for (uint32_t k = 0; k < n; k++) {
+        Hit h = emit_gps(f, r, v0, v1, lut);
+        if (h.idx != NO_HIT) write_min(depth + h.idx, ((uint64_t)h.key << 32) | (uint64_t)i);
+    }
 }
 
 // ================================================================== resolve
@@ -834,6 +854,11 @@ __global__ void k_stats_add(uint64_t *stats, int slot, uint64_t v) {
 
 __global__ void k_stats_clear(uint64_t *stats, int first, int count) {
     if ((int)threadIdx.x < count) stats[first + threadIdx.x] = 0ull;
+}
+
+__global__ void k_copy_f4(float4 *__restrict__ dst, const float4 *__restrict__ src, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 __global__ void k_accum_clear(float4 *__restrict__ a, uint64_t n) {
@@ -965,17 +990,29 @@ void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uin
 
 void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
                       const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
-                      const float *lut, uint64_t *stats, int grid) {
-    constexpr int PPT = 4;
-    if (f.method == M_GPS)
-        hipLaunchKernelGGL((k_particles<0, PPT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals,
-                           nens, first_seed, depth, sub_count, lut, stats);
-    else if (f.method == M_OCCUPANCY)
-        hipLaunchKernelGGL((k_particles<2, PPT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals,
-                           nens, first_seed, depth, sub_count, lut, stats);
-    else
-        hipLaunchKernelGGL((k_particles<1, PPT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals,
-                           nens, first_seed, depth, sub_count, lut, stats);
+                      const float *lut, uint64_t *stats, int grid, uint64_t *pkeys, uint32_t *pidx, int ppt) {
+#define GPS_LAUNCH(MK, PPT, OUT)                                                                                   \
+    hipLaunchKernelGGL((k_particles<MK, PPT, OUT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals, \
+                       nens, first_seed, depth, sub_count, lut, stats, pkeys, pidx)
+#define GPS_LAUNCH_PPT(MK, OUT)          \
+    do {                                 \
+        if (ppt == 1) GPS_LAUNCH(MK, 1, OUT); \
+        else if (ppt == 2) GPS_LAUNCH(MK, 2, OUT); \
+        else if (ppt == 8) GPS_LAUNCH(MK, 8, OUT); \
+        else GPS_LAUNCH(MK, 4, OUT);     \
+    } while (0)
+    const int mk = f.method == M_GPS ? 0 : (f.method == M_OCCUPANCY ? 2 : 1);
+    if (pkeys) {
+        if (mk == 0) GPS_LAUNCH_PPT(0, 1);
+        else if (mk == 2) GPS_LAUNCH_PPT(2, 1);
+        else GPS_LAUNCH_PPT(1, 1);
+    } else {
+        if (mk == 0) GPS_LAUNCH_PPT(0, 0);
+        else if (mk == 2) GPS_LAUNCH_PPT(2, 0);
+        else GPS_LAUNCH_PPT(1, 0);
+    }
+#undef GPS_LAUNCH_PPT
+#undef GPS_LAUNCH
 }
 
 void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
@@ -1016,6 +1053,11 @@ void launch_stats_add(hipStream_t s, uint64_t *stats, int slot, uint64_t v) {
 
 void launch_stats_clear(hipStream_t s, uint64_t *stats, int first, int count) {
     hipLaunchKernelGGL(k_stats_clear, dim3(1), dim3(64), 0, s, stats, first, count);
+}
+
+void launch_copy_f4(hipStream_t s, float4 *dst, const float4 *src, uint64_t n) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_copy_f4, dim3(blocks_for(n, 256)), dim3(256), 0, s, dst, src, n);
 }
 
 void launch_accum_clear(hipStream_t s, float4 *accum, uint64_t n) {

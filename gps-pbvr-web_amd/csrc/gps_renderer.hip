@@ -5,6 +5,7 @@
 // batches (one depth|id buffer per ensemble of a batch) on one in-order HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,9 +26,21 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
                   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> *prof);
 int sorted_read(SortedState *s, hipStream_t st, uint32_t n, uint32_t *keys, uint32_t *vals, std::string &err);
 
+// 64-bit onesweep sort and the depth-sorted resolve (gps_onesweep.hip)
+struct OnesweepState;
+OnesweepState *onesweep_create();
+void onesweep_destroy(OnesweepState *s);
+int onesweep_sort(OnesweepState *s, hipStream_t st, uint64_t *k[2], uint32_t *v[2], uint64_t n, int &result,
+                  std::string &err, std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> *prof,
+                  int first_digit);
+void launch_first_hit(hipStream_t s, const uint32_t *vals, uint64_t n, uint32_t *first, uint64_t S);
+void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *first, uint64_t S,
+                        const uint32_t *offsets, uint32_t N, uint64_t *depth);
+
 namespace {
 
 thread_local std::string g_create_error;
+GpsTuningDev g_tuning = {8u, 4u, 16u, 0u};
 
 constexpr uint64_t MAX_SPLATS = 1ull << 30;
 constexpr uint32_t LUT_O = 65, LUT_V = 129;
@@ -101,6 +114,8 @@ void make_lut(float *out) {
 
 }  // namespace
 
+GpsTuningDev gps_tuning_current() { return g_tuning; }
+
 struct GpsRenderer {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -135,6 +150,17 @@ struct GpsRenderer {
     // last particle-path batch slot for the debug hooks
     bool have_last = false;
     SortedState *sorted = nullptr;
+    // depth-sorted particle resolve (GPS_PATH_PARTICLE_SORTED)
+    OnesweepState *os = nullptr;
+    uint64_t *pkeys[2] = {nullptr, nullptr};
+    uint32_t *pidx[2] = {nullptr, nullptr};
+    uint64_t pcap = 0;
+    uint32_t *first = nullptr;
+    uint64_t first_cap = 0;
+    int psort_result = 0;
+    uint64_t last_np = 0;
+    // the stream render work is enqueued on: the renderer's own, or a batch stream under graph capture
+    hipStream_t enqueue_stream = nullptr;
     // profiling
     bool profiling = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -163,20 +189,21 @@ void dfree(T *&p) {
 
 uint64_t sub_count(const GpsRenderer *r) { return (uint64_t)r->W * r->H * r->spp_side * r->spp_side; }
 
-// Per-launch HIP events on the renderer's stream when profiling is on.
+// Per-launch HIP events on the renderer's stream when profiling is on (never inside a batch graph).
 struct Prof {
     GpsRenderer *r;
     const char *name;
+    bool on;
     hipEvent_t a = nullptr, b = nullptr;
-    Prof(GpsRenderer *r_, const char *n_) : r(r_), name(n_) {
-        if (r->profiling) {
+    Prof(GpsRenderer *r_, const char *n_) : r(r_), name(n_), on(r_->profiling && r_->enqueue_stream == r_->stream) {
+        if (on) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             (void)hipEventRecord(a, r->stream);
         }
     }
     ~Prof() {
-        if (r->profiling) {
+        if (on) {
             (void)hipEventRecord(b, r->stream);
             r->pending.push_back({name, {a, b}});
         }
@@ -219,6 +246,7 @@ int alloc_batch(GpsRenderer *r) {
     uint64_t cap = budget / per;
     if (cap < 1) cap = 1;
     if (cap > MAX_BATCH) cap = MAX_BATCH;
+    if (g_tuning.batch_ensembles && cap > g_tuning.batch_ensembles) cap = g_tuning.batch_ensembles;
     r->batch_cap = (uint32_t)cap;
     HIP_TRY(r, hipMalloc(&r->depth, cap * S * sizeof(uint64_t)));
     launch_fill_u64(r->stream, r->depth, cap * S, EMPTY64);
@@ -279,6 +307,7 @@ int check_params(GpsRenderer *r, const GpsRenderParams *p) {
     uint32_t s = p->spp_side < 1 ? 1 : (p->spp_side > 4 ? 4 : p->spp_side);
     if (s != r->spp_side) return fail(r, "params.spp_side differs from the renderer's; call resize() first");
     if (p->method > 4) return fail(r, "unknown method");
+    if (p->path > GPS_PATH_PARTICLE_SORTED) return fail(r, "unknown render path");
     return 0;
 }
 
@@ -289,10 +318,79 @@ int grid_for_particles(GpsRenderer *r) {
         if (hipGetDeviceProperties(&prop, r->device) == hipSuccess) cus = prop.multiProcessorCount;
         if (cus <= 0) cus = 256;
     }
-    const char *env = std::getenv("GPS_HIP_PARTICLE_WAVES");
-    int per_cu = env ? std::atoi(env) : 8;
+    int per_cu = (int)g_tuning.particle_waves_per_cu;
     if (per_cu < 1) per_cu = 1;
     return cus * per_cu;
+}
+
+// One ensemble at a time: counts + scan, per-particle keys, onesweep sort, first hit per subpixel,
+// depth|id from the first hit, resolve.  The particle total is read back to size the sort.
+int render_sorted_particles(GpsRenderer *r, const GpsFrame &f, uint32_t first_seed, uint32_t count) {
+    r->enqueue_stream = r->stream;
+    const uint64_t S = sub_count(r);
+    const bool keep = (r->debug & GPS_DEBUG_KEEP_BUFFERS) != 0;
+    const int grid = grid_for_particles(r);
+    if (r->first_cap < S) {
+        dfree(r->first);
+        HIP_TRY(r, hipMalloc(&r->first, S * 4));
+        r->first_cap = S;
+    }
+    for (uint32_t e = 0; e < count; e++) {
+        const uint32_t seed = first_seed + e;
+        if (r->depth_dirty) {
+            launch_fill_u64(r->stream, r->depth, S, EMPTY64);
+            r->depth_dirty = false;
+        }
+        {
+            Prof pr(r, "count_scan");
+            launch_count_scan(r->stream, f, r->cinfo, seed, 1, r->offsets, r->bs, r->tiles, r->epoch, r->stats);
+        }
+        r->epoch = (r->epoch + 1u) & ((1u << 26) - 1u);
+        if (r->epoch == 0) r->epoch = 1;
+        uint32_t total = 0;
+        HIP_TRY(r, hipMemcpyAsync(&total, r->bs.totals, 4, hipMemcpyDeviceToHost, r->stream));
+        HIP_TRY(r, hipStreamSynchronize(r->stream));
+        if (total > 0) {
+            if (total > r->pcap) {
+                for (int k = 0; k < 2; k++) {
+                    dfree(r->pkeys[k]);
+                    dfree(r->pidx[k]);
+                }
+                uint64_t cap = (uint64_t)total + total / 4;
+                for (int k = 0; k < 2; k++) {
+                    HIP_TRY(r, hipMalloc(&r->pkeys[k], cap * 8));
+                    HIP_TRY(r, hipMalloc(&r->pidx[k], cap * 4));
+                }
+                r->pcap = cap;
+            }
+            {
+                Prof pr(r, "particles_keys");
+                launch_particles(r->stream, f, r->view, r->offsets, r->bs, 1, seed, r->depth, S, r->lut, r->stats,
+                                 grid, r->pkeys[0], r->pidx[0], (int)g_tuning.particles_per_thread);
+            }
+            int res = 0;
+            // keys are written at their particle index, so the low word is already in order: 4 passes
+            if (onesweep_sort(r->os, r->stream, r->pkeys, r->pidx, total, res, r->err,
+                              r->profiling ? &r->pending : nullptr, 4) != 0)
+                return -1;
+            r->psort_result = res;
+            {
+                Prof pr(r, "first_hit");
+                launch_first_hit(r->stream, r->pidx[res], total, r->first, S);
+                launch_build_depth(r->stream, r->pkeys[res], r->first, S, r->offsets, (uint32_t)r->n, r->depth);
+            }
+        }
+        r->last_np = total;
+        {
+            Prof pr(r, "resolve");
+            launch_resolve(r->stream, f, r->depth, S, r->bs.skip, 1, r->colour, r->accum, keep ? 0 : 1, r->stats);
+        }
+        if (keep) r->depth_dirty = true;
+        r->have_last = keep;
+    }
+    HIP_TRY(r, hipGetLastError());
+    r->accumulated += count;
+    return 0;
 }
 
 }  // namespace
@@ -359,6 +457,7 @@ int gps_renderer_create(uint32_t width, uint32_t height, uint32_t spp_side, int 
     if (alloc_frame(r) != 0) return bail(r->err);
     if (gps_renderer_set_gaussians(r, nullptr, 0, nullptr, 0) != 0) return bail(r->err);
     r->sorted = sorted_create();
+    r->os = onesweep_create();
     if (hipStreamSynchronize(r->stream) != hipSuccess) return bail("stream sync failed");
     *out = r;
     return 0;
@@ -369,6 +468,12 @@ void gps_renderer_destroy(GpsRenderer *r) {
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     drain_profile(r);
     if (r->sorted) sorted_destroy(r->sorted);
+    if (r->os) onesweep_destroy(r->os);
+    for (int k = 0; k < 2; k++) {
+        dfree(r->pkeys[k]);
+        dfree(r->pidx[k]);
+    }
+    dfree(r->first);
     dfree(r->splats);
     dfree(r->sh);
     dfree(r->lut);
@@ -475,18 +580,32 @@ int gps_renderer_reset_stats(GpsRenderer *r) {
     return 0;
 }
 
+static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *cam, const GpsRenderParams *p,
+                             uint32_t first_seed, uint32_t count);
+
 int gps_renderer_render_ensembles(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p,
                                   uint32_t first_seed, uint32_t count) {
     if (count == 0) return 0;
     if (check_params(r, p) != 0) return -1;
     HIP_TRY(r, hipSetDevice(r->device));
+    return enqueue_ensembles(r, r->stream, cam, p, first_seed, count);
+}
+
+// Enqueues the ensembles on `st` (the renderer's stream, or a batch stream under capture).
+static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *cam, const GpsRenderParams *p,
+                      uint32_t first_seed, uint32_t count) {
+    r->enqueue_stream = st;
     GpsFrame f;
     pack_frame(r, cam, p, f);
     const uint64_t S = sub_count(r);
-    const bool use_particles = (p->path == GPS_PATH_PARTICLE || p->method != GPS_METHOD_GPS) && r->n > 0;
+    const bool use_particles = (p->path != GPS_PATH_PRIMITIVE || p->method != GPS_METHOD_GPS) && r->n > 0;
     if (r->n > 0) {
         Prof pr(r, "view");
-        launch_view(r->stream, f, r->splats, r->sh, r->view, r->cinfo, r->colour);
+        launch_view(st, f, r->splats, r->sh, r->view, r->cinfo, r->colour);
+    }
+    if (use_particles && p->path == GPS_PATH_PARTICLE_SORTED) {
+        if (st != r->stream) return fail(r, "the sorted particle path cannot be recorded into a batch graph");
+        return render_sorted_particles(r, f, first_seed, count);
     }
     const bool keep = (r->debug & GPS_DEBUG_KEEP_BUFFERS) != 0;
     const int grid = grid_for_particles(r);
@@ -498,30 +617,30 @@ int gps_renderer_render_ensembles(GpsRenderer *r, const GpsCamera *cam, const Gp
         const uint32_t seed0 = first_seed + done;
         if (r->depth_dirty) {
             Prof pr(r, "clear");
-            launch_fill_u64(r->stream, r->depth, (uint64_t)B * S, EMPTY64);
+            launch_fill_u64(st, r->depth, (uint64_t)B * S, EMPTY64);
             r->depth_dirty = false;
         }
         if (use_particles) {
             {
                 Prof pr(r, "count_scan");
-                launch_count_scan(r->stream, f, r->cinfo, seed0, B, r->offsets, r->bs, r->tiles, r->epoch, r->stats);
+                launch_count_scan(st, f, r->cinfo, seed0, B, r->offsets, r->bs, r->tiles, r->epoch, r->stats);
             }
             r->epoch = (r->epoch + 1u) & ((1u << 26) - 1u);
             if (r->epoch == 0) r->epoch = 1;
             {
                 Prof pr(r, "particles");
-                launch_particles(r->stream, f, r->view, r->offsets, r->bs, B, seed0, r->depth, S, r->lut, r->stats,
-                                 grid);
+                launch_particles(st, f, r->view, r->offsets, r->bs, B, seed0, r->depth, S, r->lut, r->stats,
+                                 grid, nullptr, nullptr, (int)g_tuning.particles_per_thread);
             }
         } else if (r->n > 0) {
             Prof pr(r, "splat_pass");
             for (uint32_t b = 0; b < B; b++)
-                launch_splat_pass(r->stream, f, r->view, r->cinfo, seed0 + b, r->depth + (uint64_t)b * S, r->lut,
+                launch_splat_pass(st, f, r->view, r->cinfo, seed0 + b, r->depth + (uint64_t)b * S, r->lut,
                                   r->stats);
         }
         {
             Prof pr(r, "resolve");
-            launch_resolve(r->stream, f, r->depth, S, use_particles ? r->bs.skip : nullptr, B, r->colour, r->accum,
+            launch_resolve(st, f, r->depth, S, use_particles ? r->bs.skip : nullptr, B, r->colour, r->accum,
                            keep ? 0 : 1, r->stats);
         }
         if (keep) r->depth_dirty = true;
@@ -664,6 +783,17 @@ int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out) {
     return 0;
 }
 
+int gps_renderer_read_particle_keys(GpsRenderer *r, uint64_t *keys, uint32_t *subpixels, uint64_t max,
+                                    uint64_t *count) {
+    if (count) *count = r->last_np;
+    uint64_t n = r->last_np < max ? r->last_np : max;
+    if (n == 0) return 0;
+    HIP_TRY(r, hipMemcpyAsync(keys, r->pkeys[r->psort_result], n * 8, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipMemcpyAsync(subpixels, r->pidx[r->psort_result], n * 4, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    return 0;
+}
+
 int gps_renderer_profile(GpsRenderer *r, int enable) {
     HIP_TRY(r, hipStreamSynchronize(r->stream));
     drain_profile(r);
@@ -686,6 +816,232 @@ int gps_renderer_profile_read(GpsRenderer *r, GpsKernelTime *out, int max) {
         i++;
     }
     return i;
+}
+
+// -------------------------------------------------------------- batches of views (one hipGraph per batch)
+struct GpsBatch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<float4 *> outs;
+    std::vector<uint64_t> out_px;
+    double gpu_ms = 0.0, record_ms = 0.0;
+    uint64_t nodes = 0;
+};
+
+int gps_batch_create(int device, GpsBatch **out) {
+    *out = nullptr;
+    GpsBatch *b = new GpsBatch();
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "batch stream creation failed";
+        delete b;
+        return -1;
+    }
+    *out = b;
+    return 0;
+}
+
+void gps_batch_destroy(GpsBatch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto *o : b->outs)
+        if (o) (void)hipFree(o);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use_graph) {
+    if (hipSetDevice(b->device) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return -1;
+    }
+    // validate and settle every renderer before recording (allocation never happens while recording)
+    for (uint32_t i = 0; i < n; i++) {
+        GpsRenderer *r = items[i].renderer;
+        if (!r || r->device != b->device) {
+            g_create_error = "batch item without a renderer on the batch's device";
+            return -1;
+        }
+        if (check_params(r, items[i].params) != 0) {
+            g_create_error = r->err;
+            return -1;
+        }
+        if (items[i].params->path == GPS_PATH_PARTICLE_SORTED) {
+            g_create_error = "the sorted particle path cannot be recorded into a batch";
+            return -1;
+        }
+        if (hipStreamSynchronize(r->stream) != hipSuccess) {
+            g_create_error = "renderer stream sync failed";
+            return -1;
+        }
+    }
+    if (b->outs.size() < n) {
+        b->outs.resize(n, nullptr);
+        b->out_px.resize(n, 0);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        GpsRenderer *r = items[i].renderer;
+        uint64_t X = (uint64_t)r->W * r->H;
+        if (b->out_px[i] < X) {
+            if (b->outs[i]) (void)hipFree(b->outs[i]);
+            b->outs[i] = nullptr;
+            if (hipMalloc(&b->outs[i], X * sizeof(float4)) != hipSuccess) {
+                g_create_error = "out of device memory for batch outputs";
+                return -1;
+            }
+            b->out_px[i] = X;
+        }
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph && hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        g_create_error = "hipStreamBeginCapture failed";
+        return -1;
+    }
+    int rc = 0;
+    for (uint32_t i = 0; i < n && rc == 0; i++) {
+        const GpsBatchItem &it = items[i];
+        GpsRenderer *r = it.renderer;
+        uint64_t X = (uint64_t)r->W * r->H;
+        launch_accum_clear(b->stream, r->accum, X);
+        launch_stats_clear(b->stream, r->stats, ST_SKIPPED, 1);
+        r->accumulated = 0;
+        rc = enqueue_ensembles(r, b->stream, it.camera, it.params, it.first_seed, it.count);
+        if (rc == 0) {
+            launch_copy_f4(b->stream, b->outs[i], r->accum, X);
+        } else {
+            g_create_error = r->err;
+        }
+        r->enqueue_stream = r->stream;
+    }
+    if (use_graph) {
+        if (hipStreamEndCapture(b->stream, &graph) != hipSuccess) rc = -1;
+        if (rc == 0 && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = -1;
+        size_t nn = 0;
+        if (rc == 0 && hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess) b->nodes = nn;
+        if (rc != 0 && g_create_error.empty()) g_create_error = "graph capture failed";
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    b->record_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == 0 && use_graph) {
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, b->stream);
+        if (hipGraphLaunch(exec, b->stream) != hipSuccess) rc = -1;
+        (void)hipEventRecord(e1, b->stream);
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) rc = -1;
+    if (e0) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        b->gpu_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } else {
+        b->gpu_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+}
+
+int gps_batch_read(GpsBatch *b, uint32_t item, float *out) {
+    if (item >= b->outs.size() || !b->outs[item]) return -1;
+    if (hipMemcpy(out, b->outs[item], b->out_px[item] * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
+}
+
+void gps_batch_timing(const GpsBatch *b, double *gpu_ms, double *record_ms, uint64_t *graph_nodes) {
+    if (gpu_ms) *gpu_ms = b->gpu_ms;
+    if (record_ms) *record_ms = b->record_ms;
+    if (graph_nodes) *graph_nodes = b->nodes;
+}
+
+// -------------------------------------------------------------- tuning
+void gps_tuning_get(GpsTuning *out) {
+    out->particle_waves_per_cu = g_tuning.particle_waves_per_cu;
+    out->particles_per_thread = g_tuning.particles_per_thread;
+    out->sort_items_per_thread = g_tuning.sort_items_per_thread;
+    out->batch_ensembles = g_tuning.batch_ensembles;
+}
+
+int gps_tuning_set(const GpsTuning *t) {
+    uint32_t ppt = t->particles_per_thread;
+    if (!(ppt == 1 || ppt == 2 || ppt == 4 || ppt == 8)) return -1;
+    if (!(t->sort_items_per_thread == 8 || t->sort_items_per_thread == 16)) return -1;
+    if (t->particle_waves_per_cu < 1 || t->particle_waves_per_cu > 64) return -1;
+    if (t->batch_ensembles > MAX_BATCH) return -1;
+    g_tuning.particle_waves_per_cu = t->particle_waves_per_cu;
+    g_tuning.particles_per_thread = ppt;
+    g_tuning.sort_items_per_thread = t->sort_items_per_thread;
+    g_tuning.batch_ensembles = t->batch_ensembles;
+    return 0;
+}
+
+// Times each candidate tuning on `trial_ensembles` ensembles of this renderer's scene and view (seeds from
+// first_seed) and keeps the fastest; the accumulation buffer, the counters and `submitted` are restored.
+int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p, uint32_t first_seed,
+                          uint32_t trial_ensembles, GpsTuning *best_out, double *best_ms_out) {
+    if (check_params(r, p) != 0) return -1;
+    if (trial_ensembles == 0) trial_ensembles = 1;
+    const uint64_t X = (uint64_t)r->W * r->H;
+    float4 *acc_save = nullptr;
+    uint64_t *st_save = nullptr;
+    HIP_TRY(r, hipMalloc(&acc_save, X * sizeof(float4)));
+    if (hipMalloc(&st_save, ST_WORDS * 8) != hipSuccess) {
+        dfree(acc_save);
+        return fail(r, "hipMalloc failed");
+    }
+    HIP_TRY(r, hipMemcpyAsync(acc_save, r->accum, X * sizeof(float4), hipMemcpyDeviceToDevice, r->stream));
+    HIP_TRY(r, hipMemcpyAsync(st_save, r->stats, ST_WORDS * 8, hipMemcpyDeviceToDevice, r->stream));
+    const uint32_t submitted = r->accumulated;
+    const GpsTuningDev start = g_tuning;
+    GpsTuningDev best = start;
+    double best_ms = 1e30;
+    const uint32_t waves[] = {4u, 8u, 16u};
+    const uint32_t ppts[] = {1u, 2u, 4u, 8u};
+    const uint32_t items[] = {8u, 16u};
+    const bool sorted_path = p->path == GPS_PATH_PARTICLE_SORTED;
+    hipEvent_t e0, e1;
+    HIP_TRY(r, hipEventCreate(&e0));
+    HIP_TRY(r, hipEventCreate(&e1));
+    int rc = 0;
+    // one untimed call first so that lazily sized buffers exist before timing
+    rc = gps_renderer_render_ensembles(r, cam, p, first_seed, 1);
+    for (uint32_t wv : waves)
+        for (uint32_t ppt : ppts)
+            for (uint32_t it : items) {
+                if (rc != 0) break;
+                if (!sorted_path && it != start.sort_items_per_thread) continue;
+                g_tuning.particle_waves_per_cu = wv;
+                g_tuning.particles_per_thread = ppt;
+                g_tuning.sort_items_per_thread = it;
+                (void)hipEventRecord(e0, r->stream);
+                rc = gps_renderer_render_ensembles(r, cam, p, first_seed, trial_ensembles);
+                (void)hipEventRecord(e1, r->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (rc == 0 && ms < best_ms) {
+                    best_ms = ms;
+                    best = g_tuning;
+                }
+            }
+    g_tuning = rc == 0 ? best : start;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipMemcpyAsync(r->accum, acc_save, X * sizeof(float4), hipMemcpyDeviceToDevice, r->stream);
+    (void)hipMemcpyAsync(r->stats, st_save, ST_WORDS * 8, hipMemcpyDeviceToDevice, r->stream);
+    (void)hipStreamSynchronize(r->stream);
+    dfree(acc_save);
+    dfree(st_save);
+    r->accumulated = submitted;
+    if (rc != 0) return -1;
+    if (best_out) gps_tuning_get(best_out);
+    if (best_ms_out) *best_ms_out = best_ms / trial_ensembles;
+    return 0;
 }
 
 // -------------------------------------------------------------- phase-0 unit entry points

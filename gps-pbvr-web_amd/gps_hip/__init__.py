@@ -56,9 +56,10 @@ class Calibration(enum.IntEnum):
 
 
 class RenderPath(enum.IntEnum):
-    """gps.rs:93-100"""
+    """gps.rs:93-100; ParticleSorted: depth-sorted resolve of the particle path (same bits as Particle)"""
     Primitive = 0
     Particle = 1
+    ParticleSorted = 2
 
 
 class GpsCamera(C.Structure):
@@ -119,6 +120,20 @@ class GpuStats(C.Structure):
         return self.as_dict() == other.as_dict()
 
 
+class Tuning(C.Structure):
+    """Launch tuning (include/gps_hip.h GpsTuning); rendered bits never depend on it."""
+    _fields_ = [("particle_waves_per_cu", u32), ("particles_per_thread", u32), ("sort_items_per_thread", u32),
+                ("batch_ensembles", u32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("renderer", C.c_void_p), ("camera", C.c_void_p), ("params", C.c_void_p), ("first_seed", u32),
+                ("count", u32)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", u64), ("total_ms", f64)]
 
@@ -154,6 +169,16 @@ _sig("gps_radius_lut", None, [P(f32)])
 _sig("gps_renderer_set_debug", C.c_int, [_R, u32])
 _sig("gps_renderer_read_counts", C.c_int, [_R, P(u32), P(u32), P(u32), P(u32)])
 _sig("gps_renderer_read_depth_id", C.c_int, [_R, P(u64)])
+_sig("gps_renderer_read_particle_keys", C.c_int, [_R, P(u64), P(u32), u64, P(u64)])
+_sig("gps_sort_pairs_u64", C.c_int, [C.c_int, P(u64), P(u32), u64])
+_sig("gps_tuning_get", None, [P(Tuning)])
+_sig("gps_tuning_set", C.c_int, [P(Tuning)])
+_sig("gps_renderer_autotune", C.c_int, [_R, P(GpsCamera), P(RenderParams), u32, u32, P(Tuning), P(f64)])
+_sig("gps_batch_create", C.c_int, [C.c_int, P(C.c_void_p)])
+_sig("gps_batch_destroy", None, [C.c_void_p])
+_sig("gps_batch_render", C.c_int, [C.c_void_p, P(BatchItem), u32, C.c_int])
+_sig("gps_batch_read", C.c_int, [C.c_void_p, u32, P(f32)])
+_sig("gps_batch_timing", None, [C.c_void_p, P(f64), P(f64), P(u64)])
 _sig("gps_phase0_atomic", C.c_int, [C.c_int, u32, u32, u32, P(u32), P(u32)])
 _sig("gps_scan_exclusive_u32", C.c_int, [C.c_int, P(u32), P(u32), u64])
 _sig("gps_math_eval", C.c_int, [C.c_int, C.c_int, P(f32), P(f32), P(f32), u64])
@@ -168,8 +193,10 @@ EXPORTED = [
     "gps_renderer_composite", "gps_renderer_read_accum", "gps_renderer_read_stats", "gps_renderer_read_sorted",
     "gps_renderer_wait_idle", "gps_renderer_submitted", "gps_renderer_accumulated", "gps_renderer_info",
     "gps_estimate_memory", "gps_radius_lut", "gps_renderer_set_debug", "gps_renderer_read_counts",
-    "gps_renderer_read_depth_id", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
-    "gps_renderer_profile", "gps_renderer_profile_read",
+    "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
+    "gps_renderer_profile", "gps_renderer_profile_read", "gps_tuning_get", "gps_tuning_set",
+    "gps_renderer_autotune", "gps_batch_create", "gps_batch_destroy", "gps_batch_render", "gps_batch_read",
+    "gps_batch_timing",
 ]
 
 DEBUG_KEEP_BUFFERS = 1
@@ -334,6 +361,22 @@ class GpsRenderer:
         self._check(lib.gps_renderer_read_depth_id(self._h, _ptr(out, u64)))
         return out
 
+    def read_particle_keys(self, max_count=None):
+        n = u64()
+        lib.gps_renderer_read_particle_keys(self._h, None, None, 0, C.byref(n))
+        m = n.value if max_count is None else min(n.value, max_count)
+        k = np.zeros(max(m, 1), np.uint64)
+        s = np.zeros(max(m, 1), np.uint32)
+        self._check(lib.gps_renderer_read_particle_keys(self._h, _ptr(k, u64), _ptr(s, u32), m, C.byref(n)))
+        return k[:m], s[:m]
+
+    def autotune(self, cam, params, first_seed=1_000_000, trial_ensembles=2):
+        t = Tuning()
+        ms = f64()
+        self._check(lib.gps_renderer_autotune(self._h, C.byref(cam), C.byref(params), first_seed, trial_ensembles,
+                                              C.byref(t), C.byref(ms)))
+        return t.as_dict(), ms.value
+
     # ---- timing
     def profile(self, enable=True):
         self._check(lib.gps_renderer_profile(self._h, 1 if enable else 0))
@@ -344,6 +387,53 @@ class GpsRenderer:
         if n < 0:
             raise GpsError("profile read failed")
         return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(min(n, 64))}
+
+
+class Batch:
+    """Many (renderer, view, ensembles) items recorded on one stream and launched as one hipGraph."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        if lib.gps_batch_create(device, C.byref(h)) != 0:
+            raise GpsError(lib.gps_renderer_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.gps_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, items, use_graph=True):
+        """items: list of (GpsRenderer, GpsCamera, RenderParams, first_seed, count)."""
+        self._keep = [(cam, p) for _, cam, p, _, _ in items]
+        arr = (BatchItem * len(items))()
+        for i, (r, cam, p, seed, count) in enumerate(items):
+            arr[i].renderer = r._h
+            arr[i].camera = C.addressof(cam)
+            arr[i].params = C.addressof(p)
+            arr[i].first_seed = seed & 0xFFFFFFFF
+            arr[i].count = count
+        self._sizes = [r.size() for r, _, _, _, _ in items]
+        if lib.gps_batch_render(self._h, arr, len(items), 1 if use_graph else 0) != 0:
+            raise GpsError(lib.gps_renderer_last_error(None).decode())
+
+    def read(self, i):
+        w, h = self._sizes[i]
+        out = np.zeros((h, w, 4), np.float32)
+        if lib.gps_batch_read(self._h, i, _ptr(out, f32)) != 0:
+            raise GpsError("batch read failed")
+        return out
+
+    def timing(self):
+        g, rec, nodes = f64(), f64(), u64()
+        lib.gps_batch_timing(self._h, C.byref(g), C.byref(rec), C.byref(nodes))
+        return dict(gpu_ms=g.value, record_ms=rec.value, graph_nodes=nodes.value)
 
 
 # ------------------------------------------------------------------ unit entry points
@@ -361,6 +451,31 @@ def scan_exclusive_u32(x, device=0):
     if lib.gps_scan_exclusive_u32(device, _ptr(x, u32), _ptr(out, u32), x.size) != 0:
         raise GpsError("scan kernel failed")
     return out
+
+
+def get_tuning():
+    t = Tuning()
+    lib.gps_tuning_get(C.byref(t))
+    return t.as_dict()
+
+
+def set_tuning(**kw):
+    t = Tuning()
+    lib.gps_tuning_get(C.byref(t))
+    for k, v in kw.items():
+        setattr(t, k, v)
+    if lib.gps_tuning_set(C.byref(t)) != 0:
+        raise GpsError(f"invalid tuning {t.as_dict()}")
+    return t.as_dict()
+
+
+def sort_pairs_u64(keys, vals, device=0):
+    """Stable LSD radix sort on the GPU (returns sorted copies)."""
+    k = np.array(keys, np.uint64, copy=True)
+    v = np.array(vals, np.uint32, copy=True)
+    if lib.gps_sort_pairs_u64(device, _ptr(k, u64), _ptr(v, u32), k.size) != 0:
+        raise GpsError("sort failed")
+    return k, v
 
 
 MATH_FN = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sqrt": 4, "div": 5}

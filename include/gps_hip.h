@@ -49,8 +49,10 @@ enum { GPS_METHOD_GPS = 0, GPS_METHOD_PROPORTIONAL = 1, GPS_METHOD_EXTINCTION = 
        GPS_METHOD_VIEW_CONDITIONED = 3, GPS_METHOD_SCREEN_OCCUPANCY = 4 };
 /* Calibration, gps.rs:144-154 */
 enum { GPS_CAL_NONE = 0, GPS_CAL_OBJECT_ZOOM = 1, GPS_CAL_PER_SPLAT_DEPTH = 2, GPS_CAL_PER_SPLAT_FOOTPRINT = 3 };
-/* RenderPath, gps.rs:93-100 */
-enum { GPS_PATH_PRIMITIVE = 0, GPS_PATH_PARTICLE = 1 };
+/* RenderPath, gps.rs:93-100.  GPS_PATH_PARTICLE_SORTED has no counterpart in the reference: particles are
+ * resolved by a stable 64-bit sort of (ordered depth key << 32 | particle index) and the first hit per
+ * subpixel, which gives the same depth|id buffers and images as GPS_PATH_PARTICLE bit for bit. */
+enum { GPS_PATH_PRIMITIVE = 0, GPS_PATH_PARTICLE = 1, GPS_PATH_PARTICLE_SORTED = 2 };
 
 /* PbvrParams, gps.rs:169-179 */
 typedef struct GpsPbvrParams {
@@ -160,6 +162,14 @@ int gps_renderer_read_counts(GpsRenderer *r, uint32_t *counts, uint32_t *offsets
  * the high and low halves are the reference's `depth` and `winner` buffers (gps_splat.wgsl:28-29) */
 int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out);
 
+/* GPS_PATH_PARTICLE_SORTED: the sorted keys and subpixels (0xFFFFFFFF = off screen) of the last ensemble;
+ * copies min(count, max) entries and stores the particle count in *count. */
+int gps_renderer_read_particle_keys(GpsRenderer *r, uint64_t *keys, uint32_t *subpixels, uint64_t max,
+                                    uint64_t *count);
+
+/* Stable LSD radix sort of n (u64 key, u32 value) pairs in place (the onesweep sort of the sorted path). */
+int gps_sort_pairs_u64(int device, uint64_t *keys, uint32_t *vals, uint64_t n);
+
 /* Phase-0 unit kernels (crates/gps-render/src/lib.rs:99-275; shaders/p0_sample.wgsl, gps_scan.wgsl).
  * gps_phase0_atomic: samples of p0_sample.wgsl:205-218 for one ensemble with one packed 64-bit
  * minimum; writes pixels depth keys and colours.  gps_scan_exclusive_u32: exclusive prefix sum. */
@@ -168,6 +178,45 @@ int gps_phase0_atomic(int device, uint32_t seed, uint32_t pixels, uint32_t sampl
 int gps_scan_exclusive_u32(int device, const uint32_t *in, uint32_t *out, uint64_t n);
 /* Device evaluation of the kernels' f32 functions (0 exp, 1 log, 2 sin, 3 cos, 4 sqrt, 5 div x/y). */
 int gps_math_eval(int device, int fn, const float *x, const float *y, float *out, uint64_t n);
+
+/* ---- batches of views (no counterpart in the reference: its trajectory export renders one view per frame loop,
+ * web/index.html:445-469).  Each item resets its renderer's accumulation buffer, renders `count` ensembles of
+ * one view and snapshots the image into the batch's output slot `i`.  All items are recorded on one stream;
+ * with use_graph they are captured into one hipGraph and launched once.  Items may share a renderer: a later
+ * item overwrites the earlier one's history but not its snapshot.  The sorted particle path is not batchable. */
+typedef struct GpsBatch GpsBatch;
+typedef struct GpsBatchItem {
+    GpsRenderer *renderer;
+    const GpsCamera *camera;
+    const GpsRenderParams *params;
+    uint32_t first_seed;
+    uint32_t count;
+} GpsBatchItem;
+int gps_batch_create(int device, GpsBatch **out);
+void gps_batch_destroy(GpsBatch *b);
+int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use_graph);
+/* w*h*4 floats of item `item`'s accumulation snapshot */
+int gps_batch_read(GpsBatch *b, uint32_t item, float *out);
+/* GPU time of the last gps_batch_render (graph launch to completion), host time to record it, graph nodes */
+void gps_batch_timing(const GpsBatch *b, double *gpu_ms, double *record_ms, uint64_t *graph_nodes);
+
+/* ---- launch tuning (process-wide; no counterpart in the reference).  Rendered bits never depend on it:
+ * it only changes how the work is split into launches.  Renderers pick it up on their next call
+ * (batch_ensembles on the next set_gaussians / resize). */
+typedef struct GpsTuning {
+    uint32_t particle_waves_per_cu;  /* persistent particle grid = CUs x this (1..64), default 8 */
+    uint32_t particles_per_thread;   /* 1, 2, 4 (default) or 8 */
+    uint32_t sort_items_per_thread;  /* radix sort tile: 256 threads x 8 or 16 (default) keys */
+    uint32_t batch_ensembles;        /* ensembles per batch, 0 = as many as 8 GiB allow (max 64) */
+} GpsTuning;
+void gps_tuning_get(GpsTuning *out);
+int gps_tuning_set(const GpsTuning *t);
+/* Autotuner: times every candidate (waves per CU 4/8/16 x particles per thread 1/2/4/8, and sort tiles 8/16
+ * on the sorted particle path) on trial_ensembles ensembles of this scene and view, keeps the fastest as the
+ * process tuning, and restores the accumulation buffer, counters and submitted count.  Writes the chosen
+ * tuning and its GPU ms per ensemble. */
+int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p, uint32_t first_seed,
+                          uint32_t trial_ensembles, GpsTuning *best, double *best_ms_per_ensemble);
 
 /* ---- timing: HIP events around every kernel launch on the renderer's stream ---- */
 typedef struct GpsKernelTime {

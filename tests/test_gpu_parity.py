@@ -406,3 +406,171 @@ def test_sorted_after_resize():
     r.render_sorted(CAM, p, 1)
     assert r.read_accum().shape == (64, 64, 4)
     r.close()
+
+
+# ------------------------------------------------------------------ 64-bit onesweep sort and the depth-sorted resolve
+@pytest.mark.parametrize("n", [1, 7, 4095, 4096, 4097, 100_003, 1_000_000, 20_000_000])
+def test_onesweep_sort_stable(n):
+    """Stable sort of (u64 key, u32 value) = numpy's stable argsort, with duplicate keys and constant bytes."""
+    say(f"[onesweep] n={n}")
+    rng = np.random.default_rng(n)
+    hi = (np.uint64(0xBF) << np.uint64(56)) | (rng.integers(0, 1 << 12, n, dtype=np.uint64) << np.uint64(40))
+    keys = hi | rng.integers(0, max(n // 3, 1), n, dtype=np.uint64)     # many duplicates
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = gh.sort_pairs_u64(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+
+
+def test_onesweep_sort_full_range_keys():
+    n = 3_000_000
+    rng = np.random.default_rng(9)
+    keys = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    k, v = gh.sort_pairs_u64(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+
+
+def sorted_vs_atomic(scene_arr, sh, deg, cam, p, w, h, seeds):
+    out = []
+    for path in (1, 2):
+        q = p.copy(path=path)
+        r = gh.GpsRenderer(w, h, q.spp_side)
+        if sh is not None:
+            r.set_gaussians_sh(scene_arr, sh, deg)
+        else:
+            r.set_gaussians(scene_arr)
+        r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+        r.render_ensembles(cam, q, seeds[0], seeds[1])
+        res = dict(accum=r.read_accum(), depth=r.read_depth_id(), stats=r.read_stats())
+        if path == 2:
+            res["keys"], res["sub"] = r.read_particle_keys()
+            res["total"] = r.read_counts()["total"]
+        r.close()
+        out.append(res)
+    a, b = out
+    assert np.array_equal(a["depth"], b["depth"])
+    assert np.array_equal(a["accum"].view(np.uint32), b["accum"].view(np.uint32))
+    assert a["stats"] == b["stats"]
+    k = b["keys"]
+    assert k.size == b["total"]
+    assert (np.diff(k.astype(np.uint64)) > 0).all() if k.size > 1 else True   # unique keys, ascending
+    idx = (k & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    assert np.array_equal(np.sort(idx), np.arange(k.size))                   # particle indices: a permutation
+    return b
+
+
+@pytest.mark.parametrize("si", [1, 3, 4])
+@pytest.mark.parametrize("method", [0, 2])
+def test_sorted_particle_path_equals_atomic(si, method):
+    name, scene = phase_scenes()[si]
+    say(f"[sorted particles] {name} method={method}")
+    p = gh.RenderParams(spp_side=2, background=BG, method=method, max_points_per_splat=1 << 24,
+                        pbvr=dict(radial_correction=1))
+    sorted_vs_atomic(to_gpu(scene), None, 0, CAM, p, W, H, (3, 3))
+
+
+def test_sorted_particle_path_3m_splats_1080p():
+    """3,000,000 clustered Gaussians (seed 2), 1920x1080, SPP side 2, GPS, cap 65,536, far camera: the
+    depth-sorted resolve equals the packed-minimum path bit for bit, and the sorted keys are the unique
+    ascending order of the generated particles."""
+    say("[sorted particles] 3M")
+    g, sh = S.clustered(3_000_000, seed=2, sh_degree=3)
+    cam = S.clustered_camera("far", 1920, 1080)
+    p = gh.RenderParams(spp_side=2, method=0, max_points_per_splat=65536)
+    b = sorted_vs_atomic(g, sh, 3, cam, p, 1920, 1080, (1, 1))
+    say(f"[sorted particles] 3M total particles {b['total']}")
+    assert b["total"] > 10_000_000
+
+
+# ------------------------------------------------------------------ launch tuning never changes results
+def test_tunings_are_bit_identical():
+    scene = S.small_uniform(4096, seed=0)
+    cam = S.small_uniform_camera(128, 128)
+    p = gh.RenderParams(spp_side=2, method=2, max_points_per_splat=16384, pbvr=dict(radial_correction=1))
+    saved = gh.get_tuning()
+    ref = None
+    try:
+        for waves, ppt, items, batch in ((8, 4, 16, 0), (4, 1, 8, 3), (16, 8, 16, 1), (1, 2, 8, 7)):
+            say(f"[tuning] waves={waves} ppt={ppt} items={items} batch={batch}")
+            gh.set_tuning(particle_waves_per_cu=waves, particles_per_thread=ppt, sort_items_per_thread=items,
+                          batch_ensembles=batch)
+            for path in (1, 2):
+                r = gh.GpsRenderer(128, 128, 2)
+                r.set_gaussians(scene)
+                r.render_ensembles(cam, p.copy(path=path), 5, 9)
+                acc = r.read_accum()
+                r.close()
+                if ref is None:
+                    ref = acc
+                assert np.array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    finally:
+        gh.set_tuning(**saved)
+
+
+def test_autotune_keeps_history():
+    scene = S.small_uniform(4096, seed=0)
+    cam = S.small_uniform_camera(128, 128)
+    p = gh.RenderParams(spp_side=2, method=2, max_points_per_splat=16384, pbvr=dict(radial_correction=1))
+    saved = gh.get_tuning()
+    try:
+        r = gh.GpsRenderer(128, 128, 2)
+        r.set_gaussians(scene)
+        r.render_ensembles(cam, p, 1, 5)
+        before, st_before, sub = r.read_accum(), r.read_stats(), r.submitted()
+        best, ms = r.autotune(cam, p, trial_ensembles=2)
+        say(f"[autotune] {best} {ms:.4f} ms/ens")
+        assert np.array_equal(r.read_accum(), before) and r.read_stats() == st_before and r.submitted() == sub
+        assert ms > 0 and best["particles_per_thread"] in (1, 2, 4, 8)
+        r.render_ensembles(cam, p, 6, 3)
+        after = r.read_accum()
+        r.close()
+        r2 = gh.GpsRenderer(128, 128, 2)
+        r2.set_gaussians(scene)
+        r2.render_ensembles(cam, p, 1, 8)
+        assert np.array_equal(r2.read_accum(), after)
+        r2.close()
+    finally:
+        gh.set_tuning(**saved)
+
+
+# ------------------------------------------------------------------ batched views in one hipGraph
+def test_batch_graph_equals_separate_renders():
+    a_scene = S.small_uniform(4096, seed=0)
+    b_scene, b_sh = S.clustered(20_000, seed=3, sh_degree=2)
+    ra, rb = gh.GpsRenderer(128, 96, 2), gh.GpsRenderer(160, 90, 1)
+    ra.set_gaussians(a_scene)
+    rb.set_gaussians_sh(b_scene, b_sh, 2)
+    cam_a = S.small_uniform_camera(128, 96)
+    pa = gh.RenderParams(spp_side=2, method=2, max_points_per_splat=16384, pbvr=dict(radial_correction=1))
+    pg = gh.RenderParams(spp_side=2, method=0, max_points_per_splat=4096, path=1)
+    pb = gh.RenderParams(spp_side=1, method=1, max_points_per_splat=16384)
+    items = [(ra, cam_a, pa, 11, 5), (rb, S.clustered_camera("mid", 160, 90), pb, 3, 4),
+             (ra, cam_a, pg, 1, 3), (rb, S.clustered_camera("far", 160, 90), pb, 7, 70)]
+    expect = []
+    for r, cam, p, seed, count in items:
+        r.reset_accum()
+        r.render_ensembles(cam, p, seed, count)
+        expect.append(r.read_accum())
+    b = gh.Batch()
+    for use_graph in (True, False, True):
+        say(f"[batch] graph={use_graph}")
+        b.render(items, use_graph=use_graph)
+        for i in range(len(items)):
+            assert np.array_equal(b.read(i).view(np.uint32), expect[i].view(np.uint32)), i
+        t = b.timing()
+        assert t["gpu_ms"] > 0
+        if use_graph:
+            assert t["graph_nodes"] > len(items)
+    # after a batch the renderer carries the last item's history and continues it exactly
+    rb.render_ensembles(S.clustered_camera("far", 160, 90), pb, 77, 2)
+    rc = gh.GpsRenderer(160, 90, 1)
+    rc.set_gaussians_sh(b_scene, b_sh, 2)
+    rc.render_ensembles(S.clustered_camera("far", 160, 90), pb, 7, 72)
+    assert np.array_equal(rb.read_accum(), rc.read_accum())
+    with pytest.raises(gh.GpsError):
+        b.render([(ra, cam_a, pa.copy(path=2), 1, 1)])
+    b.close()
+    for r in (ra, rb, rc):
+        r.close()
