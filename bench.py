@@ -297,9 +297,11 @@ def main():
 
     ttp = None
     if not a.no_psnr:
-        ttp = {"plain": time_to_psnr(r, S.clustered_camera(head, W, H), p, per_view[head]["ms_per_ensemble"]),
-               "denoised": None}
-        log(f"[psnr] {ttp}")
+        ttp = {}
+        for v in views:
+            ttp[v] = {"plain": time_to_psnr(r, S.clustered_camera(v, W, H), p, per_view[v]["ms_per_ensemble"]),
+                      "denoised": None}
+            log(f"[psnr {v}] {ttp[v]}")
 
     cpu = None
     if not a.no_cpu:

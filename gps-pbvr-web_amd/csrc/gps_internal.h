@@ -45,7 +45,8 @@ struct GpsTuningDev {
     uint32_t particle_waves_per_cu;   // persistent particle grid = CUs * this
     uint32_t particles_per_thread;    // 1, 2, 4 or 8
     uint32_t sort_items_per_thread;   // 8 or 16
-    uint32_t batch_ensembles;         // 0 = as many as memory allows (<= 64)
+    uint32_t batch_ensembles;         // ensembles per particle launch, 0 = from depth_window_mib
+    uint32_t depth_window_mib;        // depth|id bytes one launch may span when batch_ensembles is 0
 };
 GpsTuningDev gps_tuning_current();
 
@@ -66,7 +67,9 @@ void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, cons
                       const float *lut, uint64_t *stats, int grid, uint64_t *pkeys = nullptr,
                       uint32_t *pidx = nullptr, int ppt = 4);
 void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
-                    uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats);
+                    uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats,
+                    float4 *mom = nullptr, float *zmin = nullptr);
+void launch_features_clear(hipStream_t s, float4 *mom, float *zmin, uint64_t n);
 void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
 void launch_splat_pass(hipStream_t s, const GpsFrame &f, const float4 *view, const uint2 *cinfo, uint32_t seed,
                        uint64_t *depth, const float *lut, uint64_t *stats);
@@ -81,3 +84,8 @@ void launch_phase0(hipStream_t s, uint32_t seed, uint32_t pixels, uint32_t sampl
 void launch_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ticket, uint64_t *status,
                      uint32_t tiles, uint32_t epoch);
 void launch_math(hipStream_t s, int fn, const float *x, const float *y, float *out, uint64_t n);
+
+// device buffers of a renderer for the denoiser: accumulation, moments and nearest depth (null when the
+// renderer's features are off)
+struct GpsRenderer;
+void renderer_device_buffers(GpsRenderer *r, const float4 **accum, const float4 **mom, const float **zmin);

@@ -798,21 +798,28 @@ __global__ __launch_bounds__(64) void k_splat_pass(GpsFrame f, const float4 *__r
 
 // ================================================================== resolve
 // gps_splat.wgsl:361-386, for every ensemble of the batch in ensemble order
+// With `mom` set it also accumulates, per pixel, the sum of squares of each ensemble's pixel value (x, y, z)
+// and the number of covered subpixels (w), and keeps in `zmin` the nearest depth seen over the ensembles:
+// the denoiser's variance, particle-count and depth features.
 __global__ __launch_bounds__(256) void k_resolve(GpsFrame f, uint64_t *__restrict__ depth, uint64_t sub_count,
                                                  const uint32_t *__restrict__ skip, uint32_t nens,
                                                  const float4 *__restrict__ colour, float4 *__restrict__ accum,
-                                                 int clear, uint64_t *__restrict__ stats) {
+                                                 int clear, uint64_t *__restrict__ stats, float4 *__restrict__ mom,
+                                                 float *__restrict__ zmin) {
     const uint32_t X = f.W * f.H;
     const uint32_t spp = f.spp_side * f.spp_side;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) atomicAdd((unsigned long long *)&stats[ST_ENSEMBLES], (unsigned long long)nens);
     if (i >= X) return;
     float4 acc = accum[i];
+    float4 m2 = mom ? mom[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t zbits = mom ? __float_as_uint(zmin[i]) : 0u;
     const float fs = (float)spp;
     for (uint32_t b = 0; b < nens; b++) {
         uint64_t *d = depth + (size_t)b * sub_count + (size_t)i * spp;
         if (skip == nullptr || skip[b] == 0u) {
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            uint32_t hits = 0u;
             for (uint32_t k = 0; k < spp; k++) {
                 uint64_t w = d[k];
                 if (w == EMPTY64) {
@@ -824,17 +831,41 @@ __global__ __launch_bounds__(256) void k_resolve(GpsFrame f, uint64_t *__restric
                     sx = sx + c.x;
                     sy = sy + c.y;
                     sz = sz + c.z;
+                    hits++;
+                    // the key is the depth's bits with the sign bit set: unsigned order is depth order
+                    const uint32_t zb = (uint32_t)(w >> 32) & 0x7FFFFFFFu;
+                    zbits = zb < zbits ? zb : zbits;
                 }
             }
-            acc.x = acc.x + sx / fs;
-            acc.y = acc.y + sy / fs;
-            acc.z = acc.z + sz / fs;
+            const float vx = sx / fs, vy = sy / fs, vz = sz / fs;
+            acc.x = acc.x + vx;
+            acc.y = acc.y + vy;
+            acc.z = acc.z + vz;
             acc.w = acc.w + 1.0f;
+            if (mom) {
+                m2.x = m2.x + vx * vx;
+                m2.y = m2.y + vy * vy;
+                m2.z = m2.z + vz * vz;
+                m2.w = m2.w + (float)hits;
+            }
         }
         if (clear)
             for (uint32_t k = 0; k < spp; k++) d[k] = EMPTY64;
     }
     accum[i] = acc;
+    if (mom) {
+        mom[i] = m2;
+        zmin[i] = __uint_as_float(zbits);
+    }
+}
+
+// moments to zero and nearest depth to +inf
+__global__ void k_features_clear(float4 *__restrict__ mom, float *__restrict__ zmin, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        mom[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        zmin[i] = __uint_as_float(0x7F800000u);
+    }
 }
 
 __global__ void k_fill_u64(uint64_t *__restrict__ p, uint64_t n, uint64_t v) {
@@ -1016,10 +1047,16 @@ void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, cons
 }
 
 void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
-                    uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats) {
+                    uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats, float4 *mom,
+                    float *zmin) {
     uint64_t X = (uint64_t)f.W * f.H;
     hipLaunchKernelGGL(k_resolve, dim3(blocks_for(X, 256)), dim3(256), 0, s, f, depth, sub_count, skip, nens, colour,
-                       accum, clear, stats);
+                       accum, clear, stats, mom, zmin);
+}
+
+void launch_features_clear(hipStream_t s, float4 *mom, float *zmin, uint64_t n) {
+    if (n == 0 || !mom) return;
+    hipLaunchKernelGGL(k_features_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, mom, zmin, n);
 }
 
 void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v) {

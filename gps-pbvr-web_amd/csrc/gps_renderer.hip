@@ -40,7 +40,7 @@ void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *fir
 namespace {
 
 thread_local std::string g_create_error;
-GpsTuningDev g_tuning = {8u, 4u, 16u, 0u};
+GpsTuningDev g_tuning = {8u, 4u, 16u, 0u, 64u};
 
 constexpr uint64_t MAX_SPLATS = 1ull << 30;
 constexpr uint32_t LUT_O = 65, LUT_V = 129;
@@ -142,6 +142,9 @@ struct GpsRenderer {
     bool depth_dirty = false;
     // frame
     float4 *accum = nullptr;
+    float4 *mom = nullptr;  // denoiser features (set_features): sum of squares and covered subpixels
+    float *zmin = nullptr;  // nearest depth per pixel
+    bool features = false;
     uint64_t *stats = nullptr;
     uint32_t *composite_buf = nullptr;
     uint64_t composite_cap = 0;
@@ -227,9 +230,16 @@ void drain_profile(GpsRenderer *r) {
 int alloc_frame(GpsRenderer *r) {
     dfree(r->accum);
     dfree(r->depth);
+    dfree(r->mom);
+    dfree(r->zmin);
     uint64_t X = (uint64_t)r->W * r->H;
     HIP_TRY(r, hipMalloc(&r->accum, X * sizeof(float4)));
     launch_accum_clear(r->stream, r->accum, X);
+    if (r->features) {
+        HIP_TRY(r, hipMalloc(&r->mom, X * sizeof(float4)));
+        HIP_TRY(r, hipMalloc(&r->zmin, X * sizeof(float)));
+        launch_features_clear(r->stream, r->mom, r->zmin, X);
+    }
     r->accumulated = 0;
     return 0;
 }
@@ -246,7 +256,6 @@ int alloc_batch(GpsRenderer *r) {
     uint64_t cap = budget / per;
     if (cap < 1) cap = 1;
     if (cap > MAX_BATCH) cap = MAX_BATCH;
-    if (g_tuning.batch_ensembles && cap > g_tuning.batch_ensembles) cap = g_tuning.batch_ensembles;
     r->batch_cap = (uint32_t)cap;
     HIP_TRY(r, hipMalloc(&r->depth, cap * S * sizeof(uint64_t)));
     launch_fill_u64(r->stream, r->depth, cap * S, EMPTY64);
@@ -309,6 +318,23 @@ int check_params(GpsRenderer *r, const GpsRenderParams *p) {
     if (p->method > 4) return fail(r, "unknown method");
     if (p->path > GPS_PATH_PARTICLE_SORTED) return fail(r, "unknown render path");
     return 0;
+}
+
+// Ensembles per particle launch: the tuned count, else as many as keep the launch's depth|id buffers within
+// the tuned window (the packed minimum's read-modify-writes then stay in the last-level cache), at most the
+// allocated batch.
+uint32_t ensembles_per_launch(const GpsRenderer *r) {
+    uint32_t cap = r->batch_cap;
+    if (g_tuning.batch_ensembles) {
+        if (cap > g_tuning.batch_ensembles) cap = g_tuning.batch_ensembles;
+        return cap;
+    }
+    const uint64_t window = (uint64_t)g_tuning.depth_window_mib << 20;
+    const uint64_t per = 8 * sub_count(r);
+    uint64_t b = window / (per ? per : 1);
+    if (b < 1) b = 1;
+    if (b < cap) cap = (uint32_t)b;
+    return cap;
 }
 
 int grid_for_particles(GpsRenderer *r) {
@@ -383,7 +409,8 @@ int render_sorted_particles(GpsRenderer *r, const GpsFrame &f, uint32_t first_se
         r->last_np = total;
         {
             Prof pr(r, "resolve");
-            launch_resolve(r->stream, f, r->depth, S, r->bs.skip, 1, r->colour, r->accum, keep ? 0 : 1, r->stats);
+            launch_resolve(r->stream, f, r->depth, S, r->bs.skip, 1, r->colour, r->accum, keep ? 0 : 1, r->stats,
+                           r->mom, r->zmin);
         }
         if (keep) r->depth_dirty = true;
         r->have_last = keep;
@@ -396,6 +423,12 @@ int render_sorted_particles(GpsRenderer *r, const GpsFrame &f, uint32_t first_se
 }  // namespace
 
 // ====================================================================== C ABI
+void renderer_device_buffers(GpsRenderer *r, const float4 **accum, const float4 **mom, const float **zmin) {
+    *accum = r->accum;
+    *mom = r->mom;
+    *zmin = r->zmin;
+}
+
 extern "C" {
 
 void gps_render_params_default(GpsRenderParams *o) {
@@ -487,6 +520,8 @@ void gps_renderer_destroy(GpsRenderer *r) {
     dfree(r->bs.totals);
     dfree(r->bs.skip);
     dfree(r->accum);
+    dfree(r->mom);
+    dfree(r->zmin);
     dfree(r->stats);
     dfree(r->composite_buf);
     if (r->stream) (void)hipStreamDestroy(r->stream);
@@ -568,6 +603,7 @@ int gps_renderer_resize(GpsRenderer *r, uint32_t width, uint32_t height, uint32_
 
 int gps_renderer_reset_accum(GpsRenderer *r) {
     launch_accum_clear(r->stream, r->accum, (uint64_t)r->W * r->H);
+    launch_features_clear(r->stream, r->mom, r->zmin, (uint64_t)r->W * r->H);
     launch_stats_clear(r->stream, r->stats, ST_SKIPPED, 1);
     r->accumulated = 0;
     HIP_TRY(r, hipGetLastError());
@@ -610,9 +646,10 @@ static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *ca
     const bool keep = (r->debug & GPS_DEBUG_KEEP_BUFFERS) != 0;
     const int grid = grid_for_particles(r);
     uint32_t done = 0;
+    const uint32_t cap = ensembles_per_launch(r);
     while (done < count) {
         uint32_t B = count - done;
-        if (B > r->batch_cap) B = r->batch_cap;
+        if (B > cap) B = cap;
         if (keep) B = 1;
         const uint32_t seed0 = first_seed + done;
         if (r->depth_dirty) {
@@ -641,7 +678,7 @@ static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *ca
         {
             Prof pr(r, "resolve");
             launch_resolve(st, f, r->depth, S, use_particles ? r->bs.skip : nullptr, B, r->colour, r->accum,
-                           keep ? 0 : 1, r->stats);
+                           keep ? 0 : 1, r->stats, r->mom, r->zmin);
         }
         if (keep) r->depth_dirty = true;
         r->have_last = keep && use_particles;
@@ -700,6 +737,35 @@ int gps_renderer_composite(GpsRenderer *r, const GpsRenderParams *p, uint32_t ou
 int gps_renderer_read_accum(GpsRenderer *r, float *out) {
     HIP_TRY(r, hipMemcpyAsync(out, r->accum, (uint64_t)r->W * r->H * sizeof(float4), hipMemcpyDeviceToHost,
                               r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    return 0;
+}
+
+int gps_renderer_set_features(GpsRenderer *r, int enable) {
+    HIP_TRY(r, hipSetDevice(r->device));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    const uint64_t X = (uint64_t)r->W * r->H;
+    if (!enable) {
+        dfree(r->mom);
+        dfree(r->zmin);
+        r->features = false;
+        return 0;
+    }
+    if (!r->features) {
+        r->features = true;
+        HIP_TRY(r, hipMalloc(&r->mom, X * sizeof(float4)));
+        HIP_TRY(r, hipMalloc(&r->zmin, X * sizeof(float)));
+    }
+    launch_features_clear(r->stream, r->mom, r->zmin, X);
+    HIP_TRY(r, hipGetLastError());
+    return 0;
+}
+
+int gps_renderer_read_features(GpsRenderer *r, float *moments, float *zmin) {
+    if (!r->features) return fail(r, "features are not enabled");
+    const uint64_t X = (uint64_t)r->W * r->H;
+    if (moments) HIP_TRY(r, hipMemcpyAsync(moments, r->mom, X * sizeof(float4), hipMemcpyDeviceToHost, r->stream));
+    if (zmin) HIP_TRY(r, hipMemcpyAsync(zmin, r->zmin, X * sizeof(float), hipMemcpyDeviceToHost, r->stream));
     HIP_TRY(r, hipStreamSynchronize(r->stream));
     return 0;
 }
@@ -905,6 +971,7 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
         GpsRenderer *r = it.renderer;
         uint64_t X = (uint64_t)r->W * r->H;
         launch_accum_clear(b->stream, r->accum, X);
+        launch_features_clear(b->stream, r->mom, r->zmin, X);
         launch_stats_clear(b->stream, r->stats, ST_SKIPPED, 1);
         r->accumulated = 0;
         rc = enqueue_ensembles(r, b->stream, it.camera, it.params, it.first_seed, it.count);
@@ -965,6 +1032,7 @@ void gps_tuning_get(GpsTuning *out) {
     out->particles_per_thread = g_tuning.particles_per_thread;
     out->sort_items_per_thread = g_tuning.sort_items_per_thread;
     out->batch_ensembles = g_tuning.batch_ensembles;
+    out->depth_window_mib = g_tuning.depth_window_mib;
 }
 
 int gps_tuning_set(const GpsTuning *t) {
@@ -973,10 +1041,12 @@ int gps_tuning_set(const GpsTuning *t) {
     if (!(t->sort_items_per_thread == 8 || t->sort_items_per_thread == 16)) return -1;
     if (t->particle_waves_per_cu < 1 || t->particle_waves_per_cu > 64) return -1;
     if (t->batch_ensembles > MAX_BATCH) return -1;
+    if (t->depth_window_mib < 1 || t->depth_window_mib > (1u << 20)) return -1;
     g_tuning.particle_waves_per_cu = t->particle_waves_per_cu;
     g_tuning.particles_per_thread = ppt;
     g_tuning.sort_items_per_thread = t->sort_items_per_thread;
     g_tuning.batch_ensembles = t->batch_ensembles;
+    g_tuning.depth_window_mib = t->depth_window_mib;
     return 0;
 }
 
@@ -1029,6 +1099,32 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
                     best = g_tuning;
                 }
             }
+    double best_ms_per = best_ms / trial_ensembles;
+    // second stage: ensembles per particle launch, through the depth window
+    if (rc == 0 && !sorted_path && start.batch_ensembles == 0) {
+        const uint32_t windows[] = {8u, 16u, 32u, 64u, 128u, 256u, 512u, 4096u};
+        uint32_t last_b = 0;
+        for (uint32_t wmib : windows) {
+            if (rc != 0) break;
+            g_tuning = best;
+            g_tuning.depth_window_mib = wmib;
+            const uint32_t b = ensembles_per_launch(r);
+            if (b == last_b) continue;  // same launches as the previous window
+            last_b = b;
+            const uint32_t trial = trial_ensembles > 2 * b ? trial_ensembles : 2 * b;
+            (void)hipEventRecord(e0, r->stream);
+            rc = gps_renderer_render_ensembles(r, cam, p, first_seed, trial);
+            (void)hipEventRecord(e1, r->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = (double)ms / trial;
+            if (rc == 0 && per < best_ms_per) {
+                best_ms_per = per;
+                best = g_tuning;
+            }
+        }
+    }
     g_tuning = rc == 0 ? best : start;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1040,7 +1136,7 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
     r->accumulated = submitted;
     if (rc != 0) return -1;
     if (best_out) gps_tuning_get(best_out);
-    if (best_ms_out) *best_ms_out = best_ms / trial_ensembles;
+    if (best_ms_out) *best_ms_out = best_ms_per;
     return 0;
 }
 

@@ -123,7 +123,7 @@ class GpuStats(C.Structure):
 class Tuning(C.Structure):
     """Launch tuning (include/gps_hip.h GpsTuning); rendered bits never depend on it."""
     _fields_ = [("particle_waves_per_cu", u32), ("particles_per_thread", u32), ("sort_items_per_thread", u32),
-                ("batch_ensembles", u32)]
+                ("batch_ensembles", u32), ("depth_window_mib", u32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -132,6 +132,10 @@ class Tuning(C.Structure):
 class BatchItem(C.Structure):
     _fields_ = [("renderer", C.c_void_p), ("camera", C.c_void_p), ("params", C.c_void_p), ("first_seed", u32),
                 ("count", u32)]
+
+
+class DenoiseInfo(C.Structure):
+    _fields_ = [("gpu_ms", f64), ("width", u32), ("height", u32), ("tiles", u32), ("fallback_tiles", u32)]
 
 
 class KernelTime(C.Structure):
@@ -174,6 +178,23 @@ _sig("gps_sort_pairs_u64", C.c_int, [C.c_int, P(u64), P(u32), u64])
 _sig("gps_tuning_get", None, [P(Tuning)])
 _sig("gps_tuning_set", C.c_int, [P(Tuning)])
 _sig("gps_renderer_autotune", C.c_int, [_R, P(GpsCamera), P(RenderParams), u32, u32, P(Tuning), P(f64)])
+_sig("gps_renderer_set_features", C.c_int, [C.c_void_p, C.c_int])
+_sig("gps_renderer_read_features", C.c_int, [C.c_void_p, P(f32), P(f32)])
+_sig("gps_denoiser_param_count", u64, [])
+_sig("gps_denoiser_create", C.c_int, [C.c_int, u32, P(C.c_void_p)])
+_sig("gps_denoiser_destroy", None, [C.c_void_p])
+_sig("gps_denoiser_last_error", C.c_char_p, [C.c_void_p])
+_sig("gps_denoiser_set_params", C.c_int, [C.c_void_p, P(f32)])
+_sig("gps_denoiser_get_params", C.c_int, [C.c_void_p, P(f32)])
+_sig("gps_denoiser_apply", C.c_int, [C.c_void_p, C.c_void_p, f32, P(DenoiseInfo)])
+_sig("gps_denoiser_read", C.c_int, [C.c_void_p, P(f32)])
+_sig("gps_denoiser_read_features", C.c_int, [C.c_void_p, P(f32), P(f32)])
+_sig("gps_denoiser_add_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p])
+_sig("gps_denoiser_pairs", u32, [C.c_void_p])
+_sig("gps_denoiser_clear_pairs", C.c_int, [C.c_void_p])
+_sig("gps_denoiser_train", C.c_int, [C.c_void_p, u32, u32, f32, u32, P(f32)])
+_sig("gps_denoiser_loss_grad", C.c_int, [C.c_void_p, u32, P(f32), P(f32)])
+_sig("gps_denoiser_read_output", C.c_int, [C.c_void_p, u32, P(f32)])
 _sig("gps_batch_create", C.c_int, [C.c_int, P(C.c_void_p)])
 _sig("gps_batch_destroy", None, [C.c_void_p])
 _sig("gps_batch_render", C.c_int, [C.c_void_p, P(BatchItem), u32, C.c_int])
@@ -196,7 +217,11 @@ EXPORTED = [
     "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
     "gps_renderer_profile", "gps_renderer_profile_read", "gps_tuning_get", "gps_tuning_set",
     "gps_renderer_autotune", "gps_batch_create", "gps_batch_destroy", "gps_batch_render", "gps_batch_read",
-    "gps_batch_timing",
+    "gps_batch_timing", "gps_renderer_set_features", "gps_renderer_read_features",
+    "gps_denoiser_param_count", "gps_denoiser_create", "gps_denoiser_destroy", "gps_denoiser_last_error",
+    "gps_denoiser_set_params", "gps_denoiser_get_params", "gps_denoiser_apply", "gps_denoiser_read",
+    "gps_denoiser_read_features", "gps_denoiser_add_pair", "gps_denoiser_pairs", "gps_denoiser_clear_pairs",
+    "gps_denoiser_train", "gps_denoiser_loss_grad", "gps_denoiser_read_output",
 ]
 
 DEBUG_KEEP_BUFFERS = 1
@@ -302,6 +327,18 @@ class GpsRenderer:
         out = np.zeros((h, w, 4), np.float32)
         self._check(lib.gps_renderer_read_accum(self._h, _ptr(out, f32)))
         return out
+
+    def set_features(self, enable=True):
+        """Accumulate the denoiser's per-pixel features (sums of squares, covered subpixels, nearest depth)."""
+        self._check(lib.gps_renderer_set_features(self._h, 1 if enable else 0))
+
+    def read_features(self):
+        """(moments (h, w, 4): sum of squared ensemble values r, g, b and covered subpixels; zmin (h, w))."""
+        w, h = self.size()
+        mom = np.zeros((h, w, 4), np.float32)
+        zmin = np.zeros((h, w), np.float32)
+        self._check(lib.gps_renderer_read_features(self._h, _ptr(mom, f32), _ptr(zmin, f32)))
+        return mom, zmin
 
     def read_stats(self):
         s = GpuStats()
@@ -434,6 +471,112 @@ class Batch:
         g, rec, nodes = f64(), f64(), u64()
         lib.gps_batch_timing(self._h, C.byref(g), C.byref(rec), C.byref(nodes))
         return dict(gpu_ms=g.value, record_ms=rec.value, graph_nodes=nodes.value)
+
+
+DENOISER_LAYERS = ((8, 32), (32, 32), (32, 32), (32, 32), (32, 3))  # (cin as stored, cout)
+
+
+def denoiser_param_count():
+    return int(lib.gps_denoiser_param_count())
+
+
+def denoiser_split(params):
+    """Parameter vector -> [(W (cout, 3, 3, cin), b (cout,))] per layer."""
+    out, o = [], 0
+    for cin, cout in DENOISER_LAYERS:
+        w = np.asarray(params[o:o + cout * 9 * cin]).reshape(cout, 3, 3, cin)
+        o += cout * 9 * cin
+        out.append((w, np.asarray(params[o:o + cout])))
+        o += cout
+    return out
+
+
+class Denoiser:
+    """Learned denoiser (5 layers of 3x3 convolutions, 32 channels, BF16 on MFMA) of a renderer's accumulation."""
+
+    def __init__(self, seed=1, device=0):
+        h = C.c_void_p()
+        if lib.gps_denoiser_create(device, seed, C.byref(h)) != 0:
+            raise GpsError("denoiser creation failed")
+        self._h = h
+        self._size = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.gps_denoiser_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise GpsError(lib.gps_denoiser_last_error(self._h).decode())
+        return rc
+
+    def get_params(self):
+        out = np.zeros(denoiser_param_count(), np.float32)
+        self._check(lib.gps_denoiser_get_params(self._h, _ptr(out, f32)))
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == denoiser_param_count()
+        self._check(lib.gps_denoiser_set_params(self._h, _ptr(p, f32)))
+
+    def save(self, path):
+        self.get_params().tofile(path)
+
+    def load(self, path):
+        self.set_params(np.fromfile(path, np.float32))
+
+    def apply(self, renderer, fallback_ratio=0.0):
+        info = DenoiseInfo()
+        self._check(lib.gps_denoiser_apply(self._h, renderer._h, fallback_ratio, C.byref(info)))
+        self._size = (info.width, info.height)
+        return dict(gpu_ms=info.gpu_ms, tiles=info.tiles, fallback_tiles=info.fallback_tiles,
+                    fallback_rate=info.fallback_tiles / max(info.tiles, 1))
+
+    def read(self):
+        w, h = self._size
+        out = np.zeros((h, w, 4), np.float32)
+        self._check(lib.gps_denoiser_read(self._h, _ptr(out, f32)))
+        return out
+
+    def read_features(self):
+        w, h = self._size
+        x8 = np.zeros((h, w, 8), np.float32)
+        mean = np.zeros((h, w, 4), np.float32)
+        self._check(lib.gps_denoiser_read_features(self._h, _ptr(x8, f32), _ptr(mean, f32)))
+        return x8, mean
+
+    def add_pair(self, inp, target):
+        return self._check(lib.gps_denoiser_add_pair(self._h, inp._h, target._h))
+
+    def pairs(self):
+        return int(lib.gps_denoiser_pairs(self._h))
+
+    def clear_pairs(self):
+        self._check(lib.gps_denoiser_clear_pairs(self._h))
+
+    def train(self, steps, batch, lr=1e-3, seed=1):
+        losses = np.zeros(steps, np.float32)
+        self._check(lib.gps_denoiser_train(self._h, steps, batch, lr, seed, _ptr(losses, f32)))
+        return losses
+
+    def loss_grad(self, batch):
+        loss = np.zeros(1, np.float32)
+        grad = np.zeros(denoiser_param_count(), np.float32)
+        self._check(lib.gps_denoiser_loss_grad(self._h, batch, _ptr(loss, f32), _ptr(grad, f32)))
+        return float(loss[0]), grad
+
+    def read_output(self, images, w, h):
+        out = np.zeros((images, h, w, 4), np.float32)
+        self._check(lib.gps_denoiser_read_output(self._h, images, _ptr(out, f32)))
+        return out
 
 
 # ------------------------------------------------------------------ unit entry points

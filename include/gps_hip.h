@@ -179,6 +179,48 @@ int gps_scan_exclusive_u32(int device, const uint32_t *in, uint32_t *out, uint64
 /* Device evaluation of the kernels' f32 functions (0 exp, 1 log, 2 sin, 3 cos, 4 sqrt, 5 div x/y). */
 int gps_math_eval(int device, int fn, const float *x, const float *y, float *out, uint64_t n);
 
+/* ---- denoiser features (no counterpart in the reference).  When enabled, every resolve also accumulates per
+ * pixel the sum over ensembles of the squared ensemble pixel value (r, g, b) and the number of covered
+ * subpixels (w), and keeps the nearest particle depth seen (+inf where nothing was hit).  reset_accum clears
+ * them with the accumulation buffer.  Enabling clears them; disabling frees them. */
+int gps_renderer_set_features(GpsRenderer *r, int enable);
+/* moments: w*h*4 floats (may be NULL); zmin: w*h floats (may be NULL) */
+int gps_renderer_read_features(GpsRenderer *r, float *moments, float *zmin);
+
+/* ---- learned denoiser (no counterpart in the reference).  A 5-layer 3x3 convolution network (32 channels,
+ * ReLU, residual output) maps 7 per-pixel features of a renderer's accumulation (mean rgb, per-ensemble standard
+ * deviation, standard error of the mean, 1/(1 + nearest depth), covered fraction) to the many-ensemble image.
+ * BF16 weights and activations on MFMA with FP32 accumulation; FP32 master weights trained with Adam.
+ * Parameter vector: per layer l = 0..4, W[cout][3 x 3 taps][cin] then b[cout], with (cin, cout) =
+ * (8, 32), (32, 32), (32, 32), (32, 32), (32, 3) (input channel 7 is zero).  A tile of 16 x 16 pixels falls back
+ * to the plain mean when the mean squared change the network makes exceeds fallback_ratio times the variance
+ * of the mean (estimated from the moments); fallback_ratio <= 0 turns the fallback off. */
+typedef struct GpsDenoiser GpsDenoiser;
+typedef struct GpsDenoiseInfo {
+    double gpu_ms; /* features + network + fallback, HIP events */
+    uint32_t width, height, tiles, fallback_tiles;
+} GpsDenoiseInfo;
+uint64_t gps_denoiser_param_count(void);
+int gps_denoiser_create(int device, uint32_t seed, GpsDenoiser **out);
+void gps_denoiser_destroy(GpsDenoiser *d);
+const char *gps_denoiser_last_error(const GpsDenoiser *d);
+int gps_denoiser_set_params(GpsDenoiser *d, const float *params); /* also resets Adam */
+int gps_denoiser_get_params(GpsDenoiser *d, float *params);
+/* denoises renderer r's accumulation (its features must be enabled); waits for the renderer's work */
+int gps_denoiser_apply(GpsDenoiser *d, GpsRenderer *r, float fallback_ratio, GpsDenoiseInfo *info);
+int gps_denoiser_read(GpsDenoiser *d, float *rgba);                        /* w*h*4 of the last apply */
+int gps_denoiser_read_features(GpsDenoiser *d, float *x8, float *mean4);    /* last apply's inputs (w*h*8, w*h*4) */
+/* training pairs: the input renderer's features and the target renderer's mean (same size for every pair) */
+int gps_denoiser_add_pair(GpsDenoiser *d, GpsRenderer *input, GpsRenderer *target); /* returns the pair index */
+uint32_t gps_denoiser_pairs(const GpsDenoiser *d);
+int gps_denoiser_clear_pairs(GpsDenoiser *d);
+/* `steps` Adam steps on batches of `batch` pairs drawn with a PCG32 stream seeded by `seed`; losses[steps]
+ * (mean squared error over the batch's pixels and channels) may be NULL */
+int gps_denoiser_train(GpsDenoiser *d, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float *losses);
+/* loss and parameter gradient over pairs 0..batch-1 without an update (test hook) */
+int gps_denoiser_loss_grad(GpsDenoiser *d, uint32_t batch, float *loss, float *grad);
+int gps_denoiser_read_output(GpsDenoiser *d, uint32_t images, float *rgba); /* network output of the last step */
+
 /* ---- batches of views (no counterpart in the reference: its trajectory export renders one view per frame loop,
  * web/index.html:445-469).  Each item resets its renderer's accumulation buffer, renders `count` ensembles of
  * one view and snapshots the image into the batch's output slot `i`.  All items are recorded on one stream;
@@ -201,20 +243,23 @@ int gps_batch_read(GpsBatch *b, uint32_t item, float *out);
 void gps_batch_timing(const GpsBatch *b, double *gpu_ms, double *record_ms, uint64_t *graph_nodes);
 
 /* ---- launch tuning (process-wide; no counterpart in the reference).  Rendered bits never depend on it:
- * it only changes how the work is split into launches.  Renderers pick it up on their next call
- * (batch_ensembles on the next set_gaussians / resize). */
+ * it only changes how the work is split into launches.  Renderers pick it up on their next call. */
 typedef struct GpsTuning {
     uint32_t particle_waves_per_cu;  /* persistent particle grid = CUs x this (1..64), default 8 */
     uint32_t particles_per_thread;   /* 1, 2, 4 (default) or 8 */
     uint32_t sort_items_per_thread;  /* radix sort tile: 256 threads x 8 or 16 (default) keys */
-    uint32_t batch_ensembles;        /* ensembles per batch, 0 = as many as 8 GiB allow (max 64) */
+    uint32_t batch_ensembles;        /* ensembles per particle launch (1..64); 0 = from depth_window_mib */
+    uint32_t depth_window_mib;       /* with batch_ensembles 0: as many ensembles per launch as keep their
+                                        depth|id buffers (8 bytes per subpixel each) within this many MiB,
+                                        at least 1; default 64 */
 } GpsTuning;
 void gps_tuning_get(GpsTuning *out);
 int gps_tuning_set(const GpsTuning *t);
 /* Autotuner: times every candidate (waves per CU 4/8/16 x particles per thread 1/2/4/8, and sort tiles 8/16
- * on the sorted particle path) on trial_ensembles ensembles of this scene and view, keeps the fastest as the
- * process tuning, and restores the accumulation buffer, counters and submitted count.  Writes the chosen
- * tuning and its GPU ms per ensemble. */
+ * on the sorted particle path) on trial_ensembles ensembles of this scene and view, then the depth window
+ * (8 .. 4096 MiB, with batch_ensembles 0) on max(trial_ensembles, 2 x the ensembles per launch it gives),
+ * keeps the fastest as the process tuning, and restores the accumulation buffer, counters and submitted
+ * count.  Writes the chosen tuning and its GPU ms per ensemble. */
 int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p, uint32_t first_seed,
                           uint32_t trial_ensembles, GpsTuning *best, double *best_ms_per_ensemble);
 

@@ -651,6 +651,13 @@ static void particle(Ctx *c, uint32_t i, uint32_t k, uint32_t mode) {
     }
 }
 
+/* Denoiser feature outputs of the next mir_render_ensembles calls (NULL: off).  Test hook, not thread safe. */
+static float *g_feat_mom, *g_feat_zmin;
+EXPORT void mir_set_features(float *moments, float *zmin) {
+    g_feat_mom = moments;
+    g_feat_zmin = zmin;
+}
+
 /* gps.rs:812-856: one FrameUniform */
 static void pack_frame(MFrame *fr, const MGaussian *g, uint32_t n, uint32_t sh_stride, uint32_t sh_max_degree,
                        const MCamera *cam, const MParams *p, uint32_t W, uint32_t H, uint32_t spp_side, uint32_t seed) {
@@ -783,10 +790,28 @@ EXPORT int mir_render_ensembles(const MGaussian *g, uint32_t n, const float *sh_
                     }
                 }
                 float fs = (float)spp;
-                accum[4 * i + 0] = accum[4 * i + 0] + sx / fs;
-                accum[4 * i + 1] = accum[4 * i + 1] + sy / fs;
-                accum[4 * i + 2] = accum[4 * i + 2] + sz / fs;
+                float vx = sx / fs, vy = sy / fs, vz = sz / fs;
+                accum[4 * i + 0] = accum[4 * i + 0] + vx;
+                accum[4 * i + 1] = accum[4 * i + 1] + vy;
+                accum[4 * i + 2] = accum[4 * i + 2] + vz;
                 accum[4 * i + 3] = accum[4 * i + 3] + 1.0f;
+                if (g_feat_mom) {
+                    /* denoiser features: squared ensemble values, covered subpixels, nearest depth */
+                    uint32_t hits = 0, zb = fbits(g_feat_zmin[i]);
+                    for (uint32_t k = 0; k < spp; k++) {
+                        uint32_t d = c.depth[i * spp + k];
+                        if (d != EMPTY) {
+                            hits++;
+                            if ((d & 0x7FFFFFFFu) < zb) zb = d & 0x7FFFFFFFu;
+                        }
+                    }
+                    float *m = &g_feat_mom[4 * i];
+                    m[0] = m[0] + vx * vx;
+                    m[1] = m[1] + vy * vy;
+                    m[2] = m[2] + vz * vz;
+                    m[3] = m[3] + (float)hits;
+                    memcpy(&g_feat_zmin[i], &zb, 4);
+                }
             }
         }
         if (e + 1 == count) {

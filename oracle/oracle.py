@@ -144,6 +144,7 @@ _sig("mir_radial_keep", f32, [f32, f32])
 _sig("mir_poisson", u32, [f32, u32])
 for _n in ("mir_expf_n", "mir_logf_n", "mir_sinf_n", "mir_cosf_n"):
     _sig(_n, None, [P(f32), P(f32), sz])
+_sig("mir_set_features", None, [P(f32), P(f32)])
 _sig("mir_render_ensembles", C.c_int, [P(MGaussian), u32, P(f32), u32, P(f32), P(OrcCamera), P(MParams), u32, u32,
                                         u32, u32, u32, P(f32), P(u64), P(u32), P(u32), P(u64), P(u32), P(u32)])
 _sig("mir_splat_colours", None, [P(MGaussian), u32, P(f32), u32, P(OrcCamera), P(MParams), u32, u32, P(f32)])
@@ -275,8 +276,10 @@ STAT_NAMES = ("points", "truncated_splats", "dropped_points", "visible_splats", 
 
 
 def mirror_render(splats, sh_rest, sh_degree, cam, params, w, h, first_seed, count, accum=None, stats=None,
-                  want_buffers=False, lut=None):
-    """f32 mirror of GpsRenderer::render_ensembles.  splats: structured ndarray / bytes of GpuGaussian (64 B each)."""
+                  want_buffers=False, lut=None, features=None):
+    """f32 mirror of GpsRenderer::render_ensembles.  splats: structured ndarray / bytes of GpuGaussian (64 B each).
+    features: optional (moments (h, w, 4) f32, zmin (h, w) f32) updated in place like the renderer's
+    denoiser features."""
     sp = np.ascontiguousarray(splats).view(np.float32).reshape(-1, 16)
     n = sp.shape[0]
     spp_side = params.spp_side
@@ -294,12 +297,18 @@ def mirror_render(splats, sh_rest, sh_degree, cam, params, w, h, first_seed, cou
         counts = np.zeros(max(n, 1), np.uint32)
         offsets = np.zeros(max(n, 1), np.uint32)
         depth_id = np.zeros(w * h * spp_side * spp_side, np.uint64)
+    if features is not None:
+        assert features[0].dtype == np.float32 and features[0].size == 4 * w * h and features[0].flags.c_contiguous
+        assert features[1].dtype == np.float32 and features[1].size == w * h and features[1].flags.c_contiguous
+        lib.mir_set_features(ptr(features[0], f32), ptr(features[1], f32))
     rc = lib.mir_render_ensembles(sp.ctypes.data_as(P(MGaussian)), n, ptr(sh, f32), sh_degree if sh_rest is not None else 0,
                                   ptr(lut, f32), C.byref(camera_array(cam)), C.byref(params), w, h, spp_side,
                                   first_seed, count, ptr(accum, f32), ptr(stats, u64),
                                   ptr(counts, u32) if want_buffers else None,
                                   ptr(offsets, u32) if want_buffers else None,
                                   ptr(depth_id, u64) if want_buffers else None, C.byref(total), C.byref(skipped))
+    if features is not None:
+        lib.mir_set_features(None, None)
     if rc != 0:
         raise ValueError("mirror render_ensembles rejected the parameters")
     out.update(accum=accum, stats=stats)

@@ -1,0 +1,176 @@
+"""Learned denoiser on the GPU against the float32 PyTorch reference (CPU) of the same network: features
+bit-exact, forward within bf16 rounding of intermediate activations, gradients within bf16 rounding of the
+backward's activation gradients; training lowers the loss; the variance-based fallback."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import denoise_ref as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "gps-pbvr-web_amd"))
+import gps_hip as gh  # noqa: E402
+from gps_hip import scenes as S  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def render(w, h, count, seed, scene=None, spp=1, cam=None):
+    scene = S.small_uniform(4096, seed=0) if scene is None else scene
+    r = gh.GpsRenderer(w, h, spp)
+    r.set_gaussians(scene)
+    r.set_features(True)
+    p = gh.RenderParams(spp_side=spp, method=gh.Method.Extinction, max_points_per_splat=16384,
+                        pbvr=dict(calibration=3, radial_correction=1))
+    r.render_ensembles(cam or S.small_uniform_camera(w, h), p, seed, count)
+    return r
+
+
+def random_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for cin, cout in R.LAYERS:
+        used = 7 if cin == 8 else cin
+        w = rng.normal(0.0, scale * np.sqrt(2.0 / (9 * used)), (cout, 3, 3, cin)).astype(np.float32)
+        w[..., used:] = 0.0
+        out += [w.ravel(), rng.normal(0.0, 0.05, cout).astype(np.float32)]
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("w,h,spp", [(100, 70, 1), (64, 4, 2), (131, 33, 1)])
+def test_features_and_forward_match_reference(w, h, spp):
+    say(f"[denoise forward] {w}x{h} spp {spp}")
+    r = render(w, h, 4, 3, spp=spp)
+    d = gh.Denoiser(seed=5)
+    params = random_params(9)
+    d.set_params(params)
+    assert np.array_equal(d.get_params(), params)
+    info = d.apply(r, fallback_ratio=0.0)
+    assert info["fallback_tiles"] == 0 and info["gpu_ms"] > 0
+    x8, mean = d.read_features()
+    acc = r.read_accum()
+    mom, zmin = r.read_features()
+    ex8, emean = R.features(acc, mom, zmin, spp * spp)
+    assert np.array_equal(x8.view(np.uint32), ex8.view(np.uint32))
+    assert np.array_equal(mean.view(np.uint32), emean.view(np.uint32))
+    out = d.read()
+    ref, _ = R.forward(ex8[None], emean[None], params)
+    ref = ref.detach().numpy()[0]
+    resid = ref - emean[..., :3]
+    err = np.abs(out[..., :3] - ref)
+    scale = float(np.abs(resid).max())
+    say(f"  residual scale {scale:.4g}, max err {err.max():.3g}, mean err {err.mean():.3g}")
+    assert scale > 1e-2
+    assert err.max() <= 2e-2 * scale and err.mean() <= 1e-3 * scale
+    assert np.all(out[..., 3] == 1.0)
+    d.close()
+    r.close()
+
+
+def make_pairs(d, n, w=96, h=64, inp_ens=4, tgt_ens=64):
+    scene = S.small_uniform(4096, seed=0)
+    rs = []
+    for i in range(n):
+        cam = gh.look_at((0.3 * np.sin(i), 0.2 * np.cos(i), -3.0 + 0.1 * i), (0.0, 0.0, 0.0), (0.0, -1.0, 0.0),
+                         S.FOV_60, w, h)
+        a = render(w, h, inp_ens, 100 + 10 * i, scene=scene, cam=cam)
+        t = render(w, h, tgt_ens, 5000 + 100 * i, scene=scene, cam=cam)
+        assert d.add_pair(a, t) == i
+        rs.append((a, t))
+    return rs
+
+
+def test_gradient_matches_reference():
+    d = gh.Denoiser(seed=5)
+    params = random_params(11)
+    d.set_params(params)
+    w, h = 96, 64
+    rs = make_pairs(d, 2, w, h)
+    loss, grad = d.loss_grad(2)
+    x8s, means, tgts = [], [], []
+    for a, t in rs:
+        acc = a.read_accum()
+        mom, zmin = a.read_features()
+        x8, mean = R.features(acc, mom, zmin, 1)
+        ta = t.read_accum()
+        x8s.append(x8)
+        means.append(mean)
+        tgts.append(ta[..., :3] / np.maximum(ta[..., 3:4], 1.0))
+    out = d.read_output(2, w, h)
+    ref_out, _ = R.forward(np.stack(x8s), np.stack(means), params)
+    assert np.abs(out[..., :3] - ref_out.detach().numpy()).max() < 1e-2
+    rloss, rgrad = R.loss_grad(np.stack(x8s), np.stack(means), np.stack(tgts), params)
+    say(f"[denoise grad] loss {loss:.6g} ref {rloss:.6g}")
+    assert abs(loss - rloss) <= 2e-3 * rloss
+    o = 0
+    for li, (cin, cout) in enumerate(R.LAYERS):
+        for part, size in (("W", cout * 9 * cin), ("b", cout)):
+            g, rg = grad[o:o + size], rgrad[o:o + size]
+            o += size
+            rel = np.linalg.norm(g - rg) / max(np.linalg.norm(rg), 1e-30)
+            cos = float(g @ rg / max(np.linalg.norm(g) * np.linalg.norm(rg), 1e-30))
+            say(f"  layer {li} {part}: rel {rel:.3g} cos {cos:.6f}")
+            assert rel < 3e-2 and cos > 0.999
+    for a, t in rs:
+        a.close()
+        t.close()
+    d.close()
+
+
+def test_training_lowers_loss_and_beats_plain_mean():
+    d = gh.Denoiser(seed=3)
+    rs = make_pairs(d, 6)
+    losses = d.train(150, 4, lr=2e-3, seed=7)
+    say(f"[denoise train] first {losses[:5].mean():.4g} last {losses[-5:].mean():.4g}")
+    assert np.all(np.isfinite(losses)) and losses[-10:].mean() < 0.7 * losses[:5].mean()
+    a, t = rs[0]
+    d.apply(a, fallback_ratio=0.0)
+    out = d.read()[..., :3].astype(np.float64)
+    ta = t.read_accum()
+    tgt = ta[..., :3] / np.maximum(ta[..., 3:4], 1.0)
+    acc = a.read_accum()
+    plain = acc[..., :3] / np.maximum(acc[..., 3:4], 1.0)
+    e_plain, e_den = float(((plain - tgt) ** 2).mean()), float(((out - tgt) ** 2).mean())
+    say(f"  mse plain {e_plain:.4g} denoised {e_den:.4g}")
+    assert e_den < e_plain
+    for a, t in rs:
+        a.close()
+        t.close()
+    d.close()
+
+
+def test_fallback_returns_plain_mean():
+    r = render(80, 48, 4, 3)
+    d = gh.Denoiser(seed=5)
+    d.set_params(random_params(9))
+    info = d.apply(r, fallback_ratio=1e-9)
+    assert info["fallback_tiles"] == info["tiles"] == 5 * 3
+    out = d.read()
+    _, mean = d.read_features()
+    assert np.array_equal(out[..., :3], mean[..., :3])
+    # a tile whose ensembles all agree (zero variance, e.g. background only) has an exact plain mean: any change
+    # the network makes there is unexplained, so it falls back at any ratio
+    info = d.apply(r, fallback_ratio=1e9)
+    out = d.read()
+    fb, zero_var = 0, 0
+    for ty in range(3):
+        for tx in range(5):
+            sl = (slice(16 * ty, 16 * ty + 16), slice(16 * tx, 16 * tx + 16))
+            same = np.array_equal(out[sl][..., :3], mean[sl][..., :3])
+            zv = float(mean[sl][..., 3].sum()) == 0.0
+            zero_var += zv
+            if same and not zv:
+                pytest.fail(f"tile {tx},{ty} with variance fell back")
+            fb += same
+    assert info["fallback_tiles"] == fb <= zero_var
+    with pytest.raises(gh.GpsError):
+        r2 = gh.GpsRenderer(16, 16, 1)
+        d.apply(r2)
+    d.close()
+    r.close()

@@ -574,3 +574,29 @@ def test_batch_graph_equals_separate_renders():
     b.close()
     for r in (ra, rb, rc):
         r.close()
+
+
+# ------------------------------------------------------------------ denoiser features from the resolve
+@pytest.mark.parametrize("method,spp,path", [(0, 2, 1), (2, 1, 1), (0, 1, 0)])
+def test_features_bit_exact(method, spp, path):
+    scene = to_gpu(phase_scenes()[3][1] + phase_scenes()[4][1])
+    p = gh.RenderParams(spp_side=spp, method=method, max_points_per_splat=1 << 16, path=path,
+                        background=BG, pbvr=dict(radial_correction=1))
+    w, h = 40, 32
+    r = gh.GpsRenderer(w, h, spp)
+    r.set_gaussians(scene)
+    r.set_features(True)
+    r.render_ensembles(CAM, p, 11, 3)
+    r.render_ensembles(CAM, p, 14, 4)
+    mom, zmin = r.read_features()
+    acc = r.read_accum()
+    m_mom = np.zeros((h, w, 4), np.float32)
+    m_z = np.full((h, w), np.inf, np.float32)
+    m = O.mirror_render(scene, None, 0, ocam(CAM), mparams(p), w, h, 11, 7, features=(m_mom, m_z))
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+    assert np.array_equal(mom.view(np.uint32), m_mom.view(np.uint32))
+    assert np.array_equal(zmin.view(np.uint32), m_z.view(np.uint32))
+    r.reset_accum()
+    mom, zmin = r.read_features()
+    assert not mom.any() and np.isinf(zmin).all()
+    r.close()

@@ -260,3 +260,34 @@ def test_overflow_skips_the_ensemble():
     assert r["skipped"] == 1 and r["total"] == 0 and r["stats"][7] == 1
     assert (r["accum"] == 0).all()
     assert r["stats"][0] > 0xFFFFFFFF  # the Poisson counts were still counted
+
+
+
+@pytest.mark.parametrize("method,spp", [(0, 2), (2, 1)])
+def test_mirror_features_match_per_ensemble_restatement(method, spp):
+    """Denoiser features of a 5-ensemble call equal those rebuilt from five 1-ensemble calls: each one's image
+    (a fresh accumulation holds exactly one ensemble's values) squared and summed in ensemble order in f32,
+    covered subpixels counted from its depth|id buffer, nearest depth from the keys' low 31 bits."""
+    scene = to_gpu(phase_scenes()[3][1] + phase_scenes()[4][1])
+    p = params(method=method, spp_side=spp, pbvr_radial_correction=1)
+    w, h = 40, 32
+    mom = np.zeros((h, w, 4), np.float32)
+    zmin = np.full((h, w), np.inf, np.float32)
+    r = O.mirror_render(scene, None, 0, CAM, p, w, h, 11, 5, features=(mom, zmin))
+    ex_m = np.zeros((h, w, 4), np.float32)
+    ex_z = np.full((h, w), 0x7F800000, np.uint32)
+    acc = np.zeros((h, w, 4), np.float32)
+    for e in range(5):
+        one = O.mirror_render(scene, None, 0, CAM, p, w, h, 11 + e, 1, want_buffers=True)
+        v = one["accum"][..., :3]
+        acc[..., :3] = acc[..., :3] + v
+        ex_m[..., :3] = ex_m[..., :3] + v * v
+        d = one["depth_id"].reshape(h, w, spp * spp)
+        hit = d != np.uint64(~np.uint64(0))
+        ex_m[..., 3] = ex_m[..., 3] + hit.sum(-1).astype(np.float32)
+        zb = np.where(hit, (d >> np.uint64(32)).astype(np.uint32) & np.uint32(0x7FFFFFFF), np.uint32(0x7F800000))
+        ex_z = np.minimum(ex_z, zb.min(-1))
+    assert np.array_equal(r["accum"][..., :3], acc[..., :3])
+    assert np.array_equal(mom.view(np.uint32), ex_m.view(np.uint32))
+    assert np.array_equal(zmin.view(np.uint32), ex_z)
+    assert ex_m[..., 3].max() > 0 and np.isfinite(zmin).any() and np.isinf(zmin).any()
