@@ -53,13 +53,17 @@ def psnr(a, b):
     return 120.0 if mse <= 1e-20 else 10.0 * math.log10(1.0 / mse)
 
 
-def time_to_psnr(r, cam, p, ms_per_ens, target_db=40.0, ref_ensembles=1024, ref_seed=1_000_000):
-    """Plain ensemble averaging: smallest ensemble count whose mean reaches target_db against an independent
-    ref_ensembles-ensemble render of the same view, and the GPU time that count costs."""
+def reference_image(r, cam, p, ref_ensembles=1024, ref_seed=1_000_000):
+    """The judge of the PSNR legs: an independent ref_ensembles-ensemble plain render of the view."""
     r.reset_accum()
     r.render_ensembles(cam, p, ref_seed, ref_ensembles)
     acc = r.read_accum()
-    ref = acc[..., :3] / np.maximum(acc[..., 3:4], 1.0)
+    return acc[..., :3] / np.maximum(acc[..., 3:4], 1.0)
+
+
+def time_to_psnr(r, cam, p, ms_per_ens, ref, target_db=40.0):
+    """Plain ensemble averaging: smallest ensemble count whose mean reaches target_db against `ref`, and the
+    GPU time that count costs."""
     r.reset_accum()
     done, curve = 0, []
     for e in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 768, 1024, 1536, 2048, 2560, 3072, 4096):
@@ -206,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-psnr", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the denoised leg and the held-out evaluation")
+    ap.add_argument("--heldout", type=int, default=10, help="held-out scenes (seeds 200..) for the denoiser")
     ap.add_argument("--mode", default="bench", choices=["bench", "calib", "sort3m", "batch8"],
                     help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -240,14 +246,15 @@ def main():
     views = [v for v in a.views.split(",") if v]
     if a.view not in views:
         views.append(a.view)
-    tuning = gh.get_tuning()
-    if not a.no_autotune:
-        tuning, tms = r.autotune(S.clustered_camera(a.view, W, H), p, trial_ensembles=4)
-        log(f"[autotune] {tuning} {tms:.3f} ms/ens")
+    tuning = {}
     per_view = {}
     prof_view = {}
     for v in views:
         cam = S.clustered_camera(v, W, H)
+        if not a.no_autotune:
+            t, tms = r.autotune(cam, p, trial_ensembles=4)
+            log(f"[autotune {v}] {t} {tms:.3f} ms/ens")
+        tuning[v] = gh.get_tuning()
         seed = 1
         for _ in range(a.warmup):
             r.reset_accum()
@@ -275,6 +282,21 @@ def main():
                            candidates_per_ensemble=st.candidates / ens, truncated_per_ensemble=st.truncated_splats / ens,
                            skipped=st.skipped_ensembles, visible_splats=st.visible_splats / ens)
         prof_view[v] = (prof, st, ens, particles)
+        # the same renders with the conservative cull: bit-identical images, fewer generated particles
+        r.set_cull(True)
+        r.reset_accum()
+        r.render_ensembles(cam, p, seed, a.ensembles)
+        r.wait_idle()
+        r.profile(True)
+        for _ in range(a.steps):
+            r.reset_accum()
+            r.render_ensembles(cam, p, seed, a.ensembles)
+            seed += a.ensembles
+        r.wait_idle()
+        cprof = r.profile_read()
+        r.profile(False)
+        r.set_cull(False)
+        per_view[v]["ms_per_ensemble_cull"] = sum(t for _, t in cprof.values()) / ens
         log(f"[{v}] {per_view[v]}")
         log(f"[{v}] kernels: " + ", ".join(f"{k}={t / c:.3f}ms x{c}" for k, (c, t) in prof.items()))
 
@@ -295,13 +317,39 @@ def main():
         kernels[k] = {"launches": c, "avg_ms": round(t / c, 4),
                       "GBps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 and b else None}
 
-    ttp = None
+    ttp, heldout = None, None
     if not a.no_psnr:
+        from gps_hip import denoise as D
+        den = None
+        if not a.no_denoise:
+            if not os.path.exists(D.WEIGHTS):
+                raise SystemExit(f"denoiser weights missing: {D.WEIGHTS} (python -m gps_hip.denoise)")
+            den = gh.Denoiser()
+            den.load(D.WEIGHTS)
+        # the PSNR legs render with the cull (the same images) and price each ensemble at the exact path's
+        # measured GPU time
+        r.set_cull(True)
+        r.set_features(True)
         ttp = {}
         for v in views:
-            ttp[v] = {"plain": time_to_psnr(r, S.clustered_camera(v, W, H), p, per_view[v]["ms_per_ensemble"]),
-                      "denoised": None}
-            log(f"[psnr {v}] {ttp[v]}")
+            cam = S.clustered_camera(v, W, H)
+            ref = reference_image(r, cam, p)
+            ms = per_view[v]["ms_per_ensemble"]
+            leg = {"plain": time_to_psnr(r, cam, p, ms, ref), "denoised": None}
+            if den is not None:
+                leg["denoised"] = D.denoised_time_to_psnr(r, den, cam, p, ms, ref, fallback_ratio=4.0)
+                if leg["plain"]["gpu_ms"] and leg["denoised"]["gpu_ms"]:
+                    leg["speedup"] = round(leg["plain"]["gpu_ms"] / leg["denoised"]["gpu_ms"], 2)
+            ttp[v] = leg
+            log(f"[psnr {v}] {leg}")
+        r.set_features(False)
+        r.set_cull(False)
+        if den is not None and a.heldout > 0:
+            rows = D.evaluate_heldout(den, seeds=range(200, 200 + a.heldout), log=log)
+            keys = ("psnr_plain", "psnr_denoised", "max_abs_plain", "max_abs_denoised", "fallback_rate", "denoise_ms")
+            heldout = {"scenes": len(rows), "input_ensembles": 4, "ref_ensembles": 1024,
+                       **{f"mean_{k}": round(float(np.mean([x[k] for x in rows])), 4) for k in keys}, "rows": rows}
+            den.close()
 
     cpu = None
     if not a.no_cpu:
@@ -350,6 +398,7 @@ def main():
             "gaussians_per_s": round(n * a.ensembles / (v["ms_per_ensemble"] * a.ensembles * 1e-3), 1),
             "particles_per_s": round(v["particles_per_ensemble"] / (v["ms_per_ensemble"] * 1e-3), 1),
             "time_to_40db": ttp,
+            "denoiser_heldout": heldout,
             "tuning": tuning,
         },
         "roofline": roofline,

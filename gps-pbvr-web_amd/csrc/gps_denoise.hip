@@ -133,14 +133,16 @@ enum { EPI_RELU = 0, EPI_MASK = 1, EPI_RESID = 2 };
 // EPI_RELU:  out [img][H][W][32] bf16 = relu(conv + bias)
 // EPI_MASK:  out [img][H][W][32] bf16 = conv * (mask > 0)     (mask: forward input of the layer)
 // EPI_RESID: out4 [img][H][W] float4 = (mean.rgb + conv + bias, 1); with target: dy [img][H][W][8] bf16 the
-//            loss gradient 2 (out - target) * gscale and loss partial sums per block in lpart
+//            loss gradient 2 (out - target) * gscale * iweight[img] and the weighted loss partial sums per
+//            block in lpart
 template <int CIN, int MT, int EPI>
 __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t *__restrict__ in, uint32_t W, uint32_t H,
                                                  const uint4 *__restrict__ wpk, const float *__restrict__ bias,
                                                  uint16_t *__restrict__ out, const uint16_t *__restrict__ mask,
                                                  const float4 *__restrict__ mean, float4 *__restrict__ out4,
                                                  const float4 *__restrict__ target, uint16_t *__restrict__ dy,
-                                                 float gscale, float *__restrict__ lpart) {
+                                                 float gscale, float *__restrict__ lpart,
+                                                 const float *__restrict__ iweight) {
     constexpr int KS = ksteps(CIN);
     constexpr int PSTRIDE = CIN == HID ? 40 : 8;  // LDS pixel stride in bf16 (80 B: conflict-free 16 B reads)
     __shared__ __align__(16) uint16_t tile[HALO_H * HALO_W * PSTRIDE];
@@ -219,10 +221,12 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t *__restrict__ in
             if (target) {
                 const float4 t = target[pix];
                 const float e0 = o0 - t.x, e1 = o1 - t.y, e2 = o2 - t.z;
-                lsum += e0 * e0 + e1 * e1 + e2 * e2;
+                const float wi = iweight[img];
+                lsum += wi * (e0 * e0 + e1 * e1 + e2 * e2);
+                const float gs = 2.0f * gscale * wi;
                 uint4 d;
-                d.x = (uint32_t)f2bf(2.0f * e0 * gscale) | ((uint32_t)f2bf(2.0f * e1 * gscale) << 16);
-                d.y = (uint32_t)f2bf(2.0f * e2 * gscale);
+                d.x = (uint32_t)f2bf(e0 * gs) | ((uint32_t)f2bf(e1 * gs) << 16);
+                d.y = (uint32_t)f2bf(e2 * gs);
                 d.z = 0u;
                 d.w = 0u;
                 reinterpret_cast<uint4 *>(dy)[pix] = d;
@@ -415,9 +419,11 @@ __global__ void k_dn_fallback(float4 *__restrict__ out, const float4 *__restrict
 
 // gathers training pairs into a batch: per image, X (8 ch bf16), mean, target
 __global__ void k_dn_gather(const uint16_t *__restrict__ px, const float4 *__restrict__ pmean,
-                            const float4 *__restrict__ ptgt, const uint32_t *__restrict__ idx, uint32_t X,
-                            uint16_t *__restrict__ bx, float4 *__restrict__ bmean, float4 *__restrict__ btgt) {
+                            const float4 *__restrict__ ptgt, const float *__restrict__ pweight,
+                            const uint32_t *__restrict__ idx, uint32_t X, uint16_t *__restrict__ bx,
+                            float4 *__restrict__ bmean, float4 *__restrict__ btgt, float *__restrict__ bweight) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+    if (i == 0) bweight[b] = pweight[idx[b]];
     if (i >= X) return;
     const size_t src = (size_t)idx[b] * X + i, dst = (size_t)b * X + i;
     reinterpret_cast<uint4 *>(bx)[dst] = reinterpret_cast<const uint4 *>(px)[src];
@@ -489,6 +495,7 @@ struct GpsDenoiser {
     uint32_t pair_w = 0, pair_h = 0, pairs = 0, pair_cap = 0;
     uint16_t *px = nullptr;
     float4 *pmean = nullptr, *ptgt = nullptr;
+    float *pweight = nullptr, *bweight = nullptr;
     uint32_t *bidx = nullptr;
     uint32_t bidx_cap = 0;
 };
@@ -574,10 +581,10 @@ int ensure_act(GpsDenoiser *d, uint32_t px, uint32_t imgs, uint32_t W, uint32_t 
 template <int CIN, int MT, int EPI>
 void conv(GpsDenoiser *d, const uint16_t *in, uint32_t W, uint32_t H, uint32_t imgs, const uint4 *wpk,
           const float *bias, uint16_t *out, const uint16_t *mask, const float4 *mean, float4 *out4,
-          const float4 *target, uint16_t *dy, float gscale, float *lpart) {
+          const float4 *target, uint16_t *dy, float gscale, float *lpart, const float *iweight = nullptr) {
     dim3 grid(cdiv(W, TW), cdiv(H, TH), imgs);
     hipLaunchKernelGGL((k_dn_conv<CIN, MT, EPI>), grid, dim3(256), 0, d->stream, in, W, H, wpk, bias, out, mask,
-                       mean, out4, target, dy, gscale, lpart);
+                       mean, out4, target, dy, gscale, lpart, iweight);
 }
 
 // forward over `imgs` images of W x H in d->act[0] / d->mean; output in d->out.  With target: loss gradient
@@ -590,7 +597,7 @@ void forward(GpsDenoiser *d, uint32_t W, uint32_t H, uint32_t imgs, const float4
         conv<HID, 2, EPI_RELU>(d, d->act[l], W, H, imgs, d->fwd[l], d->w + L.b[l], d->act[l + 1], nullptr,
                                nullptr, nullptr, nullptr, nullptr, 0.0f, nullptr);
     conv<HID, 1, EPI_RESID>(d, d->act[NL - 1], W, H, imgs, d->fwd[NL - 1], d->w + L.b[NL - 1], nullptr, nullptr,
-                            d->mean, d->out, target, d->dy8, gscale, d->lpart);
+                            d->mean, d->out, target, d->dy8, gscale, d->lpart, d->bweight);
 }
 
 // backward from d->dy8 (loss gradient of the output) to the gradient of every parameter in d->g
@@ -694,6 +701,8 @@ void gps_denoiser_destroy(GpsDenoiser *d) {
     dnfree(d->pmean);
     dnfree(d->ptgt);
     dnfree(d->bidx);
+    dnfree(d->pweight);
+    dnfree(d->bweight);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -718,7 +727,8 @@ int gps_denoiser_get_params(GpsDenoiser *d, float *params) {
 }
 
 // features of renderer `r` into images [0] of the activation buffers
-static int load_renderer(GpsDenoiser *d, GpsRenderer *r, uint32_t &W, uint32_t &H) {
+// `start`, if given, is recorded on the denoiser's stream once the renderer's work has finished
+static int load_renderer(GpsDenoiser *d, GpsRenderer *r, uint32_t &W, uint32_t &H, hipEvent_t start = nullptr) {
     uint32_t spp_side = 1, deg = 0;
     uint64_t n = 0;
     gps_renderer_info(r, &W, &H, &spp_side, &n, &deg);
@@ -730,6 +740,7 @@ static int load_renderer(GpsDenoiser *d, GpsRenderer *r, uint32_t &W, uint32_t &
     if (ensure_act(d, X, 1, W, H) != 0) return -1;
     // the renderer's work is ordered before ours
     if (gps_renderer_wait_idle(r) != 0) return dfail(d, "renderer wait failed");
+    if (start) (void)hipEventRecord(start, d->stream);
     hipLaunchKernelGGL(k_dn_features, dim3(cdiv(X, 256)), dim3(256), 0, d->stream, accum, mom, zmin, X,
                        (float)(spp_side * spp_side), d->act[0], d->mean);
     return 0;
@@ -741,8 +752,7 @@ int gps_denoiser_apply(GpsDenoiser *d, GpsRenderer *r, float fallback_ratio, Gps
     hipEvent_t e0, e1;
     DN_TRY(d, hipEventCreate(&e0));
     DN_TRY(d, hipEventCreate(&e1));
-    (void)hipEventRecord(e0, d->stream);
-    int rc = load_renderer(d, r, W, H);
+    int rc = load_renderer(d, r, W, H, e0);
     if (rc == 0) {
         forward(d, W, H, 1, nullptr, 0.0f);
         const dim3 tg(cdiv(W, FB_TILE), cdiv(H, FB_TILE));
@@ -813,21 +823,26 @@ int gps_denoiser_add_pair(GpsDenoiser *d, GpsRenderer *input, GpsRenderer *targe
         const uint32_t cap = d->pair_cap ? 2 * d->pair_cap : 64;
         uint16_t *px = nullptr;
         float4 *pm = nullptr, *pt = nullptr;
+        float *pw = nullptr;
         DN_TRY(d, hipMalloc(&px, cap * X * IN_CH * 2));
         DN_TRY(d, hipMalloc(&pm, cap * X * sizeof(float4)));
         DN_TRY(d, hipMalloc(&pt, cap * X * sizeof(float4)));
+        DN_TRY(d, hipMalloc(&pw, cap * sizeof(float)));
         if (d->pairs) {
             DN_TRY(d, hipMemcpyAsync(px, d->px, d->pairs * X * IN_CH * 2, hipMemcpyDeviceToDevice, d->stream));
             DN_TRY(d, hipMemcpyAsync(pm, d->pmean, d->pairs * X * sizeof(float4), hipMemcpyDeviceToDevice, d->stream));
             DN_TRY(d, hipMemcpyAsync(pt, d->ptgt, d->pairs * X * sizeof(float4), hipMemcpyDeviceToDevice, d->stream));
+            DN_TRY(d, hipMemcpyAsync(pw, d->pweight, d->pairs * sizeof(float), hipMemcpyDeviceToDevice, d->stream));
             DN_TRY(d, hipStreamSynchronize(d->stream));
         }
         dnfree(d->px);
         dnfree(d->pmean);
         dnfree(d->ptgt);
+        dnfree(d->pweight);
         d->px = px;
         d->pmean = pm;
         d->ptgt = pt;
+        d->pweight = pw;
         d->pair_cap = cap;
     }
     const size_t o = (size_t)d->pairs * X;
@@ -838,6 +853,10 @@ int gps_denoiser_add_pair(GpsDenoiser *d, GpsRenderer *input, GpsRenderer *targe
     renderer_device_buffers(target, &tacc, &tm, &tz);
     if (gps_renderer_wait_idle(target) != 0) return dfail(d, "target wait failed");
     hipLaunchKernelGGL(k_dn_mean_of, dim3(cdiv(X, 256)), dim3(256), 0, d->stream, tacc, (uint32_t)X, d->ptgt + o);
+    // pair weight sqrt(ensembles / 4): leans the loss toward the many-ensemble inputs, whose errors are
+    // small but decide where an image reaches a high PSNR, without letting them drown the few-ensemble ones
+    const float wgt = std::sqrt((float)gps_renderer_accumulated(input) / 4.0f);
+    DN_TRY(d, hipMemcpyAsync(d->pweight + d->pairs, &wgt, sizeof(float), hipMemcpyHostToDevice, d->stream));
     DN_TRY(d, hipGetLastError());
     DN_TRY(d, hipStreamSynchronize(d->stream));
     d->pairs++;
@@ -845,6 +864,20 @@ int gps_denoiser_add_pair(GpsDenoiser *d, GpsRenderer *input, GpsRenderer *targe
 }
 
 uint32_t gps_denoiser_pairs(const GpsDenoiser *d) { return d->pairs; }
+
+int gps_denoiser_set_pair_weights(GpsDenoiser *d, const float *w) {
+    if (d->pairs == 0) return 0;
+    DN_TRY(d, hipMemcpyAsync(d->pweight, w, d->pairs * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    DN_TRY(d, hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int gps_denoiser_pair_weights(GpsDenoiser *d, float *out) {
+    if (d->pairs == 0) return 0;
+    DN_TRY(d, hipMemcpyAsync(out, d->pweight, d->pairs * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    DN_TRY(d, hipStreamSynchronize(d->stream));
+    return 0;
+}
 
 int gps_denoiser_clear_pairs(GpsDenoiser *d) {
     d->pairs = 0;
@@ -860,7 +893,9 @@ static int train_steps(GpsDenoiser *d, uint32_t steps, uint32_t batch, float lr,
     if (ensure_act(d, X * batch, batch, W, H) != 0) return -1;
     if (batch > d->bidx_cap) {
         dnfree(d->bidx);
+        dnfree(d->bweight);
         DN_TRY(d, hipMalloc(&d->bidx, batch * 4));
+        DN_TRY(d, hipMalloc(&d->bweight, batch * 4));
         d->bidx_cap = batch;
     }
     float *dloss = nullptr;
@@ -881,7 +916,7 @@ static int train_steps(GpsDenoiser *d, uint32_t steps, uint32_t batch, float lr,
             break;
         }
         hipLaunchKernelGGL(k_dn_gather, dim3(cdiv(X, 256), batch), dim3(256), 0, d->stream, d->px, d->pmean, d->ptgt,
-                           d->bidx, X, d->act[0], d->mean, d->tgt);
+                           d->pweight, d->bidx, X, d->act[0], d->mean, d->tgt, d->bweight);
         forward(d, W, H, batch, d->tgt, inv);
         hipLaunchKernelGGL(k_dn_sum, dim3(1), dim3(64), 0, d->stream, d->lpart, blocks, inv, dloss + s);
         backward(d, W, H, batch);

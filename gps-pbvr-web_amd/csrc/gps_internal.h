@@ -20,7 +20,7 @@ struct GpsFrame {
 
 enum : uint32_t {
     M_GPS = 0, M_PROPORTIONAL = 1, M_EXTINCTION = 2, M_VIEW_CONDITIONED = 3, M_OCCUPANCY = 4,
-    FLAG_RADIAL = 1, FLAG_CENTRE = 2, FLAG_JITTER = 4,
+    FLAG_RADIAL = 1, FLAG_CENTRE = 2, FLAG_JITTER = 4, FLAG_CULL = 8,
 };
 
 // per-splat count source written by the view kernel (kind in the low bits of .y)

@@ -338,6 +338,38 @@ GPS_DEV T wave_sum(T v) {
     return v;
 }
 
+// Conservative test that no particle of a PBVR splat can land in the image.  A particle's world position is
+// centre + L u with L the Cholesky factor of the 3-D covariance and u three normal draws, each bounded by
+// sqrt(-2 ln(5.96e-8)) < 5.77 (Rng::normal clamps u1), so its camera-space offset is bounded per axis by
+// 5.78 * sum_j |(R L)_kj|.  Over the part of that box in front of the near plane the pixel coordinates take
+// their extremes at the box corners; the low-pass jitter adds at most 5.78 * jitter pixels.  A 2-pixel margin
+// covers the rounding of the particle arithmetic.
+GPS_DEV bool particles_offscreen(const GpsFrame &f, float4 g0, float4 ca, float4 cb) {
+    const float B = 5.78f;
+    const float L[3][3] = {{ca.x, 0.0f, 0.0f}, {ca.y, ca.z, 0.0f}, {cb.x, cb.y, cb.z}};
+    float cc[3], e[3];
+    const float px = g0.x - f.pos[0], py = g0.y - f.pos[1], pz = g0.z - f.pos[2];
+    for (int k = 0; k < 3; k++) {
+        cc[k] = dot3(f.rot[k], px, py, pz);
+        float s = 0.0f;
+        for (int j = 0; j < 3; j++)
+            s += fabsf(f.rot[k][0] * L[0][j] + f.rot[k][1] * L[1][j] + f.rot[k][2] * L[2][j]);
+        e[k] = B * s * 1.0001f + 1e-6f * (fabsf(cc[k]) + 1.0f);
+    }
+    const float zhi = cc[2] + e[2];
+    if (zhi <= f.near_z) return true;
+    const float zlo = fmaxf(cc[2] - e[2], fmaxf(f.near_z, 1e-6f));
+    const float margin = 2.0f + B * f.jitter * 1.0001f;
+    float lo[2], hi[2];
+    for (int a = 0; a < 2; a++) {
+        const float x0 = cc[a] - e[a], x1 = cc[a] + e[a];
+        const float q0 = x0 / zlo, q1 = x0 / zhi, q2 = x1 / zlo, q3 = x1 / zhi;
+        lo[a] = f.intr[a] * fminf(fminf(q0, q1), fminf(q2, q3)) + f.intr[2 + a] - margin;
+        hi[a] = f.intr[a] * fmaxf(fmaxf(q0, q1), fmaxf(q2, q3)) + f.intr[2 + a] + margin;
+    }
+    return hi[0] < 0.0f || lo[0] >= (float)f.W || hi[1] < 0.0f || lo[1] >= (float)f.H;
+}
+
 // ================================================================== k_view
 __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restrict__ splats,
                                               const float *__restrict__ sh, float4 *__restrict__ view,
@@ -405,6 +437,7 @@ __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restri
                 v3 = make_float4(-pr.b * idet, pr.a * idet, extinction_over_linear(oo), 0.0f);
                 chol_of(g1, g2, v4, v5);
                 ci = make_uint2(gpsm::bits(lambda), kind);
+                if ((f.flags & FLAG_CULL) && particles_offscreen(f, g0, v4, v5)) ci = make_uint2(0u, KIND_NONE);
             }
         }
         float4 *v = view + (size_t)VIEW_VEC4 * i;

@@ -145,6 +145,7 @@ struct GpsRenderer {
     float4 *mom = nullptr;  // denoiser features (set_features): sum of squares and covered subpixels
     float *zmin = nullptr;  // nearest depth per pixel
     bool features = false;
+    bool cull = false;      // skip PBVR splats whose particles all miss the image (set_cull)
     uint64_t *stats = nullptr;
     uint32_t *composite_buf = nullptr;
     uint64_t composite_cap = 0;
@@ -301,6 +302,7 @@ void pack_frame(const GpsRenderer *r, const GpsCamera *cam, const GpsRenderParam
     if (p->pbvr.radial_correction) flags |= FLAG_RADIAL;
     if (p->pbvr.centre_depth) flags |= FLAG_CENTRE;
     if (p->pbvr.low_pass_jitter) flags |= FLAG_JITTER;
+    if (r->cull) flags |= FLAG_CULL;
     f.flags = flags;
     f.density = p->density_scale;
     f.near_z = p->near;
@@ -758,6 +760,11 @@ int gps_renderer_set_features(GpsRenderer *r, int enable) {
     }
     launch_features_clear(r->stream, r->mom, r->zmin, X);
     HIP_TRY(r, hipGetLastError());
+    return 0;
+}
+
+int gps_renderer_set_cull(GpsRenderer *r, int enable) {
+    r->cull = enable != 0;
     return 0;
 }
 

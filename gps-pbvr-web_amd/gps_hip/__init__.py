@@ -179,6 +179,7 @@ _sig("gps_tuning_get", None, [P(Tuning)])
 _sig("gps_tuning_set", C.c_int, [P(Tuning)])
 _sig("gps_renderer_autotune", C.c_int, [_R, P(GpsCamera), P(RenderParams), u32, u32, P(Tuning), P(f64)])
 _sig("gps_renderer_set_features", C.c_int, [C.c_void_p, C.c_int])
+_sig("gps_renderer_set_cull", C.c_int, [C.c_void_p, C.c_int])
 _sig("gps_renderer_read_features", C.c_int, [C.c_void_p, P(f32), P(f32)])
 _sig("gps_denoiser_param_count", u64, [])
 _sig("gps_denoiser_create", C.c_int, [C.c_int, u32, P(C.c_void_p)])
@@ -191,6 +192,8 @@ _sig("gps_denoiser_read", C.c_int, [C.c_void_p, P(f32)])
 _sig("gps_denoiser_read_features", C.c_int, [C.c_void_p, P(f32), P(f32)])
 _sig("gps_denoiser_add_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p])
 _sig("gps_denoiser_pairs", u32, [C.c_void_p])
+_sig("gps_denoiser_pair_weights", C.c_int, [C.c_void_p, P(f32)])
+_sig("gps_denoiser_set_pair_weights", C.c_int, [C.c_void_p, P(f32)])
 _sig("gps_denoiser_clear_pairs", C.c_int, [C.c_void_p])
 _sig("gps_denoiser_train", C.c_int, [C.c_void_p, u32, u32, f32, u32, P(f32)])
 _sig("gps_denoiser_loss_grad", C.c_int, [C.c_void_p, u32, P(f32), P(f32)])
@@ -217,11 +220,12 @@ EXPORTED = [
     "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
     "gps_renderer_profile", "gps_renderer_profile_read", "gps_tuning_get", "gps_tuning_set",
     "gps_renderer_autotune", "gps_batch_create", "gps_batch_destroy", "gps_batch_render", "gps_batch_read",
-    "gps_batch_timing", "gps_renderer_set_features", "gps_renderer_read_features",
+    "gps_batch_timing", "gps_renderer_set_features", "gps_renderer_read_features", "gps_renderer_set_cull",
     "gps_denoiser_param_count", "gps_denoiser_create", "gps_denoiser_destroy", "gps_denoiser_last_error",
     "gps_denoiser_set_params", "gps_denoiser_get_params", "gps_denoiser_apply", "gps_denoiser_read",
     "gps_denoiser_read_features", "gps_denoiser_add_pair", "gps_denoiser_pairs", "gps_denoiser_clear_pairs",
-    "gps_denoiser_train", "gps_denoiser_loss_grad", "gps_denoiser_read_output",
+    "gps_denoiser_train", "gps_denoiser_loss_grad", "gps_denoiser_read_output", "gps_denoiser_pair_weights",
+    "gps_denoiser_set_pair_weights",
 ]
 
 DEBUG_KEEP_BUFFERS = 1
@@ -327,6 +331,10 @@ class GpsRenderer:
         out = np.zeros((h, w, 4), np.float32)
         self._check(lib.gps_renderer_read_accum(self._h, _ptr(out, f32)))
         return out
+
+    def set_cull(self, enable=True):
+        """Skip PBVR splats whose particles all miss the image (same image; fewer counted points)."""
+        self._check(lib.gps_renderer_set_cull(self._h, 1 if enable else 0))
 
     def set_features(self, enable=True):
         """Accumulate the denoiser's per-pixel features (sums of squares, covered subpixels, nearest depth)."""
@@ -558,6 +566,16 @@ class Denoiser:
 
     def pairs(self):
         return int(lib.gps_denoiser_pairs(self._h))
+
+    def pair_weights(self):
+        out = np.zeros(max(self.pairs(), 1), np.float32)
+        self._check(lib.gps_denoiser_pair_weights(self._h, _ptr(out, f32)))
+        return out[:self.pairs()]
+
+    def set_pair_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.pairs()
+        self._check(lib.gps_denoiser_set_pair_weights(self._h, _ptr(w, f32)))
 
     def clear_pairs(self):
         self._check(lib.gps_denoiser_clear_pairs(self._h))

@@ -184,6 +184,11 @@ int gps_math_eval(int device, int fn, const float *x, const float *y, float *out
  * subpixels (w), and keeps the nearest particle depth seen (+inf where nothing was hit).  reset_accum clears
  * them with the accumulation buffer.  Enabling clears them; disabling frees them. */
 int gps_renderer_set_features(GpsRenderer *r, int enable);
+/* Skips PBVR splats (Proportional, Extinction, ViewConditioned) none of whose particles can land in the
+ * image: a conservative bound from the clamped normal draws (no counterpart in the reference, which
+ * generates and then discards them).  Images and depth|id buffers are bit-identical with and without it;
+ * the points, candidates and visible_splats counters then leave those splats out.  Off by default. */
+int gps_renderer_set_cull(GpsRenderer *r, int enable);
 /* moments: w*h*4 floats (may be NULL); zmin: w*h floats (may be NULL) */
 int gps_renderer_read_features(GpsRenderer *r, float *moments, float *zmin);
 
@@ -213,9 +218,12 @@ int gps_denoiser_read_features(GpsDenoiser *d, float *x8, float *mean4);    /* l
 /* training pairs: the input renderer's features and the target renderer's mean (same size for every pair) */
 int gps_denoiser_add_pair(GpsDenoiser *d, GpsRenderer *input, GpsRenderer *target); /* returns the pair index */
 uint32_t gps_denoiser_pairs(const GpsDenoiser *d);
+/* per pair: sqrt(the input's ensemble count / 4); the training loss weights each pair's squared error by it */
+int gps_denoiser_pair_weights(GpsDenoiser *d, float *out);
+int gps_denoiser_set_pair_weights(GpsDenoiser *d, const float *weights); /* one per pair */
 int gps_denoiser_clear_pairs(GpsDenoiser *d);
 /* `steps` Adam steps on batches of `batch` pairs drawn with a PCG32 stream seeded by `seed`; losses[steps]
- * (mean squared error over the batch's pixels and channels) may be NULL */
+ * (the pair-weighted mean squared error over the batch's pixels and channels) may be NULL */
 int gps_denoiser_train(GpsDenoiser *d, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float *losses);
 /* loss and parameter gradient over pairs 0..batch-1 without an update (test hook) */
 int gps_denoiser_loss_grad(GpsDenoiser *d, uint32_t batch, float *loss, float *grad);

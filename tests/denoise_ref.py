@@ -73,10 +73,16 @@ def forward(x8, mean, params, grad=False):
     return (m + h).permute(0, 2, 3, 1), ws
 
 
-def loss_grad(x8, mean, target, params):
+def pair_weight(ensembles):
+    """Training weight of a pair whose input averages `ensembles` ensembles."""
+    return float(np.sqrt(np.float32(ensembles / 4.0)))
+
+
+def loss_grad(x8, mean, target, params, weights=None):
     out, ws = forward(x8, mean, params, grad=True)
     t = torch.from_numpy(np.ascontiguousarray(target[..., :3], np.float32))
-    loss = ((out - t) ** 2).mean()
+    wt = torch.ones(out.shape[0]) if weights is None else torch.tensor(np.asarray(weights, np.float32))
+    loss = (((out - t) ** 2).mean(dim=(1, 2, 3)) * wt).mean()
     loss.backward()
     g = []
     for wt, bt in ws:

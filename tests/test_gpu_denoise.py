@@ -105,7 +105,9 @@ def test_gradient_matches_reference():
     out = d.read_output(2, w, h)
     ref_out, _ = R.forward(np.stack(x8s), np.stack(means), params)
     assert np.abs(out[..., :3] - ref_out.detach().numpy()).max() < 1e-2
-    rloss, rgrad = R.loss_grad(np.stack(x8s), np.stack(means), np.stack(tgts), params)
+    wts = d.pair_weights()
+    assert np.array_equal(wts, [R.pair_weight(4), R.pair_weight(4)])
+    rloss, rgrad = R.loss_grad(np.stack(x8s), np.stack(means), np.stack(tgts), params, wts)
     say(f"[denoise grad] loss {loss:.6g} ref {rloss:.6g}")
     assert abs(loss - rloss) <= 2e-3 * rloss
     o = 0

@@ -600,3 +600,35 @@ def test_features_bit_exact(method, spp, path):
     mom, zmin = r.read_features()
     assert not mom.any() and np.isinf(zmin).all()
     r.close()
+
+
+# ------------------------------------------------------------------ conservative particle cull
+@pytest.mark.parametrize("method", [1, 2, 3])
+def test_cull_keeps_images_bit_identical(method):
+    from gps_hip import denoise as D
+    g, sh = S.clustered(60_000, seed=21, sh_degree=2)
+    rng = np.random.default_rng(method)
+    p = gh.RenderParams(spp_side=1, method=method, max_points_per_splat=16384,
+                        pbvr=dict(calibration=3, radial_correction=1))
+    culled_any = False
+    for k in range(4):
+        full = D.random_view(rng, 320, 180)
+        cam = D.window_camera(full, int(rng.integers(0, 256)), int(rng.integers(0, 116))) if k % 2 else full
+        w, h = (64, 64) if k % 2 else (320, 180)
+        out = []
+        for cull in (False, True):
+            r = gh.GpsRenderer(w, h, 1)
+            r.set_gaussians_sh(g, sh, 2)
+            r.set_cull(cull)
+            r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+            r.render_ensembles(cam, p, 7 + k, 3)
+            out.append((r.read_accum(), r.read_depth_id(), r.read_stats()))
+            r.close()
+        (a0, d0, s0), (a1, d1, s1) = out
+        say(f"[cull m{method} view {k}] visible {s0.visible_splats} -> {s1.visible_splats}, "
+            f"candidates {s0.candidates} -> {s1.candidates}")
+        assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
+        assert np.array_equal(d0, d1)
+        assert s1.candidates <= s0.candidates and s1.visible_splats <= s0.visible_splats
+        culled_any |= s1.visible_splats < s0.visible_splats
+    assert culled_any
