@@ -33,16 +33,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, n, n_vis, sh_floats, particles, S, X, ens):
-    """Algorithmic HBM bytes of one timed launch set (summed over the timed region)."""
-    if name == "view":
-        return n * (64 + 4 * sh_floats + 96 + 16 + 8)
-    if name == "count_scan":
+def kernel_bytes(name, n, n_vis, sh_floats, particles, S, X, ens, launches):
+    """Algorithmic HBM bytes of all `launches` launches of kernel `name` in the timed region (`ens` ensembles,
+    `particles` generated particles and `n_vis` visible splats per ensemble in total)."""
+    if name == "view":        # per render call: splat + SH read, 6 x float4 cache + colour + count info written
+        return launches * n * (64 + 4 * sh_floats + 96 + 16 + 8)
+    if name == "count_scan":  # per splat and ensemble: count info read, offset written
         return ens * n * (8 + 4)
-    if name == "particles":
+    if name == "particles":   # 8-byte read-modify-write per particle, the splat's cache per visible splat
         return particles * 16 + ens * n_vis * 100
-    if name == "resolve":
-        return ens * S * (8 + 8 + 16) + 32 * X
+    if name == "resolve":     # depth|id read + reset, winner colour per subpixel; accumulation read + write
+        return ens * S * (8 + 8 + 16) + launches * 32 * X
     return 0
 
 
@@ -183,6 +184,16 @@ def bench_batch8(a):
     exact = all(np.array_equal(snap[i].view(np.uint32), items[i][0].read_accum().view(np.uint32)) for i in last)
     out["per_ensemble_submits"] = {"wall_ms": round(per_ens_ms, 2), "bit_identical_to_graph": bool(exact)}
     log(f"[batch8 per-ensemble] {out['per_ensemble_submits']}")
+    # the same graph with the conservative cull (bit-identical images)
+    for r in rends:
+        r.set_cull(True)
+    b.render(items, use_graph=True)
+    t = b.timing()
+    same = all(np.array_equal(snap[i].view(np.uint32), b.read(i).view(np.uint32)) for i in last)
+    out["graph_cull"] = {"gpu_ms": round(t["gpu_ms"], 2), "bit_identical_to_graph": bool(same)}
+    log(f"[batch8 graph+cull] {out['graph_cull']}")
+    for r in rends:
+        r.set_cull(False)
     gm = out["graph"]["gpu_ms"]
     print(json.dumps({
         "mode": "batch8",
@@ -198,6 +209,40 @@ def bench_batch8(a):
         r.close()
 
 
+# FLOPs of one denoiser forward per pixel: 3x3 convolutions 7->32, 3 x 32->32, 32->3 (multiply-adds x 2)
+DENOISE_FLOP_PER_PIXEL = 2 * 9 * (7 * 32 + 3 * 32 * 32 + 32 * 3)
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def bench_denoise(a):
+    """The learned denoiser on the 560k clustered scene's near view at 1280x720 (320 ensembles, features on):
+    GPU ms per forward (features + 5 convolutions + fallback) from HIP events, and its MFMA roofline."""
+    import gps_hip as gh
+    from gps_hip import scenes as S
+    from gps_hip import denoise as D
+    W, H = 1280, 720
+    g, sh = S.clustered(560_000, seed=1, sh_degree=3)
+    r = gh.GpsRenderer(W, H, 1)
+    r.set_gaussians_sh(g, sh, 3)
+    r.set_cull(True)
+    r.set_features(True)
+    r.render_ensembles(S.clustered_camera("near", W, H), D.params_ext_c3r(), 1, 320)
+    d = gh.Denoiser()
+    d.load(D.WEIGHTS)
+    for _ in range(max(a.warmup, 1)):
+        d.apply(r, fallback_ratio=4.0)
+    ms = [d.apply(r, fallback_ratio=4.0)["gpu_ms"] for _ in range(max(a.steps, 1) * 10)]
+    avg = float(np.mean(ms))
+    flop = DENOISE_FLOP_PER_PIXEL * W * H
+    tf = flop / (avg * 1e-3) / 1e12
+    print(json.dumps({"mode": "denoise", "workload": "denoiser forward, 1280x720, 7 feature channels, 5 layers",
+                      "forwards": len(ms), "avg_ms": round(avg, 4), "flop_per_forward": flop,
+                      "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": round(tf / BF16_MFMA_PEAK_TFLOPS, 4)}}), flush=True)
+    d.close()
+    r.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,8 +256,11 @@ def main():
     ap.add_argument("--no-psnr", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoised leg and the held-out evaluation")
+    ap.add_argument("--no-cull", action="store_true", help="skip the timing of the culled renders")
     ap.add_argument("--heldout", type=int, default=10, help="held-out scenes (seeds 200..) for the denoiser")
-    ap.add_argument("--mode", default="bench", choices=["bench", "calib", "sort3m", "batch8"],
+    ap.add_argument("--tuning", default=None,
+                    help="waves,ppt,items,batch,window: fixed launch tuning (implies --no-autotune)")
+    ap.add_argument("--mode", default="bench", choices=["bench", "calib", "sort3m", "batch8", "denoise"],
                     help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
@@ -224,6 +272,11 @@ def main():
     import gps_hip as gh
     from gps_hip import scenes as S
 
+    if a.tuning:
+        v = [int(x) for x in a.tuning.split(",")]
+        gh.set_tuning(particle_waves_per_cu=v[0], particles_per_thread=v[1], sort_items_per_thread=v[2],
+                      batch_ensembles=v[3], depth_window_mib=v[4])
+        a.no_autotune = True
     if a.mode == "calib":
         x = np.ones(64 << 20, np.uint32)
         y = gh.scan_exclusive_u32(x)
@@ -235,6 +288,8 @@ def main():
         return bench_sort3m(a)
     if a.mode == "batch8":
         return bench_batch8(a)
+    if a.mode == "denoise":
+        return bench_denoise(a)
 
     W, H = 1280, 720
     g, sh = S.clustered(560_000, seed=1, sh_degree=3)
@@ -283,6 +338,10 @@ def main():
                            skipped=st.skipped_ensembles, visible_splats=st.visible_splats / ens)
         prof_view[v] = (prof, st, ens, particles)
         # the same renders with the conservative cull: bit-identical images, fewer generated particles
+        if a.no_cull:
+            log(f"[{v}] {per_view[v]}")
+            log(f"[{v}] kernels: " + ", ".join(f"{k}={t / c:.3f}ms x{c}" for k, (c, t) in prof.items()))
+            continue
         r.set_cull(True)
         r.reset_accum()
         r.render_ensembles(cam, p, seed, a.ensembles)
@@ -306,14 +365,21 @@ def main():
     Sx = W * H
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     dname, (dcount, dms) = dom
-    dbytes = kernel_bytes(dname, n, n_vis, 45, particles, Sx, Sx, ens)
+    dbytes = kernel_bytes(dname, n, n_vis, 45, particles, Sx, Sx, ens, dcount)
     achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "avg_launch_ms": round(dms / dcount, 4), "algorithmic_bytes_per_launch": int(dbytes / dcount)}
+    # HBM bytes per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE, scaled by a known-size scan) on
+    # this workload, scaled to this run's ensembles per launch
+    tpath = os.path.join(ROOT, "profiles", "particles_traffic.json")
+    if dname == "particles" and head == "near" and os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        roofline["traffic"] = int(tr["hbm_bytes_per_ensemble"] * ens / dcount)
+        roofline["traffic_source"] = "profiles/particles_traffic.json"
     kernels = {}
     for k, (c, t) in prof.items():
-        b = kernel_bytes(k, n, n_vis, 45, particles, Sx, Sx, ens)
+        b = kernel_bytes(k, n, n_vis, 45, particles, Sx, Sx, ens, c)
         kernels[k] = {"launches": c, "avg_ms": round(t / c, 4),
                       "GBps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 and b else None}
 

@@ -892,9 +892,15 @@ int gps_renderer_profile_read(GpsRenderer *r, GpsKernelTime *out, int max) {
 }
 
 // -------------------------------------------------------------- batches of views (one hipGraph per batch)
+// Items of different renderers are independent: the batch forks into up to BATCH_LANES streams (graph
+// branches under capture), one per renderer in order of first appearance (round robin), and joins them.
+constexpr int BATCH_LANES = 4;
+
 struct GpsBatch {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t lanes[BATCH_LANES] = {};
+    hipEvent_t fork = nullptr, join[BATCH_LANES] = {};
     std::vector<float4 *> outs;
     std::vector<uint64_t> out_px;
     double gpu_ms = 0.0, record_ms = 0.0;
@@ -905,9 +911,15 @@ int gps_batch_create(int device, GpsBatch **out) {
     *out = nullptr;
     GpsBatch *b = new GpsBatch();
     b->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    bool ok = hipSetDevice(device) == hipSuccess &&
+              hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int l = 0; ok && l < BATCH_LANES; l++)
+        ok = hipStreamCreateWithFlags(&b->lanes[l], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         g_create_error = "batch stream creation failed";
-        delete b;
+        gps_batch_destroy(b);
         return -1;
     }
     *out = b;
@@ -917,6 +929,14 @@ int gps_batch_create(int device, GpsBatch **out) {
 void gps_batch_destroy(GpsBatch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (int l = 0; l < BATCH_LANES; l++) {
+        if (b->lanes[l]) {
+            (void)hipStreamSynchronize(b->lanes[l]);
+            (void)hipStreamDestroy(b->lanes[l]);
+        }
+        if (b->join[l]) (void)hipEventDestroy(b->join[l]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     for (auto *o : b->outs)
         if (o) (void)hipFree(o);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -972,23 +992,48 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
         g_create_error = "hipStreamBeginCapture failed";
         return -1;
     }
+    // lanes: one per renderer in order of first appearance, round robin
+    std::vector<int> lane(n);
+    std::vector<GpsRenderer *> seen;
+    for (uint32_t i = 0; i < n; i++) {
+        int g = -1;
+        for (size_t k = 0; k < seen.size(); k++)
+            if (seen[k] == items[i].renderer) g = (int)k;
+        if (g < 0) {
+            g = (int)seen.size();
+            seen.push_back(items[i].renderer);
+        }
+        lane[i] = g % BATCH_LANES;
+    }
+    const int nl = seen.size() < (size_t)BATCH_LANES ? (int)seen.size() : BATCH_LANES;
     int rc = 0;
+    if (hipEventRecord(b->fork, b->stream) != hipSuccess) rc = -1;
+    for (int l = 0; l < nl && rc == 0; l++)
+        if (hipStreamWaitEvent(b->lanes[l], b->fork, 0) != hipSuccess) rc = -1;
     for (uint32_t i = 0; i < n && rc == 0; i++) {
         const GpsBatchItem &it = items[i];
         GpsRenderer *r = it.renderer;
+        hipStream_t st = b->lanes[lane[i]];
         uint64_t X = (uint64_t)r->W * r->H;
-        launch_accum_clear(b->stream, r->accum, X);
-        launch_features_clear(b->stream, r->mom, r->zmin, X);
-        launch_stats_clear(b->stream, r->stats, ST_SKIPPED, 1);
+        launch_accum_clear(st, r->accum, X);
+        launch_features_clear(st, r->mom, r->zmin, X);
+        launch_stats_clear(st, r->stats, ST_SKIPPED, 1);
         r->accumulated = 0;
-        rc = enqueue_ensembles(r, b->stream, it.camera, it.params, it.first_seed, it.count);
+        rc = enqueue_ensembles(r, st, it.camera, it.params, it.first_seed, it.count);
         if (rc == 0) {
-            launch_copy_f4(b->stream, b->outs[i], r->accum, X);
+            launch_copy_f4(st, b->outs[i], r->accum, X);
         } else {
             g_create_error = r->err;
         }
         r->enqueue_stream = r->stream;
     }
+    // join every lane that was forked, also after a failure, so that a capture can end
+    for (int l = 0; l < nl; l++)
+        if (hipEventRecord(b->join[l], b->lanes[l]) != hipSuccess ||
+            hipStreamWaitEvent(b->stream, b->join[l], 0) != hipSuccess)
+            rc = -1;
+    if (rc != 0 && g_create_error.empty()) g_create_error = "batch recording failed";
+    b->nodes = 0;
     if (use_graph) {
         if (hipStreamEndCapture(b->stream, &graph) != hipSuccess) rc = -1;
         if (rc == 0 && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = -1;
