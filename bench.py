@@ -65,9 +65,10 @@ def reference_image(r, cam, p, ref_ensembles=1024, ref_seed=1_000_000):
 def time_to_psnr(r, cam, p, ms_per_ens, ref, target_db=40.0):
     """Plain ensemble averaging: smallest ensemble count whose mean reaches target_db against `ref`, and the
     GPU time that count costs."""
+    from gps_hip.denoise import ladder
     r.reset_accum()
     done, curve = 0, []
-    for e in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 768, 1024, 1536, 2048, 2560, 3072, 4096):
+    for e in ladder(1):
         r.render_ensembles(cam, p, 1 + done, e - done)
         done = e
         acc = r.read_accum()

@@ -215,6 +215,8 @@ GPS_DEV Proj project(const GpsFrame &f, float4 g0, float4 g1, float4 g2) {
 }
 
 // gps_splat.wgsl:210-251 (evaluated once per splat per view)
+// `sh` holds splat `id`'s rest coefficients at sh + id * 3 * sh_stride (global memory, or a block's LDS copy
+// with a block-local id)
 GPS_DEV float4 splat_colour(const GpsFrame &f, uint32_t id, float4 g0, float4 g3, const float *__restrict__ sh) {
     float res[3] = {g3.x, g3.y, g3.z};
     uint32_t deg = f.sh_degree;
@@ -371,13 +373,31 @@ GPS_DEV bool particles_offscreen(const GpsFrame &f, float4 g0, float4 ca, float4
 }
 
 // ================================================================== k_view
+// The block's splats and SH rest coefficients are contiguous; they are staged in LDS with 16-byte loads
+// across the block (a per-thread gather of 180-byte records would re-fetch cache lines), and each thread
+// then reads its own record from LDS.
 __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restrict__ splats,
                                               const float *__restrict__ sh, float4 *__restrict__ view,
                                               uint2 *__restrict__ cinfo, float4 *__restrict__ colour) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ float4 s_stage[];  // 4 float4 per splat, then 3 * sh_stride floats per splat
+    const uint32_t i0 = blockIdx.x * blockDim.x;
+    const uint32_t nb = min((uint32_t)blockDim.x, f.N - i0);
+    for (uint32_t t = threadIdx.x; t < 4u * nb; t += blockDim.x) s_stage[t] = splats[4u * i0 + t];
+    float *s_sh = reinterpret_cast<float *>(s_stage + 4u * blockDim.x);
+    if (f.sh_degree > 0u) {
+        const uint32_t per = 3u * f.sh_stride, cnt = nb * per;
+        const float *src = sh + (size_t)i0 * per;   // 16-byte aligned: 256 * 12 * sh_stride bytes per block
+        const uint32_t c4 = cnt / 4u;
+        for (uint32_t t = threadIdx.x; t < c4; t += blockDim.x)
+            reinterpret_cast<float4 *>(s_sh)[t] = reinterpret_cast<const float4 *>(src)[t];
+        for (uint32_t t = 4u * c4 + threadIdx.x; t < cnt; t += blockDim.x) s_sh[t] = src[t];
+    }
+    __syncthreads();
+    const uint32_t i = i0 + threadIdx.x;
     if (i >= f.N) return;
-    float4 g0 = splats[4u * i], g1 = splats[4u * i + 1u], g2 = splats[4u * i + 2u], g3 = splats[4u * i + 3u];
-    colour[i] = splat_colour(f, i, g0, g3, sh);
+    const uint32_t li = threadIdx.x;
+    float4 g0 = s_stage[4u * li], g1 = s_stage[4u * li + 1u], g2 = s_stage[4u * li + 2u], g3 = s_stage[4u * li + 3u];
+    colour[i] = splat_colour(f, li, g0, g3, s_sh);
     uint2 ci = make_uint2(0u, KIND_NONE);
     Proj pr = project(f, g0, g1, g2);
     float o = clamp(g0.w, 0.0f, 1.0f);
@@ -1043,7 +1063,8 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((
 void launch_view(hipStream_t s, const GpsFrame &f, const float4 *splats, const float *sh, float4 *view, uint2 *cinfo,
                  float4 *colour) {
     if (f.N == 0) return;
-    hipLaunchKernelGGL(k_view, dim3(blocks_for(f.N, 256)), dim3(256), 0, s, f, splats, sh, view, cinfo, colour);
+    const size_t lds = 256 * (4 * sizeof(float4) + (f.sh_degree > 0 ? 3 * f.sh_stride * sizeof(float) : 0));
+    hipLaunchKernelGGL(k_view, dim3(blocks_for(f.N, 256)), dim3(256), lds, s, f, splats, sh, view, cinfo, colour);
 }
 
 void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uint32_t first_seed, uint32_t nens,

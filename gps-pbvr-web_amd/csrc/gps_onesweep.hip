@@ -1,7 +1,7 @@
 // Onesweep LSD radix sort of 64-bit keys with 32-bit values, and the depth-sorted particle resolve.
 //
 // Sort (8-bit digits, stable):
-//   k_os_hist   one read of the keys builds the histograms of all eight digit positions at once
+//   k_os_hist   one read of the keys builds the histograms of every digit position the sort uses at once
 //   k_os_pass   per digit pass, one kernel: ticketed 4096-key tiles rank their keys with wave64 ballots,
 //               publish per-digit counts and resolve their per-digit global offsets by decoupled look-back
 //               (one digit per thread), stage the tile in LDS in digit order, then write it out in
@@ -25,8 +25,9 @@ constexpr int OS_WAVES = OS_BLOCK / 64;
 constexpr uint64_t OS_VAL_MASK = (1ull << 38) - 1ull;
 constexpr uint32_t OS_EPOCH_MASK = (1u << 24) - 1u;
 
+// histograms of digit positions dlo..7 (the lower ones are not sorted and stay zero)
 __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict__ keys, uint64_t n,
-                                                      uint32_t *__restrict__ ghist) {
+                                                      uint32_t *__restrict__ ghist, int dlo) {
     __shared__ uint32_t h[8][256];
     for (int i = threadIdx.x; i < 8 * 256; i += OS_BLOCK) (&h[0][0])[i] = 0u;
     __syncthreads();
@@ -36,6 +37,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict
         const bool full = i - (threadIdx.x & 63) + 64 <= n;   // every lane of this wave holds a key
 #pragma unroll
         for (int d = 0; d < 8; d++) {
+            if (d < dlo) continue;
             uint32_t dg = (uint32_t)((k >> (8 * d)) & 255u);
             // a wave whose 64 keys share the digit adds 64 once instead of 64 serialised LDS atomics
             uint32_t first = __builtin_amdgcn_readfirstlane(dg);
@@ -70,7 +72,9 @@ __device__ __forceinline__ uint32_t scan256_exclusive(uint32_t v, uint32_t *s_tm
     return base + inc - v;
 }
 
-// OS_ITEMS keys per thread: tiles of 256 * OS_ITEMS keys (LDS staging 12 B per key)
+// OS_ITEMS keys per thread: tiles of 256 * OS_ITEMS keys.  The tile is staged in digit order through one
+// 8-byte-per-key LDS buffer, keys first and then values, so that a 4096-key tile takes 32 KB of LDS and four
+// blocks fit on a CU.
 template <int OS_ITEMS>
 __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
@@ -83,7 +87,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
     __shared__ uint32_t s_toff[256];
     __shared__ uint64_t s_gpre[256];
     __shared__ uint64_t s_keys[OS_TILE];
-    __shared__ uint32_t s_vals[OS_TILE];
+    uint32_t *s_vals = reinterpret_cast<uint32_t *>(s_keys);
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
     for (uint32_t d = lane; d < 257; d += 64) s_wh[w][d] = 0u;
@@ -92,7 +96,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
     const uint64_t tile = s_tile;
     const uint64_t lt = (1ull << lane) - 1ull;
     uint64_t key[OS_ITEMS];
-    uint32_t val[OS_ITEMS], dig[OS_ITEMS], rank[OS_ITEMS];
+    uint32_t val[OS_ITEMS], rank[OS_ITEMS];  // rank, then the key's position in the staged tile (~0: none)
 #pragma unroll
     for (int j = 0; j < OS_ITEMS; j++) {
         uint64_t i = tile * OS_TILE + (uint64_t)w * (64 * OS_ITEMS) + (uint64_t)j * 64 + lane;
@@ -100,7 +104,6 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
         key[j] = valid ? kin[i] : 0ull;
         val[j] = valid ? vin[i] : 0u;
         uint32_t d = valid ? (uint32_t)((key[j] >> shift) & 255ull) : 256u;
-        dig[j] = d;
         uint64_t peers;
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
         if (__ballot(d != d0) == 0ull) {
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
         }
         uint32_t before = s_wh[w][d];
         __builtin_amdgcn_wave_barrier();
-        rank[j] = before + (uint32_t)__popcll(peers & lt);
+        rank[j] = valid ? before + (uint32_t)__popcll(peers & lt) : 0xFFFFFFFFu;
         if ((peers & lt) == 0ull) s_wh[w][d] = before + (uint32_t)__popcll(peers);
         __builtin_amdgcn_wave_barrier();
     }
@@ -158,25 +161,40 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
         s_gpre[t] = (uint64_t)gbase + excl;
     }
     __syncthreads();
-    // stage the tile in digit order
+    // stage the tile's keys in digit order, write them out; then the values through the same buffer
+    // (destinations fit 32 bits: the sort takes fewer than 2^32 keys)
 #pragma unroll
     for (int j = 0; j < OS_ITEMS; j++) {
-        uint32_t d = dig[j];
-        if (d < 256u) {
-            uint32_t pos = s_toff[d] + s_wh[w][d] + rank[j];
-            s_keys[pos] = key[j];
-            s_vals[pos] = val[j];
+        if (rank[j] != 0xFFFFFFFFu) {
+            const uint32_t d = (uint32_t)((key[j] >> shift) & 255ull);
+            rank[j] += s_toff[d] + s_wh[w][d];
+            s_keys[rank[j]] = key[j];
         }
     }
     __syncthreads();
     const uint64_t tbase = tile * OS_TILE;
     const uint32_t cnt = (uint32_t)(n - tbase < (uint64_t)OS_TILE ? n - tbase : (uint64_t)OS_TILE);
-    for (uint32_t q = t; q < cnt; q += OS_BLOCK) {
-        uint64_t k = s_keys[q];
-        uint32_t d = (uint32_t)((k >> shift) & 255ull);
-        uint64_t g = s_gpre[d] + (q - s_toff[d]);
-        kout[g] = k;
-        vout[g] = s_vals[q];
+    constexpr int QN = (OS_TILE + OS_BLOCK - 1) / OS_BLOCK;
+    uint32_t gdst[QN];
+#pragma unroll
+    for (int qi = 0; qi < QN; qi++) {
+        const uint32_t q = t + (uint32_t)qi * OS_BLOCK;
+        if (q < cnt) {
+            uint64_t k = s_keys[q];
+            uint32_t d = (uint32_t)((k >> shift) & 255ull);
+            gdst[qi] = (uint32_t)(s_gpre[d] + (q - s_toff[d]));
+            kout[gdst[qi]] = k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < OS_ITEMS; j++)
+        if (rank[j] != 0xFFFFFFFFu) s_vals[rank[j]] = val[j];
+    __syncthreads();
+#pragma unroll
+    for (int qi = 0; qi < QN; qi++) {
+        const uint32_t q = t + (uint32_t)qi * OS_BLOCK;
+        if (q < cnt) vout[gdst[qi]] = s_vals[q];
     }
     if (t == 0 && tile * OS_TILE + OS_TILE >= n) *ticket = 0u;   // last tile: all tickets drawn
 }
@@ -304,7 +322,7 @@ int onesweep_sort(OnesweepState *s, hipStream_t st, uint64_t *k[2], uint32_t *v[
         OProf p(prof, st, "sort_hist");
         unsigned grid = nb(n, OS_BLOCK * 16);
         if (grid > 4096) grid = 4096;
-        hipLaunchKernelGGL(k_os_hist, dim3(grid), dim3(OS_BLOCK), 0, st, k[0], n, s->ghist);
+        hipLaunchKernelGGL(k_os_hist, dim3(grid), dim3(OS_BLOCK), 0, st, k[0], n, s->ghist, first_digit);
     }
     std::vector<uint32_t> h(2048);
     O_TRY(hipMemcpyAsync(h.data(), s->ghist, 2048 * 4, hipMemcpyDeviceToHost, st));
@@ -348,6 +366,7 @@ void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *fir
 // standalone stable sort (test hook): in place on host arrays
 extern "C" int gps_sort_pairs_u64(int device, uint64_t *keys, uint32_t *vals, uint64_t n) {
     if (n == 0) return 0;
+    if (n >= (1ull << 32)) return -1;
     if (hipSetDevice(device) != hipSuccess) return -1;
     std::string err;
     OnesweepState *s = onesweep_create();

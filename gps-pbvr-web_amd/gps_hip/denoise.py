@@ -109,14 +109,28 @@ def train(d, steps=16000, batch=16, lr=1e-3, seed=1, chunks=8, log=print):
     return np.concatenate(losses)
 
 
-def denoised_time_to_psnr(r, d, cam, p, ms_per_ens, ref, target_db=40.0, ladder=None, fallback_ratio=4.0,
+def ladder(lo=1, hi=4096, steps_per_octave=8):
+    """Ensemble counts lo..hi in geometric steps of 2^(1/steps_per_octave) (about 9%), rounded, distinct."""
+    out, k = [], 0
+    while True:
+        e = int(round(2.0 ** (k / steps_per_octave)))
+        k += 1
+        if e < lo or (out and e == out[-1]):
+            continue
+        if e > hi:
+            break
+        out.append(e)
+    return tuple(out)
+
+
+def denoised_time_to_psnr(r, d, cam, p, ms_per_ens, ref, target_db=40.0, ladder_=None, fallback_ratio=4.0,
                           first_seed=1):
-    """Smallest ensemble count on `ladder` whose denoised mean reaches target_db against `ref`, and its GPU time:
-    the ensembles' render time plus the denoiser's measured time.  `r` must have its features enabled."""
-    ladder = ladder or (4, 8, 16, 32, 64, 128, 192, 256, 320, 384, 448, 512, 640, 768, 1024, 1536, 2048, 3072, 4096)
+    """Smallest ensemble count on the ladder whose denoised mean reaches target_db against `ref`, and its GPU
+    time: the ensembles' render time plus the denoiser's measured time.  `r` must have its features enabled."""
+    ladder_ = ladder_ or ladder(4)
     r.reset_accum()
     done, curve = 0, []
-    for e in ladder:
+    for e in ladder_:
         r.render_ensembles(cam, p, first_seed + done, e - done)
         done = e
         info = d.apply(r, fallback_ratio=fallback_ratio)

@@ -167,7 +167,8 @@ int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out);
 int gps_renderer_read_particle_keys(GpsRenderer *r, uint64_t *keys, uint32_t *subpixels, uint64_t max,
                                     uint64_t *count);
 
-/* Stable LSD radix sort of n (u64 key, u32 value) pairs in place (the onesweep sort of the sorted path). */
+/* Stable LSD radix sort of n (u64 key, u32 value) pairs in place (the onesweep sort of the sorted path);
+ * n must be below 2^32. */
 int gps_sort_pairs_u64(int device, uint64_t *keys, uint32_t *vals, uint64_t n);
 
 /* Phase-0 unit kernels (crates/gps-render/src/lib.rs:99-275; shaders/p0_sample.wgsl, gps_scan.wgsl).
