@@ -21,7 +21,6 @@
 namespace {
 
 constexpr int OS_BLOCK = 256;
-constexpr int OS_WAVES = OS_BLOCK / 64;
 constexpr uint64_t OS_VAL_MASK = (1ull << 38) - 1ull;
 constexpr uint32_t OS_EPOCH_MASK = (1u << 24) - 1u;
 
@@ -55,8 +54,9 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict
     }
 }
 
-// exclusive scan of 256 values held one per thread (256 threads)
-__device__ __forceinline__ uint32_t scan256_exclusive(uint32_t v, uint32_t *s_tmp) {
+// exclusive scan of the values held one per thread over the block (s_tmp: one word per wave); every thread
+// of the block calls it
+__device__ __forceinline__ uint32_t scan_block_exclusive(uint32_t v, uint32_t *s_tmp) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t inc = v;
 #pragma unroll
@@ -72,18 +72,19 @@ __device__ __forceinline__ uint32_t scan256_exclusive(uint32_t v, uint32_t *s_tm
     return base + inc - v;
 }
 
-// OS_ITEMS keys per thread: tiles of 256 * OS_ITEMS keys.  The tile is staged in digit order through one
-// 8-byte-per-key LDS buffer, keys first and then values, so that a 4096-key tile takes 32 KB of LDS and four
-// blocks fit on a CU.
-template <int OS_ITEMS>
-__global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+// BLK threads x OS_ITEMS keys per thread per tile.  The tile is staged in digit order through one
+// 8-byte-per-key LDS buffer, keys first and then values (a 4096-key tile takes 32 KB of LDS).  Threads 0..255
+// own one digit each for the per-digit counts, offsets and look-back.
+template <int BLK, int OS_ITEMS>
+__global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                       uint64_t n, uint32_t shift, const uint32_t *__restrict__ ghist,
                                                       uint32_t *ticket, uint64_t *status, uint32_t epoch) {
-    constexpr int OS_TILE = OS_BLOCK * OS_ITEMS;
+    constexpr int WAVES = BLK / 64;
+    constexpr int OS_TILE = BLK * OS_ITEMS;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_tmp[OS_WAVES];
-    __shared__ uint32_t s_wh[OS_WAVES][257];
+    __shared__ uint32_t s_tmp[WAVES];
+    __shared__ uint32_t s_wh[WAVES][257];
     __shared__ uint32_t s_toff[256];
     __shared__ uint64_t s_gpre[256];
     __shared__ uint64_t s_keys[OS_TILE];
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
     for (uint32_t d = lane; d < 257; d += 64) s_wh[w][d] = 0u;
     // global base of digit t for this pass
-    const uint32_t gbase = scan256_exclusive(ghist[t], s_tmp);
+    const uint32_t gbase = scan_block_exclusive(t < 256u ? ghist[t] : 0u, s_tmp);
     const uint64_t tile = s_tile;
     const uint64_t lt = (1ull << lane) - 1ull;
     uint64_t key[OS_ITEMS];
@@ -125,16 +126,17 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
     __syncthreads();
     // per digit t: offsets of the waves inside the tile and the tile's count
     uint32_t run = 0;
-    for (int ww = 0; ww < OS_WAVES; ww++) {
-        uint32_t c = s_wh[ww][t];
-        s_wh[ww][t] = run;
-        run += c;
-    }
+    if (t < 256u)
+        for (int ww = 0; ww < WAVES; ww++) {
+            uint32_t c = s_wh[ww][t];
+            s_wh[ww][t] = run;
+            run += c;
+        }
     const uint32_t tcount = run;
-    const uint32_t toff = scan256_exclusive(tcount, s_tmp);
-    s_toff[t] = toff;
+    const uint32_t toff = scan_block_exclusive(tcount, s_tmp);
+    if (t < 256u) s_toff[t] = toff;
     // decoupled look-back for digit t
-    {
+    if (t < 256u) {
         const uint64_t tag = (uint64_t)(epoch & OS_EPOCH_MASK) << 40;
         uint64_t excl = 0;
         if (tile == 0) {
@@ -174,11 +176,11 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
     __syncthreads();
     const uint64_t tbase = tile * OS_TILE;
     const uint32_t cnt = (uint32_t)(n - tbase < (uint64_t)OS_TILE ? n - tbase : (uint64_t)OS_TILE);
-    constexpr int QN = (OS_TILE + OS_BLOCK - 1) / OS_BLOCK;
+    constexpr int QN = OS_ITEMS;
     uint32_t gdst[QN];
 #pragma unroll
     for (int qi = 0; qi < QN; qi++) {
-        const uint32_t q = t + (uint32_t)qi * OS_BLOCK;
+        const uint32_t q = t + (uint32_t)qi * BLK;
         if (q < cnt) {
             uint64_t k = s_keys[q];
             uint32_t d = (uint32_t)((k >> shift) & 255ull);
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_pass(const uint64_t *__restrict
     __syncthreads();
 #pragma unroll
     for (int qi = 0; qi < QN; qi++) {
-        const uint32_t q = t + (uint32_t)qi * OS_BLOCK;
+        const uint32_t q = t + (uint32_t)qi * BLK;
         if (q < cnt) vout[gdst[qi]] = s_vals[q];
     }
     if (t == 0 && tile * OS_TILE + OS_TILE >= n) *ticket = 0u;   // last tile: all tickets drawn
@@ -307,8 +309,10 @@ int onesweep_sort(OnesweepState *s, hipStream_t st, uint64_t *k[2], uint32_t *v[
         O_TRY(hipMalloc(&s->ticket, 4));
         O_TRY(hipMemset(s->ticket, 0, 4));
     }
-    const int items = gps_tuning_current().sort_items_per_thread == 8 ? 8 : 16;
-    const uint64_t tile_keys = (uint64_t)OS_BLOCK * items;
+    // tiles of 256 x items keys; 32 runs as 512 threads x 16, 64 as 1024 threads x 16
+    const uint32_t ti = gps_tuning_current().sort_items_per_thread;
+    const int items = (ti == 8 || ti == 16 || ti == 32) ? (int)ti : 64;
+    const uint64_t tile_keys = (uint64_t)256 * items;
     uint64_t tiles = (n + tile_keys - 1) / tile_keys;
     if (tiles > s->status_tiles) {
         if (s->status) (void)hipFree(s->status);
@@ -335,11 +339,19 @@ int onesweep_sort(OnesweepState *s, hipStream_t st, uint64_t *k[2], uint32_t *v[
         if (constant) continue;
         OProf p(prof, st, "sort_pass");
         if (items == 8)
-            hipLaunchKernelGGL(k_os_pass<8>, dim3((unsigned)tiles), dim3(OS_BLOCK), 0, st, k[cur], v[cur],
+            hipLaunchKernelGGL((k_os_pass<256, 8>), dim3((unsigned)tiles), dim3(256), 0, st, k[cur], v[cur],
+                               k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
+                               s->epoch);
+        else if (items == 16)
+            hipLaunchKernelGGL((k_os_pass<256, 16>), dim3((unsigned)tiles), dim3(256), 0, st, k[cur], v[cur],
+                               k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
+                               s->epoch);
+        else if (items == 32)
+            hipLaunchKernelGGL((k_os_pass<512, 16>), dim3((unsigned)tiles), dim3(512), 0, st, k[cur], v[cur],
                                k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
                                s->epoch);
         else
-            hipLaunchKernelGGL(k_os_pass<16>, dim3((unsigned)tiles), dim3(OS_BLOCK), 0, st, k[cur], v[cur],
+            hipLaunchKernelGGL((k_os_pass<1024, 16>), dim3((unsigned)tiles), dim3(1024), 0, st, k[cur], v[cur],
                                k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
                                s->epoch);
         s->epoch = (s->epoch + 1u) & OS_EPOCH_MASK;

@@ -40,7 +40,7 @@ void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *fir
 namespace {
 
 thread_local std::string g_create_error;
-GpsTuningDev g_tuning = {8u, 4u, 16u, 0u, 64u};
+GpsTuningDev g_tuning = {8u, 4u, 64u, 0u, 64u};
 
 constexpr uint64_t MAX_SPLATS = 1ull << 30;
 constexpr uint32_t LUT_O = 65, LUT_V = 129;
@@ -1090,7 +1090,8 @@ void gps_tuning_get(GpsTuning *out) {
 int gps_tuning_set(const GpsTuning *t) {
     uint32_t ppt = t->particles_per_thread;
     if (!(ppt == 1 || ppt == 2 || ppt == 4 || ppt == 8)) return -1;
-    if (!(t->sort_items_per_thread == 8 || t->sort_items_per_thread == 16)) return -1;
+    const uint32_t si = t->sort_items_per_thread;
+    if (!(si == 8 || si == 16 || si == 32 || si == 64)) return -1;
     if (t->particle_waves_per_cu < 1 || t->particle_waves_per_cu > 64) return -1;
     if (t->batch_ensembles > MAX_BATCH) return -1;
     if (t->depth_window_mib < 1 || t->depth_window_mib > (1u << 20)) return -1;
@@ -1124,7 +1125,7 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
     double best_ms = 1e30;
     const uint32_t waves[] = {4u, 8u, 16u};
     const uint32_t ppts[] = {1u, 2u, 4u, 8u};
-    const uint32_t items[] = {8u, 16u};
+    const uint32_t items[] = {8u, 16u, 32u, 64u};
     const bool sorted_path = p->path == GPS_PATH_PARTICLE_SORTED;
     hipEvent_t e0, e1;
     HIP_TRY(r, hipEventCreate(&e0));

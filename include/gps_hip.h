@@ -256,7 +256,8 @@ void gps_batch_timing(const GpsBatch *b, double *gpu_ms, double *record_ms, uint
 typedef struct GpsTuning {
     uint32_t particle_waves_per_cu;  /* persistent particle grid = CUs x this (1..64), default 8 */
     uint32_t particles_per_thread;   /* 1, 2, 4 (default) or 8 */
-    uint32_t sort_items_per_thread;  /* radix sort tile: 256 threads x 8 or 16 (default) keys */
+    uint32_t sort_items_per_thread;  /* radix sort tile of 256 x this keys: 8, 16, 32 or 64 (default; run
+                                        as 512 or 1024 threads x 16) */
     uint32_t batch_ensembles;        /* ensembles per particle launch (1..64); 0 = from depth_window_mib */
     uint32_t depth_window_mib;       /* with batch_ensembles 0: as many ensembles per launch as keep their
                                         depth|id buffers (8 bytes per subpixel each) within this many MiB,
@@ -264,7 +265,7 @@ typedef struct GpsTuning {
 } GpsTuning;
 void gps_tuning_get(GpsTuning *out);
 int gps_tuning_set(const GpsTuning *t);
-/* Autotuner: times every candidate (waves per CU 4/8/16 x particles per thread 1/2/4/8, and sort tiles 8/16
+/* Autotuner: times every candidate (waves per CU 4/8/16 x particles per thread 1/2/4/8, and sort tiles 8/16/32/64
  * on the sorted particle path) on trial_ensembles ensembles of this scene and view, then the depth window
  * (8 .. 4096 MiB, with batch_ensembles 0) on max(trial_ensembles, 2 x the ensembles per launch it gives),
  * keeps the fastest as the process tuning, and restores the accumulation buffer, counters and submitted

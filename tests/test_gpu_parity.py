@@ -422,6 +422,23 @@ def test_onesweep_sort_stable(n):
     assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
 
 
+@pytest.mark.parametrize("items", [8, 32, 64])
+@pytest.mark.parametrize("n", [8191, 8193, 16385, 1_000_003])
+def test_onesweep_sort_tiles(items, n):
+    """Every sort tile size (256 x 8, 256 x 16 default, 512 x 16, 1024 x 16) gives numpy's stable order."""
+    rng = np.random.default_rng(n + items)
+    keys = rng.integers(0, 1 << 20, n, dtype=np.uint64) << np.uint64(40) | rng.integers(0, 1 << 12, n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    saved = gh.get_tuning()
+    try:
+        gh.set_tuning(sort_items_per_thread=items)
+        k2, v2 = gh.sort_pairs_u64(keys, vals)
+    finally:
+        gh.set_tuning(**saved)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k2, keys[order]) and np.array_equal(v2, vals[order])
+
+
 def test_onesweep_sort_full_range_keys():
     n = 3_000_000
     rng = np.random.default_rng(9)
@@ -492,7 +509,7 @@ def test_tunings_are_bit_identical():
     saved = gh.get_tuning()
     ref = None
     try:
-        for waves, ppt, items, batch in ((8, 4, 16, 0), (4, 1, 8, 3), (16, 8, 16, 1), (1, 2, 8, 7)):
+        for waves, ppt, items, batch in ((8, 4, 16, 0), (4, 1, 8, 3), (16, 8, 32, 1), (1, 2, 8, 7)):
             say(f"[tuning] waves={waves} ppt={ppt} items={items} batch={batch}")
             gh.set_tuning(particle_waves_per_cu=waves, particles_per_thread=ppt, sort_items_per_thread=items,
                           batch_ensembles=batch)
