@@ -27,6 +27,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #include "f32math.h"
 
@@ -297,12 +298,19 @@ static void splat_colour(const Ctx *cx, uint32_t id, float out[3]) {
 /* gps_splat.wgsl:47-50 (64-bit counters) */
 static inline void stat_add(Ctx *c, int k, uint64_t n) { c->stats[k] += n; }
 
+/* atomic u32 minimum (the particle loop may run on several threads; the result is order independent) */
+static inline void atomic_min_u32(uint32_t *p, uint32_t v) {
+    uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+
 /* the two 32-bit passes of gps_splat.wgsl:276-281 / 526-531 */
 static inline void write_subpixel(Ctx *c, uint32_t mode, uint32_t idx, uint32_t key, uint32_t id) {
     if (mode == 0u) {
-        if (key < c->depth[idx]) c->depth[idx] = key;
+        atomic_min_u32(&c->depth[idx], key);
     } else if (c->depth[idx] == key) {
-        if (id < c->winner[idx]) c->winner[idx] = id;
+        atomic_min_u32(&c->winner[idx], id);
     }
 }
 
@@ -651,6 +659,9 @@ static void particle(Ctx *c, uint32_t i, uint32_t k, uint32_t mode) {
     }
 }
 
+/* Threads of the particle loop (OpenMP); 1 runs it on the calling thread only. */
+EXPORT void mir_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 /* Denoiser feature outputs of the next mir_render_ensembles calls (NULL: off).  Test hook, not thread safe. */
 static float *g_feat_mom, *g_feat_zmin;
 EXPORT void mir_set_features(float *moments, float *zmin) {
@@ -760,10 +771,21 @@ EXPORT int mir_render_ensembles(const MGaussian *g, uint32_t n, const float *sh_
                 total = 0u;
                 stats[S_SKIPPED] += 1;
             }
+            /* every particle is seeded by (splat, k, frame) alone, so splats can be split over threads; each
+               thread keeps its own counters, added at the end (the minima are atomic) */
             for (uint32_t mode = 0; mode < 2u; mode++) {
                 if (total == 0u) break;
-                for (uint32_t i = 0; i < n; i++)
-                    for (uint32_t k = 0; k < counts[i]; k++) particle(&c, i, k, mode);
+#pragma omp parallel
+                {
+                    Ctx lc = c;
+                    uint64_t ls[16] = {0};
+                    lc.stats = ls;
+#pragma omp for schedule(dynamic, 16)
+                    for (int64_t si = 0; si < (int64_t)n; si++)
+                        for (uint32_t k = 0; k < counts[si]; k++) particle(&lc, (uint32_t)si, k, mode);
+                    for (int q = 0; q < 16; q++)
+                        if (ls[q]) __atomic_fetch_add(&stats[q], ls[q], __ATOMIC_RELAXED);
+                }
             }
         } else if (n > 0) {
             for (uint32_t mode = 0; mode < 2u; mode++)

@@ -145,6 +145,7 @@ _sig("mir_poisson", u32, [f32, u32])
 for _n in ("mir_expf_n", "mir_logf_n", "mir_sinf_n", "mir_cosf_n"):
     _sig(_n, None, [P(f32), P(f32), sz])
 _sig("mir_set_features", None, [P(f32), P(f32)])
+_sig("mir_set_threads", None, [C.c_int])
 _sig("mir_render_ensembles", C.c_int, [P(MGaussian), u32, P(f32), u32, P(f32), P(OrcCamera), P(MParams), u32, u32,
                                         u32, u32, u32, P(f32), P(u64), P(u32), P(u32), P(u64), P(u32), P(u32)])
 _sig("mir_splat_colours", None, [P(MGaussian), u32, P(f32), u32, P(OrcCamera), P(MParams), u32, u32, P(f32)])
