@@ -42,10 +42,8 @@ GPS_DEV float rint(float x) { return __builtin_rintf(x); }
 GPS_DEV float sqrt(float x) { return __builtin_sqrtf(x); }
 GPS_DEV float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-GPS_DEV float exp(float x) {
-    if (x != x) return x;
-    if (x > 88.72283935546875f) return fbits(0x7f800000u);
-    if (x < -103.972084045410156f) return 0.0f;
+// exp of x in [-103.97, 88.72] (the checked entry points below handle the rest)
+GPS_DEV float exp_core(float x) {
     float kf = rint(x * LOG2E);
     int k = (int)kf;
     float r = fma(-kf, LN2_HI, x);
@@ -63,19 +61,23 @@ GPS_DEV float exp(float x) {
     return p * fbits((uint32_t)(k + 127) << 23);
 }
 
-GPS_DEV float log(float x) {
+GPS_DEV float exp(float x) {
     if (x != x) return x;
-    if (x < 0.0f) return fbits(0x7fc00000u);
-    if (x == 0.0f) return fbits(0xff800000u);
-    if (x == fbits(0x7f800000u)) return x;
-    int e = 0;
+    if (x > 88.72283935546875f) return fbits(0x7f800000u);
+    if (x < -103.972084045410156f) return 0.0f;
+    return exp_core(x);
+}
+
+// exp of a non-NaN x <= 88.72 (the pipeline calls it with x <= 0): the same bits as exp() there
+GPS_DEV float exp_nonpos(float x) {
+    if (x < -103.972084045410156f) return 0.0f;
+    return exp_core(x);
+}
+
+// log of a positive, normal, finite x times 2^-e0 (e0 undoes a subnormal prescale)
+GPS_DEV float log_core(float x, int e0) {
     uint32_t u = bits(x);
-    if (u < 0x00800000u) {
-        x = x * 0x1p23f;
-        u = bits(x);
-        e = -23;
-    }
-    e += (int)(u >> 23) - 127;
+    int e = e0 + (int)(u >> 23) - 127;
     float m = fbits((u & 0x007fffffu) | 0x3f800000u);
     if (m > SQRT2) {
         m = m * 0.5f;
@@ -93,6 +95,18 @@ GPS_DEV float log(float x) {
     float ef = (float)e;
     return fma(ef, LN2_HI, fma(ef, LN2_LO, lm));
 }
+
+GPS_DEV float log(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return fbits(0x7fc00000u);
+    if (x == 0.0f) return fbits(0xff800000u);
+    if (x == fbits(0x7f800000u)) return x;
+    if (bits(x) < 0x00800000u) return log_core(x * 0x1p23f, -23);
+    return log_core(x, 0);
+}
+
+// log of a positive, normal, finite x: the same bits as log() there
+GPS_DEV float log_pos(float x) { return log_core(x, 0); }
 
 GPS_DEV float sin_poly(float r) {
     float z = r * r;
@@ -122,13 +136,8 @@ GPS_DEV float reduce(float x, int &q) {
     return r;
 }
 
-// sin and cos of the same argument (the pipeline always needs both)
-GPS_DEV void sincos(float x, float &s, float &c) {
-    if (x != x) {
-        s = x;
-        c = x;
-        return;
-    }
+// sin and cos of a finite argument (the pipeline always needs both)
+GPS_DEV void sincos_fin(float x, float &s, float &c) {
     int q;
     float r = reduce(x, q);
     float ps = sin_poly(r), pc = cos_poly(r);
@@ -138,6 +147,15 @@ GPS_DEV void sincos(float x, float &s, float &c) {
     case 2: s = -ps; c = -pc; break;
     default: s = -pc; c = ps; break;
     }
+}
+
+GPS_DEV void sincos(float x, float &s, float &c) {
+    if (x != x) {
+        s = x;
+        c = x;
+        return;
+    }
+    sincos_fin(x, s, c);
 }
 
 GPS_DEV float sin(float x) { float s, c; sincos(x, s, c); return s; }

@@ -71,10 +71,10 @@ struct Rng {
         }
         float u1 = fmax(next(), 5.96e-8f);
         float u2 = next();
-        float r = gpsm::sqrt(-2.0f * gpsm::log(u1));
+        float r = gpsm::sqrt(-2.0f * gpsm::log_pos(u1));   // u1 >= 5.96e-8: positive and normal
         float th = 2.0f * PI_F * u2;
         float sn, cs;
-        gpsm::sincos(th, sn, cs);
+        gpsm::sincos_fin(th, sn, cs);
         spare = r * sn;
         has_spare = true;
         return r * cs;
@@ -119,7 +119,7 @@ GPS_DEV float inv_dilog(float tgt, float hi_in) {
 
 GPS_DEV float radius_lut(const float *__restrict__ lut, float o, float u) {
     if (o <= 0.0f) return 0.0f;
-    float v = -gpsm::log(fmax(1.0f - u, 5.96e-8f));
+    float v = -gpsm::log_pos(fmax(1.0f - u, 5.96e-8f));
     float fo = (1.0f - gpsm::sqrt(fmax(1.0f - fmin(o, 1.0f), 0.0f))) * 64.0f;
     float fv = fmin(gpsm::sqrt(v) * (float)(128.0 / 4.123105625617661), 128.0f);
     uint32_t i0 = f2u(fo);
@@ -146,7 +146,7 @@ GPS_DEV float corrected_radius(float o, float u) {
 GPS_DEV uint32_t poisson(Rng &r, float lambda) {
     if (lambda <= 0.0f) return 0u;
     if (lambda < 30.0f) {
-        float l = gpsm::exp(-lambda);
+        float l = gpsm::exp_nonpos(-lambda);
         uint32_t k = 0u;
         float p = 1.0f;
         for (;;) {
@@ -158,7 +158,9 @@ GPS_DEV uint32_t poisson(Rng &r, float lambda) {
     }
     float u1 = fmax(r.next(), 5.96e-8f);
     float u2 = r.next();
-    float z = gpsm::sqrt(-2.0f * gpsm::log(u1)) * gpsm::cos(2.0f * PI_F * u2);
+    float sn, cs;
+    gpsm::sincos_fin(2.0f * PI_F * u2, sn, cs);
+    float z = gpsm::sqrt(-2.0f * gpsm::log_pos(u1)) * cs;
     return f2u(fmax(gpsm::rint(lambda + gpsm::sqrt(lambda) * z), 0.0f));
 }
 
@@ -277,7 +279,7 @@ GPS_DEV float pixel_density_scale(const GpsFrame &f, float depth) {
 
 GPS_DEV float extinction_over_linear(float x) {
     if (x < 1e-4f) return 1.0f + 0.5f * x;
-    return -gpsm::log(1.0f - fmin(x, 1.0f - 1e-6f)) / x;
+    return -gpsm::log_pos(1.0f - fmin(x, 1.0f - 1e-6f)) / x;   // argument in [1e-6, 1]
 }
 
 GPS_DEV float pbvr_lambda(const GpsFrame &f, const Proj &pr, float o) {
@@ -642,7 +644,7 @@ GPS_DEV Hit emit_gps(const GpsFrame &f, Rng &r, float4 v0, float4 v1, const floa
     float rad = f.exact_radius == 0u ? radius_lut(lut, v0.w, u) : corrected_radius(v0.w, u);
     float th = 2.0f * PI_F * r.next();
     float sn, cs;
-    gpsm::sincos(th, sn, cs);
+    gpsm::sincos_fin(th, sn, cs);
     float wx = rad * cs, wy = rad * sn;
     float pxf = v0.x + v1.x * wx, pyf = v0.y + (v1.y * wx + v1.z * wy);
     h.key = gpsm::bits(v0.z) | 0x80000000u;
@@ -687,7 +689,7 @@ GPS_DEV Hit emit_pbvr(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) {
         float r2 = c.w * dvx * dvx + 2.0f * d.x * dvx * dvy + d.y * dvy * dvy;
         // radial_keep, gps_splat.wgsl:434-438, with h(min(o, 1-1e-6)) cached per splat in d.z
         float oo = fmin(a.w, 1.0f - 1e-6f);
-        float y = oo * gpsm::exp(-0.5f * fmax(r2, 0.0f));
+        float y = oo * gpsm::exp_nonpos(-0.5f * fmax(r2, 0.0f));
         float keep = clamp(extinction_over_linear(y) / d.z, 0.0f, 1.0f);
         if (r.next() >= keep) return h;
     }
@@ -725,10 +727,10 @@ GPS_DEV Hit emit_occupancy(const GpsFrame &f, Rng &r, uint32_t k, const float4 *
     float uvy = ((float)(sub / side) + n2) / (float)side;
     float dx = (float)xyx + uvx - a.x, dy = (float)xyy + uvy - a.y;
     float r2 = fmax(inv.x * dx * dx + 2.0f * inv.y * dx * dy + inv.z * dy * dy, 0.0f);
-    float alpha = fmin(a.w * gpsm::exp(-0.5f * r2), 1.0f - 1e-6f);
+    float alpha = fmin(a.w * gpsm::exp_nonpos(-0.5f * r2), 1.0f - 1e-6f);
     float coverage;
-    if (sparse) coverage = -gpsm::log(1.0f - alpha) / inv.w;
-    else coverage = 1.0f - gpsm::exp(f.density * gpsm::log(1.0f - alpha));
+    if (sparse) coverage = -gpsm::log_pos(1.0f - alpha) / inv.w;   // 1 - alpha in [1e-6, 1]
+    else coverage = 1.0f - gpsm::exp(f.density * gpsm::log_pos(1.0f - alpha));
     if (r.next() >= coverage) return h;
     h.idx = (xyy * f.W + xyx) * samples + sub;
     h.key = gpsm::bits(a.z) | 0x80000000u;
