@@ -210,6 +210,56 @@ def bench_batch8(a):
         r.close()
 
 
+def bench_ply(a):
+    """gps-verify bench-ply (crates/gps-verify/src/main.rs:104-143): per-ensemble GPU time on a 3DGS PLY file for
+    GPS, PBVR Extinction C3+R and screen occupancy at three framed views (the robust sphere at x1, x1.85 and
+    x3.2 distance, up -Y, yaw 0.6, pitch 0.25, fov 0.9), 6 single-ensemble renders each after a warm-up."""
+    import gps_hip as gh
+    path = a.ply or os.path.join(ROOT, "tests", "golden", "synthetic_sh1.ply")
+    w, h = (int(x) for x in a.size.split("x"))
+    g = gh.load_ply(path)
+    gpu = g.to_gpu()
+    r = gh.GpsRenderer(w, h, a.spp)
+    if g.sh_degree > 0:
+        r.set_gaussians_sh(gpu, g.sh_rest, g.sh_degree)
+    else:
+        r.set_gaussians(gpu)
+    c, rad = gh.robust_sphere(gpu["pos_opacity"][:, :3])
+    log(f"[bench-ply] {path}: {g.count} Gaussians, SH degree {g.sh_degree}, {w}x{h} spp side {a.spp}")
+    env_max = int(os.environ.get("MAXPTS", "16384"))
+    rows = []
+    for vname, mul in (("near (x1.25)", 1.0), ("mid (x2.3)", 1.85), ("far (x4)", 3.2)):
+        cam = gh.orbit_camera(tuple(c), gh.frame_sphere_distance(rad) * mul, 0.6, 0.25, (0.0, -1.0, 0.0), 0.9, w, h)
+        for mname, method, cal, radial in (("GPS", gh.Method.Gps, 3, 0), ("PBVR Ext C3+R", gh.Method.Extinction, 3, 1),
+                                           ("Screen occupancy", gh.Method.ScreenOccupancy, 0, 0)):
+            p = gh.RenderParams(spp_side=a.spp, max_points_per_splat=env_max, method=method,
+                                pbvr=dict(calibration=cal, radial_correction=radial))
+            r.render_ensembles(cam, p, 1, 1)
+            r.wait_idle()
+            r.reset_stats()
+            r.profile(True)
+            reps = 6
+            t0 = time.perf_counter()
+            for i in range(reps):
+                r.render_ensembles(cam, p, 100 + i, 1)
+                r.wait_idle()
+            wall = (time.perf_counter() - t0) * 1000.0 / reps
+            prof = r.profile_read()
+            r.profile(False)
+            st = r.read_stats()
+            row = {"view": vname, "method": mname, "gpu_ms_per_ensemble": round(sum(t for _, t in prof.values()) / reps, 4),
+                   "wall_ms_per_ensemble": round(wall, 4), "points_per_ensemble": st.points // reps,
+                   "truncated_per_ensemble": st.truncated_splats // reps}
+            if method == gh.Method.ScreenOccupancy:
+                row.update(candidates_per_ensemble=st.candidates // reps, skipped=st.skipped_ensembles,
+                           orphan=st.orphan_subpixels)
+            rows.append(row)
+            log(f"[bench-ply] {row}")
+    print(json.dumps({"mode": "bench-ply", "file": os.path.basename(path), "gaussians": g.count,
+                      "sh_degree": g.sh_degree, "size": a.size, "spp_side": a.spp, "rows": rows}), flush=True)
+    r.close()
+
+
 # FLOPs of one denoiser forward per pixel: 3x3 convolutions 7->32, 3 x 32->32, 32->3 (multiply-adds x 2)
 DENOISE_FLOP_PER_PIXEL = 2 * 9 * (7 * 32 + 3 * 32 * 32 + 32 * 3)
 BF16_MFMA_PEAK_TFLOPS = 2500.0
@@ -261,7 +311,11 @@ def main():
     ap.add_argument("--heldout", type=int, default=10, help="held-out scenes (seeds 200..) for the denoiser")
     ap.add_argument("--tuning", default=None,
                     help="waves,ppt,items,batch,window: fixed launch tuning (implies --no-autotune)")
-    ap.add_argument("--mode", default="bench", choices=["bench", "calib", "sort3m", "batch8", "denoise"],
+    ap.add_argument("--ply", default=None, help="bench-ply: the 3DGS PLY file (default: the test fixture)")
+    ap.add_argument("--size", default="480x270", help="bench-ply: WxH")
+    ap.add_argument("--spp", type=int, default=1, help="bench-ply: SPP side")
+    ap.add_argument("--mode", default="bench",
+                    choices=["bench", "calib", "sort3m", "batch8", "denoise", "bench-ply"],
                     help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
@@ -291,6 +345,8 @@ def main():
         return bench_batch8(a)
     if a.mode == "denoise":
         return bench_denoise(a)
+    if a.mode == "bench-ply":
+        return bench_ply(a)
 
     W, H = 1280, 720
     g, sh = S.clustered(560_000, seed=1, sh_degree=3)

@@ -944,6 +944,13 @@ void gps_batch_destroy(GpsBatch *b) {
 }
 
 int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use_graph) {
+    b->gpu_ms = b->record_ms = 0.0;
+    b->nodes = 0;
+    if (n == 0) return 0;
+    if (!items) {
+        g_create_error = "null batch items";
+        return -1;
+    }
     if (hipSetDevice(b->device) != hipSuccess) {
         g_create_error = "hipSetDevice failed";
         return -1;
@@ -1110,15 +1117,25 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
     if (check_params(r, p) != 0) return -1;
     if (trial_ensembles == 0) trial_ensembles = 1;
     const uint64_t X = (uint64_t)r->W * r->H;
-    float4 *acc_save = nullptr;
+    float4 *acc_save = nullptr, *mom_save = nullptr;
     uint64_t *st_save = nullptr;
+    float *z_save = nullptr;
     HIP_TRY(r, hipMalloc(&acc_save, X * sizeof(float4)));
-    if (hipMalloc(&st_save, ST_WORDS * 8) != hipSuccess) {
+    if (hipMalloc(&st_save, ST_WORDS * 8) != hipSuccess ||
+        (r->features && (hipMalloc(&mom_save, X * sizeof(float4)) != hipSuccess ||
+                         hipMalloc(&z_save, X * sizeof(float)) != hipSuccess))) {
         dfree(acc_save);
+        dfree(st_save);
+        dfree(mom_save);
+        dfree(z_save);
         return fail(r, "hipMalloc failed");
     }
     HIP_TRY(r, hipMemcpyAsync(acc_save, r->accum, X * sizeof(float4), hipMemcpyDeviceToDevice, r->stream));
     HIP_TRY(r, hipMemcpyAsync(st_save, r->stats, ST_WORDS * 8, hipMemcpyDeviceToDevice, r->stream));
+    if (r->features) {
+        HIP_TRY(r, hipMemcpyAsync(mom_save, r->mom, X * sizeof(float4), hipMemcpyDeviceToDevice, r->stream));
+        HIP_TRY(r, hipMemcpyAsync(z_save, r->zmin, X * sizeof(float), hipMemcpyDeviceToDevice, r->stream));
+    }
     const uint32_t submitted = r->accumulated;
     const GpsTuningDev start = g_tuning;
     GpsTuningDev best = start;
@@ -1183,9 +1200,15 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
     (void)hipEventDestroy(e1);
     (void)hipMemcpyAsync(r->accum, acc_save, X * sizeof(float4), hipMemcpyDeviceToDevice, r->stream);
     (void)hipMemcpyAsync(r->stats, st_save, ST_WORDS * 8, hipMemcpyDeviceToDevice, r->stream);
+    if (r->features) {
+        (void)hipMemcpyAsync(r->mom, mom_save, X * sizeof(float4), hipMemcpyDeviceToDevice, r->stream);
+        (void)hipMemcpyAsync(r->zmin, z_save, X * sizeof(float), hipMemcpyDeviceToDevice, r->stream);
+    }
     (void)hipStreamSynchronize(r->stream);
     dfree(acc_save);
     dfree(st_save);
+    dfree(mom_save);
+    dfree(z_save);
     r->accumulated = submitted;
     if (rc != 0) return -1;
     if (best_out) gps_tuning_get(best_out);

@@ -198,6 +198,12 @@ _sig("gps_denoiser_clear_pairs", C.c_int, [C.c_void_p])
 _sig("gps_denoiser_train", C.c_int, [C.c_void_p, u32, u32, f32, u32, P(f32)])
 _sig("gps_denoiser_loss_grad", C.c_int, [C.c_void_p, u32, P(f32), P(f32)])
 _sig("gps_denoiser_read_output", C.c_int, [C.c_void_p, u32, P(f32)])
+_sig("gps_ply_load", C.c_int, [C.c_char_p, u64, P(C.c_void_p)])
+_sig("gps_ply_free", None, [C.c_void_p])
+_sig("gps_ply_last_error", C.c_char_p, [])
+_sig("gps_ply_info", None, [C.c_void_p, P(u64), P(u32), P(C.c_int)])
+_sig("gps_ply_read", None, [C.c_void_p, P(f32), P(f32), P(f32), P(f32), P(f32), P(f32)])
+_sig("gps_ply_to_gpu", None, [C.c_void_p, C.c_void_p])
 _sig("gps_batch_create", C.c_int, [C.c_int, P(C.c_void_p)])
 _sig("gps_batch_destroy", None, [C.c_void_p])
 _sig("gps_batch_render", C.c_int, [C.c_void_p, P(BatchItem), u32, C.c_int])
@@ -225,7 +231,8 @@ EXPORTED = [
     "gps_denoiser_set_params", "gps_denoiser_get_params", "gps_denoiser_apply", "gps_denoiser_read",
     "gps_denoiser_read_features", "gps_denoiser_add_pair", "gps_denoiser_pairs", "gps_denoiser_clear_pairs",
     "gps_denoiser_train", "gps_denoiser_loss_grad", "gps_denoiser_read_output", "gps_denoiser_pair_weights",
-    "gps_denoiser_set_pair_weights",
+    "gps_denoiser_set_pair_weights", "gps_ply_load", "gps_ply_free", "gps_ply_last_error", "gps_ply_info",
+    "gps_ply_read", "gps_ply_to_gpu",
 ]
 
 DEBUG_KEEP_BUFFERS = 1
@@ -479,6 +486,83 @@ class Batch:
         g, rec, nodes = f64(), f64(), u64()
         lib.gps_batch_timing(self._h, C.byref(g), C.byref(rec), C.byref(nodes))
         return dict(gpu_ms=g.value, record_ms=rec.value, graph_nodes=nodes.value)
+
+
+# ------------------------------------------------------------------ 3DGS PLY ingest (crates/gps-io/src/lib.rs)
+PLY_ERRORS = {1: "NotPly", 2: "BadHeader", 3: "UnsupportedFormat", 4: "MissingProperty", 5: "TooManyVertices",
+              6: "Truncated", 7: "BadAscii", 8: "NonFinite", 9: "BadShCount"}
+PLY_MAX_VERTICES = 200_000_000
+
+
+class PlyError(GpsError):
+    """gps_io::PlyError: `kind` is the variant name, str(e) its message."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+        self.kind = PLY_ERRORS.get(code, "Unknown")
+
+
+class PlyGaussians:
+    """gps_io::Gaussians: file-form struct-of-arrays (log-scales, opacity logits, (w, x, y, z) rotations,
+    f_dc, channel-major f_rest per splat)."""
+
+    def __init__(self, handle):
+        c, d, b = u64(), u32(), C.c_int()
+        lib.gps_ply_info(handle, C.byref(c), C.byref(d), C.byref(b))
+        self.count, self.sh_degree, self.binary = int(c.value), int(d.value), bool(b.value)
+        n, r = self.count, self.sh_rest_per_channel()
+        self.position = np.zeros((n, 3), np.float32)
+        self.log_scale = np.zeros((n, 3), np.float32)
+        self.rot_wxyz = np.zeros((n, 4), np.float32)
+        self.opacity_logit = np.zeros(n, np.float32)
+        self.f_dc = np.zeros((n, 3), np.float32)
+        self.sh_rest = np.zeros(n * 3 * r, np.float32)
+        lib.gps_ply_read(handle, _ptr(self.position, f32), _ptr(self.log_scale, f32), _ptr(self.rot_wxyz, f32),
+                         _ptr(self.opacity_logit, f32), _ptr(self.f_dc, f32),
+                         _ptr(self.sh_rest, f32) if self.sh_rest.size else None)
+        self._gpu = np.zeros(n, GAUSSIAN_DTYPE)
+        if n:
+            lib.gps_ply_to_gpu(handle, self._gpu.ctypes.data)
+
+    def sh_rest_per_channel(self):
+        return (self.sh_degree + 1) ** 2 - 1 if self.sh_degree else 0
+
+    def to_gpu(self):
+        """GpuGaussian::from_activated(activated(i)) for every splat (gps.rs:41-49)."""
+        return self._gpu.copy()
+
+
+def load_ply(data):
+    """gps_io::load_ply: `data` is the file's bytes or a path."""
+    if not isinstance(data, (bytes, bytearray, memoryview)):
+        with open(data, "rb") as fh:
+            data = fh.read()
+    data = bytes(data)
+    h = C.c_void_p()
+    rc = lib.gps_ply_load(data, len(data), C.byref(h))
+    if rc != 0:
+        raise PlyError(rc, lib.gps_ply_last_error().decode())
+    try:
+        return PlyGaussians(h)
+    finally:
+        lib.gps_ply_free(h)
+
+
+def robust_sphere(points):
+    """OrbitCamera::robust_sphere (crates/gps-core/src/camera.rs:70-84): per-axis median centre and the 75th
+    percentile distance to it; None for no points."""
+    p = np.asarray(points, np.float64).reshape(-1, 3)
+    if p.shape[0] == 0:
+        return None
+    c = np.array([np.sort(p[:, a])[p.shape[0] // 2] for a in range(3)])
+    d = np.sort(np.sqrt(((p - c) ** 2).sum(axis=1)))
+    return c, float(d[min(int(d.size * 0.75), d.size - 1)])
+
+
+def frame_sphere_distance(radius, fov_y=0.9):
+    """OrbitCamera::frame_sphere (camera.rs:87-90): the orbit distance that fills about 80% of the view."""
+    return (max(radius, 1e-6) / math.sin(0.5 * fov_y)) * 1.25
 
 
 DENOISER_LAYERS = ((8, 32), (32, 32), (32, 32), (32, 32), (32, 3))  # (cin as stored, cout)

@@ -182,8 +182,10 @@ int gps_math_eval(int device, int fn, const float *x, const float *y, float *out
 
 /* ---- denoiser features (no counterpart in the reference).  When enabled, every resolve also accumulates per
  * pixel the sum over ensembles of the squared ensemble pixel value (r, g, b) and the number of covered
- * subpixels (w), and keeps the nearest particle depth seen (+inf where nothing was hit).  reset_accum clears
- * them with the accumulation buffer.  Enabling clears them; disabling frees them. */
+ * subpixels (w), and keeps the nearest particle depth seen (+inf where nothing was hit), in
+ * gps_renderer_render_ensembles and batches on every particle path (gps_renderer_render_sorted's alpha path
+ * does not resolve subpixels and leaves them alone).  reset_accum clears them with the accumulation buffer.
+ * Enabling clears them; disabling frees them. */
 int gps_renderer_set_features(GpsRenderer *r, int enable);
 /* Skips PBVR splats (Proportional, Extinction, ViewConditioned) none of whose particles can land in the
  * image: a conservative bound from the clamped normal draws (no counterpart in the reference, which
@@ -251,6 +253,37 @@ int gps_batch_read(GpsBatch *b, uint32_t item, float *out);
 /* GPU time of the last gps_batch_render (graph launch to completion), host time to record it, graph nodes */
 void gps_batch_timing(const GpsBatch *b, double *gpu_ms, double *record_ms, uint64_t *graph_nodes);
 
+/* ---- 3DGS PLY ingest (crates/gps-io/src/lib.rs:1-373; the activation of crates/gps-render/src/gps.rs:41-49).
+ * ASCII or binary little-endian, one vertex element first, scalar properties of any PLY type, SH degree from
+ * 0 / 9 / 24 / 45 f_rest_* properties; other properties are skipped.  Values are kept in file form
+ * (log-scales, opacity logit, (w, x, y, z) rotation, f_dc, channel-major f_rest) until gps_ply_to_gpu.
+ * Errors: a GPS_PLY_* code (the PlyError variants, lib.rs:16-26) and gps_ply_last_error()'s message. */
+#define GPS_PLY_MAX_VERTICES 200000000ull /* lib.rs:13 */
+enum {
+    GPS_PLY_NOT_PLY = 1,
+    GPS_PLY_BAD_HEADER = 2,
+    GPS_PLY_UNSUPPORTED = 3,
+    GPS_PLY_MISSING_PROPERTY = 4,
+    GPS_PLY_TOO_MANY_VERTICES = 5,
+    GPS_PLY_TRUNCATED = 6,
+    GPS_PLY_BAD_ASCII = 7,
+    GPS_PLY_NON_FINITE = 8,
+    GPS_PLY_BAD_SH_COUNT = 9
+};
+typedef struct GpsPly GpsPly;
+/* load_ply (lib.rs:260): parses `len` bytes; 0 and *out, or a GPS_PLY_* code */
+int gps_ply_load(const uint8_t *bytes, uint64_t len, GpsPly **out);
+void gps_ply_free(GpsPly *g);
+const char *gps_ply_last_error(void); /* message of the calling thread's last failed load */
+void gps_ply_info(const GpsPly *g, uint64_t *count, uint32_t *sh_degree, int *binary);
+/* file-form arrays (any may be NULL): 3, 3, 4, 1, 3 and 3 * rest-per-channel floats per splat */
+void gps_ply_read(const GpsPly *g, float *position, float *log_scale, float *rot_wxyz, float *opacity_logit,
+                  float *f_dc, float *sh_rest);
+/* GpuGaussian::from_activated(Gaussians::activated(i)) for every splat: opacity = sigmoid(logit) in f64,
+ * colour = f_dc * SH_C0 + 0.5 in f64, the rest as stored; pass sh_rest from gps_ply_read with the degree to
+ * gps_renderer_set_gaussians */
+void gps_ply_to_gpu(const GpsPly *g, GpsGaussian *out);
+
 /* ---- launch tuning (process-wide; no counterpart in the reference).  Rendered bits never depend on it:
  * it only changes how the work is split into launches.  Renderers pick it up on their next call. */
 typedef struct GpsTuning {
@@ -268,8 +301,8 @@ int gps_tuning_set(const GpsTuning *t);
 /* Autotuner: times every candidate (waves per CU 4/8/16 x particles per thread 1/2/4/8, and sort tiles 8/16/32/64
  * on the sorted particle path) on trial_ensembles ensembles of this scene and view, then the depth window
  * (8 .. 4096 MiB, with batch_ensembles 0) on max(trial_ensembles, 2 x the ensembles per launch it gives),
- * keeps the fastest as the process tuning, and restores the accumulation buffer, counters and submitted
- * count.  Writes the chosen tuning and its GPU ms per ensemble. */
+ * keeps the fastest as the process tuning, and restores the accumulation buffer, denoiser features, counters
+ * and submitted count.  Writes the chosen tuning and its GPU ms per ensemble. */
 int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p, uint32_t first_seed,
                           uint32_t trial_ensembles, GpsTuning *best, double *best_ms_per_ensemble);
 

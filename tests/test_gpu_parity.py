@@ -534,11 +534,16 @@ def test_autotune_keeps_history():
     try:
         r = gh.GpsRenderer(128, 128, 2)
         r.set_gaussians(scene)
+        r.set_features(True)
         r.render_ensembles(cam, p, 1, 5)
         before, st_before, sub = r.read_accum(), r.read_stats(), r.submitted()
+        mom_before, z_before = r.read_features()
         best, ms = r.autotune(cam, p, trial_ensembles=2)
         say(f"[autotune] {best} {ms:.4f} ms/ens")
         assert np.array_equal(r.read_accum(), before) and r.read_stats() == st_before and r.submitted() == sub
+        mom_after, z_after = r.read_features()
+        assert np.array_equal(mom_after, mom_before)
+        assert np.array_equal(z_after.view(np.uint32), z_before.view(np.uint32))
         assert ms > 0 and best["particles_per_thread"] in (1, 2, 4, 8)
         r.render_ensembles(cam, p, 6, 3)
         after = r.read_accum()
@@ -588,6 +593,8 @@ def test_batch_graph_equals_separate_renders():
     assert np.array_equal(rb.read_accum(), rc.read_accum())
     with pytest.raises(gh.GpsError):
         b.render([(ra, cam_a, pa.copy(path=2), 1, 1)])
+    b.render([])  # an empty batch is a no-op
+    assert b.timing()["graph_nodes"] == 0
     b.close()
     for r in (ra, rb, rc):
         r.close()
@@ -649,3 +656,22 @@ def test_cull_keeps_images_bit_identical(method):
         assert s1.candidates <= s0.candidates and s1.visible_splats <= s0.visible_splats
         culled_any |= s1.visible_splats < s0.visible_splats
     assert culled_any
+
+
+# ------------------------------------------------------------------ 3DGS PLY ingest -> renderer
+@pytest.mark.parametrize("method,path", [(0, 1), (0, 0), (2, 1), (4, 1)])
+def test_ply_fixture_renders_bit_exact(method, path):
+    """tests/fixtures/synthetic_sh1.ply through gps_ply_load / gps_ply_to_gpu, framed as gps-verify bench-ply
+    frames a file (crates/gps-verify/src/main.rs:107-126: robust sphere, up -Y, yaw 0.6, pitch 0.25)."""
+    import os
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "synthetic_sh1.ply")
+    g = gh.load_ply(fixture)
+    gg = g.to_gpu()
+    c, rad = gh.robust_sphere(gg["pos_opacity"][:, :3])
+    cam = gh.orbit_camera(tuple(c), gh.frame_sphere_distance(rad), 0.6, 0.25, (0.0, -1.0, 0.0), 0.9, 96, 72)
+    p = gh.RenderParams(spp_side=1, method=method, path=path, max_points_per_splat=16384,
+                        pbvr=dict(radial_correction=1))
+    say(f"[ply] method {method} path {path}")
+    gpu, m = run_pair(gg, g.sh_rest, g.sh_degree, cam, p, 96, 72, 1, 3)
+    assert_exact(gpu, m)
+    assert gpu["stats"].points > 0
