@@ -777,10 +777,7 @@ int gps_renderer_read_features(GpsRenderer *r, float *moments, float *zmin) {
     return 0;
 }
 
-int gps_renderer_read_stats(GpsRenderer *r, GpsStats *out) {
-    uint64_t w[ST_WORDS];
-    HIP_TRY(r, hipMemcpyAsync(w, r->stats, sizeof w, hipMemcpyDeviceToHost, r->stream));
-    HIP_TRY(r, hipStreamSynchronize(r->stream));
+static void stats_from_words(const uint64_t *w, GpsStats *out) {
     std::memset(out, 0, sizeof *out);
     out->points = w[ST_POINTS];
     out->truncated_splats = w[ST_TRUNC];
@@ -792,6 +789,13 @@ int gps_renderer_read_stats(GpsRenderer *r, GpsStats *out) {
     out->skipped_ensembles = w[ST_SKIPPED];
     out->ensembles = w[ST_ENSEMBLES];
     out->probe_demand = w[ST_PROBE];
+}
+
+int gps_renderer_read_stats(GpsRenderer *r, GpsStats *out) {
+    uint64_t w[ST_WORDS];
+    HIP_TRY(r, hipMemcpyAsync(w, r->stats, sizeof w, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    stats_from_words(w, out);
     return 0;
 }
 
@@ -903,6 +907,11 @@ struct GpsBatch {
     hipEvent_t fork = nullptr, join[BATCH_LANES] = {};
     std::vector<float4 *> outs;
     std::vector<uint64_t> out_px;
+    uint64_t *item_stats = nullptr;   // ST_WORDS counters per item
+    uint32_t item_stats_cap = 0;
+    std::vector<uint32_t> item_w, item_h;   // sizes of the last render's items
+    uint32_t *composite_buf = nullptr;
+    uint64_t composite_cap = 0;
     double gpu_ms = 0.0, record_ms = 0.0;
     uint64_t nodes = 0;
 };
@@ -939,6 +948,8 @@ void gps_batch_destroy(GpsBatch *b) {
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (auto *o : b->outs)
         if (o) (void)hipFree(o);
+    if (b->item_stats) (void)hipFree(b->item_stats);
+    if (b->composite_buf) (void)hipFree(b->composite_buf);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -978,6 +989,22 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
     if (b->outs.size() < n) {
         b->outs.resize(n, nullptr);
         b->out_px.resize(n, 0);
+    }
+    if (b->item_stats_cap < n) {
+        if (b->item_stats) (void)hipFree(b->item_stats);
+        b->item_stats = nullptr;
+        b->item_stats_cap = 0;
+        if (hipMalloc(&b->item_stats, (size_t)n * ST_WORDS * sizeof(uint64_t)) != hipSuccess) {
+            g_create_error = "out of device memory for batch counters";
+            return -1;
+        }
+        b->item_stats_cap = n;
+    }
+    b->item_w.assign(n, 0);
+    b->item_h.assign(n, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        b->item_w[i] = items[i].renderer->W;
+        b->item_h[i] = items[i].renderer->H;
     }
     for (uint32_t i = 0; i < n; i++) {
         GpsRenderer *r = items[i].renderer;
@@ -1024,11 +1051,14 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
         uint64_t X = (uint64_t)r->W * r->H;
         launch_accum_clear(st, r->accum, X);
         launch_features_clear(st, r->mom, r->zmin, X);
-        launch_stats_clear(st, r->stats, ST_SKIPPED, 1);
+        launch_stats_clear(st, r->stats, 0, ST_WORDS);
         r->accumulated = 0;
         rc = enqueue_ensembles(r, st, it.camera, it.params, it.first_seed, it.count);
         if (rc == 0) {
             launch_copy_f4(st, b->outs[i], r->accum, X);
+            // the item's counters (ST_WORDS u64 = ST_WORDS / 2 float4)
+            launch_copy_f4(st, reinterpret_cast<float4 *>(b->item_stats + (size_t)i * ST_WORDS),
+                           reinterpret_cast<const float4 *>(r->stats), ST_WORDS / 2);
         } else {
             g_create_error = r->err;
         }
@@ -1076,6 +1106,35 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
 int gps_batch_read(GpsBatch *b, uint32_t item, float *out) {
     if (item >= b->outs.size() || !b->outs[item]) return -1;
     if (hipMemcpy(out, b->outs[item], b->out_px[item] * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
+}
+
+int gps_batch_composite(GpsBatch *b, uint32_t item, const GpsRenderParams *p, uint32_t out_w, uint32_t out_h,
+                        uint8_t *rgba8) {
+    if (item >= b->item_w.size() || !b->outs[item] || out_w == 0 || out_h == 0) return -1;
+    if (hipSetDevice(b->device) != hipSuccess) return -1;
+    const uint64_t px = (uint64_t)out_w * out_h;
+    if (px > b->composite_cap) {
+        if (b->composite_buf) (void)hipFree(b->composite_buf);
+        b->composite_buf = nullptr;
+        b->composite_cap = 0;
+        if (hipMalloc(&b->composite_buf, px * 4) != hipSuccess) return -1;
+        b->composite_cap = px;
+    }
+    launch_composite(b->stream, b->outs[item], b->item_w[item], b->item_h[item], p->exposure,
+                     p->srgb_output ? 1u : 0u, out_w, out_h, b->composite_buf);
+    if (hipMemcpyAsync(rgba8, b->composite_buf, px * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int gps_batch_read_stats(GpsBatch *b, uint32_t item, GpsStats *out) {
+    if (item >= b->item_stats_cap || item >= b->outs.size() || !b->outs[item]) return -1;
+    uint64_t w[ST_WORDS];
+    if (hipMemcpy(w, b->item_stats + (size_t)item * ST_WORDS, sizeof w, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    stats_from_words(w, out);
     return 0;
 }
 

@@ -208,6 +208,8 @@ _sig("gps_batch_create", C.c_int, [C.c_int, P(C.c_void_p)])
 _sig("gps_batch_destroy", None, [C.c_void_p])
 _sig("gps_batch_render", C.c_int, [C.c_void_p, P(BatchItem), u32, C.c_int])
 _sig("gps_batch_read", C.c_int, [C.c_void_p, u32, P(f32)])
+_sig("gps_batch_read_stats", C.c_int, [C.c_void_p, u32, C.c_void_p])
+_sig("gps_batch_composite", C.c_int, [C.c_void_p, u32, C.c_void_p, u32, u32, P(C.c_uint8)])
 _sig("gps_batch_timing", None, [C.c_void_p, P(f64), P(f64), P(u64)])
 _sig("gps_phase0_atomic", C.c_int, [C.c_int, u32, u32, u32, P(u32), P(u32)])
 _sig("gps_scan_exclusive_u32", C.c_int, [C.c_int, P(u32), P(u32), u64])
@@ -232,7 +234,7 @@ EXPORTED = [
     "gps_denoiser_read_features", "gps_denoiser_add_pair", "gps_denoiser_pairs", "gps_denoiser_clear_pairs",
     "gps_denoiser_train", "gps_denoiser_loss_grad", "gps_denoiser_read_output", "gps_denoiser_pair_weights",
     "gps_denoiser_set_pair_weights", "gps_ply_load", "gps_ply_free", "gps_ply_last_error", "gps_ply_info",
-    "gps_ply_read", "gps_ply_to_gpu",
+    "gps_ply_read", "gps_ply_to_gpu", "gps_batch_read_stats", "gps_batch_composite",
 ]
 
 DEBUG_KEEP_BUFFERS = 1
@@ -481,6 +483,20 @@ class Batch:
         if lib.gps_batch_read(self._h, i, _ptr(out, f32)) != 0:
             raise GpsError("batch read failed")
         return out
+
+    def composite(self, i, params, out_w=None, out_h=None):
+        w, h = self._sizes[i]
+        ow, oh = out_w or w, out_h or h
+        out = np.zeros((oh, ow, 4), np.uint8)
+        if lib.gps_batch_composite(self._h, i, C.byref(params), ow, oh, out.ctypes.data_as(P(C.c_uint8))) != 0:
+            raise GpsError("batch composite failed")
+        return out
+
+    def read_stats(self, i):
+        s = GpuStats()
+        if lib.gps_batch_read_stats(self._h, i, C.byref(s)) != 0:
+            raise GpsError("batch stats read failed")
+        return s
 
     def timing(self):
         g, rec, nodes = f64(), f64(), u64()

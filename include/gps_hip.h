@@ -233,9 +233,10 @@ int gps_denoiser_loss_grad(GpsDenoiser *d, uint32_t batch, float *loss, float *g
 int gps_denoiser_read_output(GpsDenoiser *d, uint32_t images, float *rgba); /* network output of the last step */
 
 /* ---- batches of views (no counterpart in the reference: its trajectory export renders one view per frame loop,
- * web/index.html:445-469).  Each item resets its renderer's accumulation buffer, renders `count` ensembles of
- * one view and snapshots the image into the batch's output slot `i`.  All items are recorded on one stream;
- * with use_graph they are captured into one hipGraph and launched once.  Items may share a renderer: a later
+ * web/index.html:445-469).  Each item resets its renderer's accumulation buffer and counters, renders
+ * `count` ensembles of one view and snapshots the image and the counters into the batch's slot `i`.  Items
+ * of different renderers are recorded on up to four forked streams (parallel branches of the graph); with
+ * use_graph the batch is captured into one hipGraph and launched once.  Items may share a renderer: a later
  * item overwrites the earlier one's history but not its snapshot.  The sorted particle path is not batchable. */
 typedef struct GpsBatch GpsBatch;
 typedef struct GpsBatchItem {
@@ -250,6 +251,11 @@ void gps_batch_destroy(GpsBatch *b);
 int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use_graph);
 /* w*h*4 floats of item `item`'s accumulation snapshot */
 int gps_batch_read(GpsBatch *b, uint32_t item, float *out);
+/* item `item`'s counters: every item also resets its renderer's counters before rendering */
+int gps_batch_read_stats(GpsBatch *b, uint32_t item, GpsStats *out);
+/* composite (gps.rs:967-986, as gps_renderer_composite) of item `item`'s snapshot into out_w*out_h RGBA8 */
+int gps_batch_composite(GpsBatch *b, uint32_t item, const GpsRenderParams *p, uint32_t out_w, uint32_t out_h,
+                        uint8_t *rgba8);
 /* GPU time of the last gps_batch_render (graph launch to completion), host time to record it, graph nodes */
 void gps_batch_timing(const GpsBatch *b, double *gpu_ms, double *record_ms, uint64_t *graph_nodes);
 

@@ -570,17 +570,20 @@ def test_batch_graph_equals_separate_renders():
     pb = gh.RenderParams(spp_side=1, method=1, max_points_per_splat=16384)
     items = [(ra, cam_a, pa, 11, 5), (rb, S.clustered_camera("mid", 160, 90), pb, 3, 4),
              (ra, cam_a, pg, 1, 3), (rb, S.clustered_camera("far", 160, 90), pb, 7, 70)]
-    expect = []
+    expect, expect_st = [], []
     for r, cam, p, seed, count in items:
         r.reset_accum()
+        r.reset_stats()
         r.render_ensembles(cam, p, seed, count)
         expect.append(r.read_accum())
+        expect_st.append(r.read_stats())
     b = gh.Batch()
     for use_graph in (True, False, True):
         say(f"[batch] graph={use_graph}")
         b.render(items, use_graph=use_graph)
         for i in range(len(items)):
             assert np.array_equal(b.read(i).view(np.uint32), expect[i].view(np.uint32)), i
+            assert b.read_stats(i) == expect_st[i], i
         t = b.timing()
         assert t["gpu_ms"] > 0
         if use_graph:
@@ -675,3 +678,41 @@ def test_ply_fixture_renders_bit_exact(method, path):
     gpu, m = run_pair(gg, g.sh_rest, g.sh_degree, cam, p, 96, 72, 1, 3)
     assert_exact(gpu, m)
     assert gpu["stats"].points > 0
+
+
+# ------------------------------------------------------------------ trajectory export on one batch graph
+def test_trajectory_export_matches_single_renders():
+    """Each exported frame (PNG and CSV counters) equals a separate render of its camera."""
+    import io
+    import os
+    import zipfile
+    from gps_hip import trajectory as T
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "synthetic_sh1.ply")
+    g = gh.load_ply(fixture)
+    gg = g.to_gpu()
+    c, rad = gh.robust_sphere(g.position)
+    key = {"target": [float(x) for x in c], "distance": gh.frame_sphere_distance(rad), "yaw": 0.6, "pitch": 0.25,
+           "fov_y": 0.9, "up_y": -1}
+    p = gh.RenderParams(spp_side=1, method=2, max_points_per_splat=16384, pbvr=dict(radial_correction=1))
+    r = gh.GpsRenderer(80, 60, 1)
+    r.set_gaussians_sh(gg, g.sh_rest, g.sh_degree)
+    keys = T.orbit_keys(key)
+    data, rows, t = T.render_trajectory(r, p, keys, 5, 4, "pbvr-ext-c3r", first_seed=3)
+    z = zipfile.ZipFile(io.BytesIO(data))
+    names = sorted(z.namelist())
+    assert names == [f"frame_{i:04d}.png" for i in range(5)] + ["session.json", "stats.csv"]
+    csv = z.read("stats.csv").decode().strip().split("\n")
+    assert csv[0].split(",") == ["frame"] + T.CSV_COLS and len(csv) == 6
+    for i in range(5):
+        k = T.cam_at(keys, i / 4)
+        r.reset_accum()
+        r.reset_stats()
+        r.render_ensembles(T.key_camera(k, 80, 60), p, 3, 4)
+        want = T.png_rgba8(r.composite(p))
+        assert z.read(f"frame_{i:04d}.png") == want, i
+        st = r.read_stats()
+        cols = dict(zip(["frame"] + T.CSV_COLS, csv[1 + i].split(",")))
+        assert int(cols["points"]) == st.points and int(cols["candidates"]) == st.candidates
+        assert int(cols["accumulated"]) == 4 and int(cols["visible"]) == st.visible_splats
+    assert t["graph_nodes"] > 5
+    r.close()
