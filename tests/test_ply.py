@@ -192,3 +192,47 @@ def test_robust_sphere_and_framing():
     d = np.sort(np.linalg.norm(p - c, axis=1))
     assert r == d[300]
     assert abs(gh.frame_sphere_distance(1.0, 0.9) - 1.25 / math.sin(0.45)) < 1e-12
+
+
+def test_fuzzed_inputs_return_errors_not_crashes():
+    """lib.rs:8 (no panics on bad input): thousands of corrupted fixtures (byte flips in header and body,
+    truncations, duplicated or deleted header lines, random bytes) either load or raise PlyError."""
+    raw = open(FIXTURE, "rb").read()
+    end = raw.index(b"end_header\n") + len(b"end_header\n")
+    rng = np.random.default_rng(2024)
+    lines = raw[:end].split(b"\n")
+    kinds = {}
+    for trial in range(3000):
+        k = trial % 6
+        if k == 0:      # byte flips anywhere
+            b = bytearray(raw)
+            for _ in range(int(rng.integers(1, 8))):
+                b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        elif k == 1:    # flips inside the header
+            b = bytearray(raw)
+            for _ in range(int(rng.integers(1, 4))):
+                b[int(rng.integers(0, end))] = int(rng.integers(0, 256))
+        elif k == 2:    # truncation
+            b = raw[:int(rng.integers(0, len(raw)))]
+        elif k == 3:    # delete or duplicate header lines
+            ls = list(lines)
+            i = int(rng.integers(1, len(ls) - 2))
+            ls = ls[:i] + ls[i + 1:] if rng.random() < 0.5 else ls[:i] + [ls[i]] + ls[i:]
+            b = b"\n".join(ls) + raw[end:]
+        elif k == 4:    # random bytes after a valid prefix
+            b = raw[:int(rng.integers(0, end))] + rng.integers(0, 256, int(rng.integers(0, 200)), np.uint8).tobytes()
+        else:           # ASCII bodies with junk tokens
+            n = int(rng.integers(0, 4))
+            body = "".join(" ".join(rng.choice(["1", "-0.5", "nan", "inf", "1e40", "x", "", "1e", ".", "+2"],
+                                               size=int(rng.integers(0, 30)))) + "\n" for _ in range(n))
+            b = (f"ply\nformat ascii 1.0\nelement vertex {int(rng.integers(0, 4))}\n" +
+                 "".join(f"property float {p}\n" for p in ["x", "y", "z", "f_dc_0", "f_dc_1", "f_dc_2", "opacity",
+                                                             "scale_0", "scale_1", "scale_2", "rot_0", "rot_1",
+                                                             "rot_2", "rot_3"]) + "end_header\n" + body).encode()
+        try:
+            g = gh.load_ply(bytes(b))
+            assert g.count <= gh.PLY_MAX_VERTICES and np.isfinite(g.position).all()
+            kinds["ok"] = kinds.get("ok", 0) + 1
+        except gh.PlyError as e:
+            kinds[e.kind] = kinds.get(e.kind, 0) + 1
+    assert len(kinds) >= 6, kinds   # many distinct outcomes were exercised
