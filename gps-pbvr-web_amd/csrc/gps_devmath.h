@@ -42,6 +42,31 @@ GPS_DEV float rint(float x) { return __builtin_rintf(x); }
 GPS_DEV float sqrt(float x) { return __builtin_sqrtf(x); }
 GPS_DEV float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// a / b where a, b and a / b are finite and far from the ends of the normal range (exponents within
+// [-100, 100], b != 0).  For such operands v_div_scale leaves both unscaled, v_div_fmas reduces to a plain
+// fused multiply-add and v_div_fixup returns its input, so the steps below are the correctly rounded division
+// (-fhip-fp32-correctly-rounded-divide-sqrt) without those three instructions: the same bits.
+GPS_DEV float div_normal(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    float e = fma(-b, r, 1.0f);
+    r = fma(e, r, r);
+    float q = a * r;
+    e = fma(-b, q, a);
+    q = fma(e, r, q);
+    e = fma(-b, q, a);
+    return fma(e, r, q);
+}
+
+// sqrt(x) for a positive normal x in [2^-96, 2^126]: v_sqrt_f32 is within one ulp, and the neighbour whose
+// residual x - s' s changes sign is the correctly rounded root.  These are the correctly rounded sqrt's steps
+// without its subnormal prescale and its zero/infinity select, which do not apply to such x: the same bits.
+GPS_DEV float sqrt_normal(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    float dn = __uint_as_float(__float_as_uint(s) - 1u), up = __uint_as_float(__float_as_uint(s) + 1u);
+    float t = fma(-dn, s, x) <= 0.0f ? dn : s;
+    return fma(-up, s, x) > 0.0f ? up : t;
+}
+
 // exp of x in [-103.97, 88.72] (the checked entry points below handle the rest)
 GPS_DEV float exp_core(float x) {
     float kf = rint(x * LOG2E);
@@ -84,7 +109,7 @@ GPS_DEV float log_core(float x, int e0) {
         e += 1;
     }
     float f = m - 1.0f;
-    float s = f / (2.0f + f);
+    float s = div_normal(f, 2.0f + f);   // |f| in {0} u [2^-24, 0.42], 2 + f in [1.7, 2.5]
     float z = s * s;
     float q = 0x1.c71c72p-4f;
     q = fma(q, z, 0x1.24924ap-3f);
@@ -141,12 +166,11 @@ GPS_DEV void sincos_fin(float x, float &s, float &c) {
     int q;
     float r = reduce(x, q);
     float ps = sin_poly(r), pc = cos_poly(r);
-    switch (q) {
-    case 0: s = ps; c = pc; break;
-    case 1: s = pc; c = -ps; break;
-    case 2: s = -ps; c = -pc; break;
-    default: s = -pc; c = ps; break;
-    }
+    // quadrant q: (s, c) = (ps, pc), (pc, -ps), (-ps, -pc), (-pc, ps), as selects and sign flips
+    const bool odd = (q & 1) != 0;
+    float a = odd ? pc : ps, b = odd ? ps : pc;
+    s = fbits(bits(a) ^ ((uint32_t)(q & 2) << 30));
+    c = fbits(bits(b) ^ ((uint32_t)((q + 1) & 2) << 30));
 }
 
 GPS_DEV void sincos(float x, float &s, float &c) {

@@ -71,7 +71,8 @@ struct Rng {
         }
         float u1 = fmax(next(), 5.96e-8f);
         float u2 = next();
-        float r = gpsm::sqrt(-2.0f * gpsm::log_pos(u1));   // u1 >= 5.96e-8: positive and normal
+        // u1 in [5.96e-8, 1 - 2^-24]: log is negative and normal, -2 log u1 in [1.19e-7, 33.3]
+        float r = gpsm::sqrt_normal(-2.0f * gpsm::log_pos(u1));
         float th = 2.0f * PI_F * u2;
         float sn, cs;
         gpsm::sincos_fin(th, sn, cs);
@@ -279,7 +280,8 @@ GPS_DEV float pixel_density_scale(const GpsFrame &f, float depth) {
 
 GPS_DEV float extinction_over_linear(float x) {
     if (x < 1e-4f) return 1.0f + 0.5f * x;
-    return -gpsm::log_pos(1.0f - fmin(x, 1.0f - 1e-6f)) / x;   // argument in [1e-6, 1]
+    // at both call sites x <= 1 - 1e-6 here: the log's argument is in [1e-6, 1], the quotient in [1e-4, 14]
+    return gpsm::div_normal(-gpsm::log_pos(1.0f - fmin(x, 1.0f - 1e-6f)), x);
 }
 
 GPS_DEV float pbvr_lambda(const GpsFrame &f, const Proj &pr, float o) {
