@@ -41,6 +41,8 @@ GPS_DEV float fbits(uint32_t u) { return __uint_as_float(u); }
 GPS_DEV float rint(float x) { return __builtin_rintf(x); }
 GPS_DEV float sqrt(float x) { return __builtin_sqrtf(x); }
 GPS_DEV float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+GPS_DEV float fmin(float a, float b) { return a < b ? a : b; }
+GPS_DEV float fmax(float a, float b) { return a > b ? a : b; }
 
 // a / b where a, b and a / b are finite and far from the ends of the normal range (exponents within
 // [-100, 100], b != 0).  For such operands v_div_scale leaves both unscaled, v_div_fmas reduces to a plain
@@ -133,6 +135,13 @@ GPS_DEV float log(float x) {
 // log of a positive, normal, finite x: the same bits as log() there
 GPS_DEV float log_pos(float x) { return log_core(x, 0); }
 
+// h(x) = -ln(1 - x) / x, the extinction-over-linear ratio of gps_splat.wgsl:417-460, for x <= 1 - 1e-6
+// (both call sites clamp x there): the log's argument is in [1e-6, 1], the quotient in [1e-4, 14]
+GPS_DEV float extinction_over_linear(float x) {
+    if (x < 1e-4f) return 1.0f + 0.5f * x;
+    return div_normal(-log_pos(1.0f - fmin(x, 1.0f - 1e-6f)), x);
+}
+
 GPS_DEV float sin_poly(float r) {
     float z = r * r;
     float p = 0x1.71de3ap-19f;
@@ -192,8 +201,6 @@ GPS_DEV uint32_t f2u(float f) {
     return (uint32_t)f;
 }
 
-GPS_DEV float fmin(float a, float b) { return a < b ? a : b; }
-GPS_DEV float fmax(float a, float b) { return a > b ? a : b; }
 GPS_DEV float clamp(float x, float lo, float hi) { return fmin(fmax(x, lo), hi); }
 GPS_DEV float mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 

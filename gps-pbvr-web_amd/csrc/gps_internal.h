@@ -21,6 +21,7 @@ struct GpsFrame {
 enum : uint32_t {
     M_GPS = 0, M_PROPORTIONAL = 1, M_EXTINCTION = 2, M_VIEW_CONDITIONED = 3, M_OCCUPANCY = 4,
     FLAG_RADIAL = 1, FLAG_CENTRE = 2, FLAG_JITTER = 4, FLAG_CULL = 8,
+    FLAG_FAST = 16,   // PBVR: count the particles of off-screen splats on the count-only path
 };
 
 // per-splat count source written by the view kernel (kind in the low bits of .y)
@@ -47,6 +48,7 @@ struct GpsTuningDev {
     uint32_t sort_items_per_thread;   // 8 or 16
     uint32_t batch_ensembles;         // ensembles per particle launch, 0 = from depth_window_mib
     uint32_t depth_window_mib;        // depth|id bytes one launch may span when batch_ensembles is 0
+    uint32_t count_only;              // 1: count-only path for the particles of off-screen PBVR splats
 };
 GpsTuningDev gps_tuning_current();
 
@@ -58,14 +60,18 @@ struct BatchState {
 };
 
 // kernel launchers (gps_kernels.hip)
+// perm (optional, [N + 2]): the view kernel writes the splat order with the count-only splats first (the two
+// words after it count them, zeroed before); the count scan and the packed-minimum particle kernel then work
+// in that order (results do not depend on the order)
 void launch_view(hipStream_t s, const GpsFrame &f, const float4 *splats, const float *sh, float4 *view,
-                 uint2 *cinfo, float4 *colour);
+                 uint2 *cinfo, float4 *colour, uint32_t *perm = nullptr);
 void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uint32_t first_seed, uint32_t nens,
-                       uint32_t *offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch, uint64_t *stats);
+                       uint32_t *offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch, uint64_t *stats,
+                       const uint32_t *perm = nullptr);
 void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
                       const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
                       const float *lut, uint64_t *stats, int grid, uint64_t *pkeys = nullptr,
-                      uint32_t *pidx = nullptr, int ppt = 4);
+                      uint32_t *pidx = nullptr, int ppt = 4, const uint32_t *perm = nullptr);
 void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
                     uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats,
                     float4 *mom = nullptr, float *zmin = nullptr);

@@ -20,6 +20,7 @@
 // Every f32 expression keeps the operation order of the WGSL it restates;
 // see gps_devmath.h for the elementary functions.
 #include "gps_devmath.h"
+#include "gps_fastcount.h"
 #include "gps_internal.h"
 
 namespace {
@@ -278,11 +279,6 @@ GPS_DEV float pixel_density_scale(const GpsFrame &f, float depth) {
     return fmin(ratio * ratio * f.intr[0] * f.intr[1], 1.0e12f);
 }
 
-GPS_DEV float extinction_over_linear(float x) {
-    if (x < 1e-4f) return 1.0f + 0.5f * x;
-    // at both call sites x <= 1 - 1e-6 here: the log's argument is in [1e-6, 1], the quotient in [1e-4, 14]
-    return gpsm::div_normal(-gpsm::log_pos(1.0f - fmin(x, 1.0f - 1e-6f)), x);
-}
 
 GPS_DEV float pbvr_lambda(const GpsFrame &f, const Proj &pr, float o) {
     float g = extinction_term(o);
@@ -344,31 +340,53 @@ GPS_DEV T wave_sum(T v) {
     return v;
 }
 
-// Conservative test that no particle of a PBVR splat can land in the image.  A particle's world position is
-// centre + L u with L the Cholesky factor of the 3-D covariance and u three normal draws, each bounded by
-// sqrt(-2 ln(5.96e-8)) < 5.77 (Rng::normal clamps u1), so its camera-space offset is bounded per axis by
-// 5.78 * sum_j |(R L)_kj|.  Over the part of that box in front of the near plane the pixel coordinates take
-// their extremes at the box corners; the low-pass jitter adds at most 5.78 * jitter pixels.  A 2-pixel margin
-// covers the rounding of the particle arithmetic.
-GPS_DEV bool particles_offscreen(const GpsFrame &f, float4 g0, float4 ca, float4 cb) {
-    const float B = 5.78f;
+// Camera-space box of a PBVR splat's particles.  A particle's world position is centre + L u, with L the
+// Cholesky factor of the 3-D covariance and u three normal draws of magnitude at most gpsf::U_MAX (Rng::normal
+// clamps u1, so |u| <= sqrt(-2 ln 5.96e-8) = 5.77), so its real camera-space offset along axis k is at most
+// U_MAX sum_j |(R L)_kj|.  bnd[k] bounds, for one particle, the difference between emit_pbvr's f32 value of
+// camera axis k and count_only's.  On either path, with row sums l_m = sum_j |L_mj| and p = w - pos: the sum
+// L u carries at most 3 EPS U_MAX l_m, w = c + L u adds EPS |w_m| <= EPS (|c_m| + U_MAX l_m), p adds EPS |p_m|
+// and the dot product 3 EPS sum_m |R_km| |p_m|, with |p_m| <= |c_m - pos_m| + U_MAX l_m: together
+// EPS sum_m |R_km| (|c_m| + 4 |c_m - pos_m| + 8 U_MAX l_m) from the real value of its own draws, and the two
+// paths' draws differ by at most DELTA_N each.  Both paths' values of every particle lie in
+// [cc_k - ext_k, cc_k + ext_k].
+struct CamBox {
+    float cc[3], ext[3], bnd[3];
+};
+
+GPS_DEV CamBox cam_box(const GpsFrame &f, float4 g0, float4 ca, float4 cb) {
+    using namespace gpsf;
     const float L[3][3] = {{ca.x, 0.0f, 0.0f}, {ca.y, ca.z, 0.0f}, {cb.x, cb.y, cb.z}};
-    float cc[3], e[3];
+    const float c[3] = {g0.x, g0.y, g0.z};
     const float px = g0.x - f.pos[0], py = g0.y - f.pos[1], pz = g0.z - f.pos[2];
+    const float p[3] = {px, py, pz};
+    float row[3];
+    for (int m = 0; m < 3; m++)
+        row[m] = fabsf(c[m]) + 4.0f * fabsf(p[m]) + 8.0f * U_MAX * (fabsf(L[m][0]) + fabsf(L[m][1]) + fabsf(L[m][2]));
+    CamBox b;
     for (int k = 0; k < 3; k++) {
-        cc[k] = dot3(f.rot[k], px, py, pz);
-        float s = 0.0f;
+        b.cc[k] = dot3(f.rot[k], px, py, pz);
+        float s = 0.0f, m = 0.0f;
         for (int j = 0; j < 3; j++)
             s += fabsf(f.rot[k][0] * L[0][j] + f.rot[k][1] * L[1][j] + f.rot[k][2] * L[2][j]);
-        e[k] = B * s * 1.0001f + 1e-6f * (fabsf(cc[k]) + 1.0f);
+        for (int q = 0; q < 3; q++) m += fabsf(f.rot[k][q]) * row[q];
+        b.bnd[k] = 2.02f * EPS * m + 1.001f * DELTA_N * s;
+        b.ext[k] = U_MAX * s * 1.0001f + 1e-6f * (fabsf(b.cc[k]) + 1.0f) + 2.0f * b.bnd[k];
     }
-    const float zhi = cc[2] + e[2];
+    return b;
+}
+
+// No particle of the box can land in the image: over the part of the box in front of the near plane the pixel
+// coordinates take their extremes at its corners, the low-pass jitter adds at most U_MAX * jitter pixels, and a
+// 2-pixel margin covers the rounding of the projection.
+GPS_DEV bool box_offscreen(const GpsFrame &f, const CamBox &b) {
+    const float zhi = b.cc[2] + b.ext[2];
     if (zhi <= f.near_z) return true;
-    const float zlo = fmaxf(cc[2] - e[2], fmaxf(f.near_z, 1e-6f));
-    const float margin = 2.0f + B * f.jitter * 1.0001f;
+    const float zlo = fmaxf(b.cc[2] - b.ext[2], fmaxf(f.near_z, 1e-6f));
+    const float margin = 2.0f + gpsf::U_MAX * f.jitter * 1.0001f;
     float lo[2], hi[2];
     for (int a = 0; a < 2; a++) {
-        const float x0 = cc[a] - e[a], x1 = cc[a] + e[a];
+        const float x0 = b.cc[a] - b.ext[a], x1 = b.cc[a] + b.ext[a];
         const float q0 = x0 / zlo, q1 = x0 / zhi, q2 = x1 / zlo, q3 = x1 / zhi;
         lo[a] = f.intr[a] * fminf(fminf(q0, q1), fminf(q2, q3)) + f.intr[2 + a] - margin;
         hi[a] = f.intr[a] * fmaxf(fmaxf(q0, q1), fmaxf(q2, q3)) + f.intr[2 + a] + margin;
@@ -382,7 +400,8 @@ GPS_DEV bool particles_offscreen(const GpsFrame &f, float4 g0, float4 ca, float4
 // then reads its own record from LDS.
 __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restrict__ splats,
                                               const float *__restrict__ sh, float4 *__restrict__ view,
-                                              uint2 *__restrict__ cinfo, float4 *__restrict__ colour) {
+                                              uint2 *__restrict__ cinfo, float4 *__restrict__ colour,
+                                              uint32_t *__restrict__ perm) {
     extern __shared__ float4 s_stage[];  // 4 float4 per splat, then 3 * sh_stride floats per splat
     const uint32_t i0 = blockIdx.x * blockDim.x;
     const uint32_t nb = min((uint32_t)blockDim.x, f.N - i0);
@@ -398,81 +417,127 @@ __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restri
     }
     __syncthreads();
     const uint32_t i = i0 + threadIdx.x;
-    if (i >= f.N) return;
-    const uint32_t li = threadIdx.x;
-    float4 g0 = s_stage[4u * li], g1 = s_stage[4u * li + 1u], g2 = s_stage[4u * li + 2u], g3 = s_stage[4u * li + 3u];
-    colour[i] = splat_colour(f, li, g0, g3, s_sh);
-    uint2 ci = make_uint2(0u, KIND_NONE);
-    Proj pr = project(f, g0, g1, g2);
-    float o = clamp(g0.w, 0.0f, 1.0f);
-    if (pr.ok && o > 0.0f) {
-        float l00 = gpsm::sqrt(fmax(pr.a, 1e-30f));
-        float l10 = pr.b / l00;
-        float l11 = gpsm::sqrt(fmax(pr.c - l10 * l10, 1e-30f));
-        float4 v0 = make_float4(pr.mx, pr.my, pr.depth, o);
-        float4 v1 = make_float4(l00, l10, l11, 1.0f);
-        float4 v2 = make_float4(0, 0, 0, 0), v3 = v2, v4 = v2, v5 = v2;
-        if (f.method == M_GPS) {
-            // prepare_gps, gps_splat.wgsl:535-550
-            if (!offscreen(f, pr)) {
-                float en = expected_count(f, pr, o);
-                if (en > 0.0f) ci = make_uint2(gpsm::bits(en), KIND_POISSON);
-            }
-        } else if (f.method == M_OCCUPANCY) {
-            // prepare_pbvr occupancy branch, gps_splat.wgsl:553-581
-            float density = fmax(f.density, 1e-8f);
-            float amin = fmax(1.0f - gpsm::exp(gpsm::log(1.0f - 1.0f / 255.0f) / density), 1e-8f);
-            if (o > amin) {
-                float radius = gpsm::sqrt(2.0f * gpsm::log(o / amin));
-                float ex = radius * gpsm::sqrt(pr.a), ey = radius * gpsm::sqrt(pr.c);
-                float Wf = (float)f.W, Hf = (float)f.H;
-                float lox = clamp(floorf(pr.mx - ex), 0.0f, Wf), loy = clamp(floorf(pr.my - ey), 0.0f, Hf);
-                float hix = clamp(ceilf(pr.mx + ex), 0.0f, Wf), hiy = clamp(ceilf(pr.my + ey), 0.0f, Hf);
-                float szx = fmax(hix - lox, 0.0f), szy = fmax(hiy - loy, 0.0f);
-                uint32_t count = f2u(szx) * f2u(szy) * f.spp_side * f.spp_side;
-                if (count != 0u) {
-                    float invdet = 1.0f / fmax(pr.a * pr.c - pr.b * pr.b, 1e-30f);
-                    v1 = make_float4(lox, loy, szx, szy);
-                    float peak_tau = -gpsm::log(1.0f - fmin(o, 1.0f - 1e-6f));
-                    bool sparse = density * peak_tau < 1.0f;
-                    v2 = make_float4(pr.c * invdet, -pr.b * invdet, pr.a * invdet, peak_tau);
-                    v3 = make_float4(0.0f, 0.0f, 0.0f, sparse ? 1.0f : 0.0f);
-                    if (sparse) ci = make_uint2(gpsm::bits((float)count * density * peak_tau), KIND_POISSON);
-                    else ci = make_uint2(count, KIND_FIXED);
+    const bool live = i < f.N;
+    bool marked = false;
+    if (live) {
+        const uint32_t li = threadIdx.x;
+        float4 g0 = s_stage[4u * li], g1 = s_stage[4u * li + 1u], g2 = s_stage[4u * li + 2u];
+        float4 g3 = s_stage[4u * li + 3u];
+        colour[i] = splat_colour(f, li, g0, g3, s_sh);
+        uint2 ci = make_uint2(0u, KIND_NONE);
+        Proj pr = project(f, g0, g1, g2);
+        float o = clamp(g0.w, 0.0f, 1.0f);
+        if (pr.ok && o > 0.0f) {
+            float l00 = gpsm::sqrt(fmax(pr.a, 1e-30f));
+            float l10 = pr.b / l00;
+            float l11 = gpsm::sqrt(fmax(pr.c - l10 * l10, 1e-30f));
+            float4 v0 = make_float4(pr.mx, pr.my, pr.depth, o);
+            float4 v1 = make_float4(l00, l10, l11, 1.0f);
+            float4 v2 = make_float4(0, 0, 0, 0), v3 = v2, v4 = v2, v5 = v2;
+            if (f.method == M_GPS) {
+                // prepare_gps, gps_splat.wgsl:535-550
+                if (!offscreen(f, pr)) {
+                    float en = expected_count(f, pr, o);
+                    if (en > 0.0f) ci = make_uint2(gpsm::bits(en), KIND_POISSON);
+                }
+            } else if (f.method == M_OCCUPANCY) {
+                // prepare_pbvr occupancy branch, gps_splat.wgsl:553-581
+                float density = fmax(f.density, 1e-8f);
+                float amin = fmax(1.0f - gpsm::exp(gpsm::log(1.0f - 1.0f / 255.0f) / density), 1e-8f);
+                if (o > amin) {
+                    float radius = gpsm::sqrt(2.0f * gpsm::log(o / amin));
+                    float ex = radius * gpsm::sqrt(pr.a), ey = radius * gpsm::sqrt(pr.c);
+                    float Wf = (float)f.W, Hf = (float)f.H;
+                    float lox = clamp(floorf(pr.mx - ex), 0.0f, Wf), loy = clamp(floorf(pr.my - ey), 0.0f, Hf);
+                    float hix = clamp(ceilf(pr.mx + ex), 0.0f, Wf), hiy = clamp(ceilf(pr.my + ey), 0.0f, Hf);
+                    float szx = fmax(hix - lox, 0.0f), szy = fmax(hiy - loy, 0.0f);
+                    uint32_t count = f2u(szx) * f2u(szy) * f.spp_side * f.spp_side;
+                    if (count != 0u) {
+                        float invdet = 1.0f / fmax(pr.a * pr.c - pr.b * pr.b, 1e-30f);
+                        v1 = make_float4(lox, loy, szx, szy);
+                        float peak_tau = -gpsm::log(1.0f - fmin(o, 1.0f - 1e-6f));
+                        bool sparse = density * peak_tau < 1.0f;
+                        v2 = make_float4(pr.c * invdet, -pr.b * invdet, pr.a * invdet, peak_tau);
+                        v3 = make_float4(0.0f, 0.0f, 0.0f, sparse ? 1.0f : 0.0f);
+                        if (sparse) ci = make_uint2(gpsm::bits((float)count * density * peak_tau), KIND_POISSON);
+                        else ci = make_uint2(count, KIND_FIXED);
+                    }
+                }
+            } else {
+                // prepare_pbvr, gps_splat.wgsl:582-611
+                float lambda = pbvr_lambda(f, pr, o);
+                if (lambda > 0.0f) {
+                    float det = pr.a * pr.c - pr.b * pr.b;
+                    float keep = 1.0f;
+                    uint32_t kind = KIND_POISSON;
+                    if (f.method == M_VIEW_CONDITIONED) {
+                        float spp = (float)(f.spp_side * f.spp_side);
+                        float target = f.density * spp * 2.0f * PI_F * gpsm::sqrt(det) * dilog(o);
+                        if (lambda > 0.5f) keep = clamp(target / lambda, 0.0f, 1.0f); else keep = 0.0f;
+                        if (target > lambda) kind |= KIND_UNDERCOVERED;
+                    }
+                    float idet = 1.0f / fmax(det, 1e-30f);
+                    v1.w = keep;
+                    v2 = make_float4(g0.x, g0.y, g0.z, pr.c * idet);
+                    float oo = fmin(o, 1.0f - 1e-6f);
+                    v3 = make_float4(-pr.b * idet, pr.a * idet, gpsm::extinction_over_linear(oo), 0.0f);
+                    chol_of(g1, g2, v4, v5);
+                    ci = make_uint2(gpsm::bits(lambda), kind);
+                    if (f.flags & (FLAG_CULL | FLAG_FAST)) {
+                        const CamBox bx = cam_box(f, g0, v4, v5);
+                        if (box_offscreen(f, bx)) {
+                            if (f.flags & FLAG_CULL) ci = make_uint2(0u, KIND_NONE);
+                            else {   // count_only's marker and camera-space bounds
+                                v1.x = bx.bnd[0];
+                                v1.y = bx.bnd[1];
+                                v1.z = bx.bnd[2];
+                                v3.w = 1.0f;
+                                marked = true;
+                            }
+                        }
+                    }
                 }
             }
-        } else {
-            // prepare_pbvr, gps_splat.wgsl:582-611
-            float lambda = pbvr_lambda(f, pr, o);
-            if (lambda > 0.0f) {
-                float det = pr.a * pr.c - pr.b * pr.b;
-                float keep = 1.0f;
-                uint32_t kind = KIND_POISSON;
-                if (f.method == M_VIEW_CONDITIONED) {
-                    float spp = (float)(f.spp_side * f.spp_side);
-                    float target = f.density * spp * 2.0f * PI_F * gpsm::sqrt(det) * dilog(o);
-                    if (lambda > 0.5f) keep = clamp(target / lambda, 0.0f, 1.0f); else keep = 0.0f;
-                    if (target > lambda) kind |= KIND_UNDERCOVERED;
-                }
-                float idet = 1.0f / fmax(det, 1e-30f);
-                v1.w = keep;
-                v2 = make_float4(g0.x, g0.y, g0.z, pr.c * idet);
-                float oo = fmin(o, 1.0f - 1e-6f);
-                v3 = make_float4(-pr.b * idet, pr.a * idet, extinction_over_linear(oo), 0.0f);
-                chol_of(g1, g2, v4, v5);
-                ci = make_uint2(gpsm::bits(lambda), kind);
-                if ((f.flags & FLAG_CULL) && particles_offscreen(f, g0, v4, v5)) ci = make_uint2(0u, KIND_NONE);
-            }
+            float4 *v = view + (size_t)VIEW_VEC4 * i;
+            v[0] = v0;
+            v[1] = v1;
+            v[2] = v2;
+            v[3] = v3;
+            v[4] = v4;
+            v[5] = v5;
         }
-        float4 *v = view + (size_t)VIEW_VEC4 * i;
-        v[0] = v0;
-        v[1] = v1;
-        v[2] = v2;
-        v[3] = v3;
-        v[4] = v4;
-        v[5] = v5;
+        cinfo[i] = ci;
     }
-    cinfo[i] = ci;
+    if (perm) {   // marked splats from the front, the others from the back; one pair of atomics per block
+        __shared__ uint32_t s_cnt[2][4], s_base[2];
+        const uint64_t bm = __ballot(live && marked), bu = __ballot(live && !marked);
+        const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+        if (lane == 0) {
+            s_cnt[0][wave] = (uint32_t)__popcll(bm);
+            s_cnt[1][wave] = (uint32_t)__popcll(bu);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tm = 0, tu = 0;
+            for (uint32_t w = 0; w < blockDim.x / 64u; w++) {
+                tm += s_cnt[0][w];
+                tu += s_cnt[1][w];
+            }
+            s_base[0] = tm ? atomicAdd(&perm[f.N], tm) : 0u;
+            s_base[1] = tu ? atomicAdd(&perm[f.N + 1u], tu) : 0u;
+        }
+        __syncthreads();
+        uint32_t om = s_base[0], ou = s_base[1];
+        for (uint32_t w = 0; w < wave; w++) {
+            om += s_cnt[0][w];
+            ou += s_cnt[1][w];
+        }
+        if (live) {
+            if (marked) perm[om + (uint32_t)__popcll(bm & below)] = i;
+            else perm[f.N - 1u - (ou + (uint32_t)__popcll(bu & below))] = i;
+        }
+    }
 }
 
 // ================================================================== decoupled look-back
@@ -539,7 +604,8 @@ GPS_DEV uint64_t block_exclusive_scan(uint64_t v, uint64_t &excl, uint64_t *s_wa
 __global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan(GpsFrame f, const uint2 *__restrict__ cinfo,
                                                            uint32_t first_seed, uint32_t *__restrict__ offsets,
                                                            BatchState bs, uint32_t tiles, uint32_t epoch,
-                                                           uint64_t *__restrict__ stats) {
+                                                           uint64_t *__restrict__ stats,
+                                                           const uint32_t *__restrict__ perm) {
     const uint32_t e = blockIdx.y;
     const uint32_t seed = first_seed + e;
     __shared__ uint32_t s_tile;
@@ -559,7 +625,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan(GpsFrame f, const uin
         uint32_t i = base + (uint32_t)j;
         uint32_t c = 0;
         if (i < N) {
-            uint2 ci = cinfo[i];
+            const uint32_t sp = perm ? perm[i] : i;   // position i holds splat sp
+            uint2 ci = cinfo[sp];
             uint32_t kind = ci.y & 3u;
             if (kind != KIND_NONE) {
                 st_vis += 1;
@@ -568,7 +635,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan(GpsFrame f, const uin
                 if (kind == KIND_FIXED) {
                     n = ci.x;
                 } else {
-                    Rng r(particle_seed(i, 0u, seed));
+                    Rng r(particle_seed(sp, 0u, seed));
                     n = poisson(r, gpsm::fbits(ci.x));
                 }
                 if (gps) st_pts += n; else st_cand += n;
@@ -692,7 +759,7 @@ GPS_DEV Hit emit_pbvr(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) {
         // radial_keep, gps_splat.wgsl:434-438, with h(min(o, 1-1e-6)) cached per splat in d.z
         float oo = fmin(a.w, 1.0f - 1e-6f);
         float y = oo * gpsm::exp_nonpos(-0.5f * fmax(r2, 0.0f));
-        float keep = clamp(extinction_over_linear(y) / d.z, 0.0f, 1.0f);
+        float keep = clamp(gpsm::extinction_over_linear(y) / d.z, 0.0f, 1.0f);
         if (r.next() >= keep) return h;
     }
     float depth_z = (f.flags & FLAG_CENTRE) ? a.z : camz;
@@ -740,6 +807,90 @@ GPS_DEV Hit emit_occupancy(const GpsFrame &f, Rng &r, uint32_t k, const float4 *
     return h;
 }
 
+// Count-only evaluation (gps_fastcount.h) of a particle of a splat that the view kernel marked off-screen
+// (v[3].w != 0; no particle of it can write a subpixel): 1 or 0 when emit_pbvr's generated-points decision is
+// certain, -1 when it is not and the particle must go through emit_pbvr.  It draws the uniforms in emit_pbvr's
+// order (two Box-Muller pairs for the three normals, the view-conditioned draw, the jitter normals: the second
+// pair's sine and a third pair, the radial keep draw).  Each decision compares the count-only value with a bound on its
+// difference from emit_pbvr's: camz against near_z with margin bnd_z (v[1].z, cam_box); with the radial
+// correction, the keep ratio K = clamp(h(min(o e^(-r2/2), 1 - 1e-6)) / d.z, 0, 1) against the keep draw with
+// margin dk.  dk collects the pixel difference (bx, by: camera bounds bnd_x, bnd_y, bnd_z through
+// intr * cam / camz with camz >= camz - bnd_z on both paths, plus the roundings of the projection (v_rcp_f32 on
+// this path), the jitter and dv), the r2 difference through the conic (10 EPS of its absolute terms for the
+// roundings of both evaluations), its effect on e^(-r2/2) and on h (h' <= 1/(1-x)), and the relative errors of
+// both evaluations of e^(-r2/2), h and the divisions.
+GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) {
+    using namespace gpsf;
+    const float u1 = fmax(r.next(), 5.96e-8f), u2 = r.next();
+    const float u3 = fmax(r.next(), 5.96e-8f), u4 = r.next();
+    const float4 b = v[1];
+    if (f.method == M_VIEW_CONDITIONED && r.next() >= b.w) return 0;
+    const float r1 = bm_radius(u1), r3 = bm_radius(u3);
+    const float uz = r1 * cos_turn(u2), uy = r1 * sin_turn(u2), ux = r3 * cos_turn(u4);
+    const float4 c = v[2], ca = v[4], cb = v[5];
+    const float wx = c.x + ca.x * ux;
+    const float wy = c.y + (ca.y * ux + ca.z * uy);
+    const float wz = c.z + (cb.x * ux + cb.y * uy + cb.z * uz);
+    const float px = wx - f.pos[0], py = wy - f.pos[1], pz = wz - f.pos[2];
+    const float camz = dot3(f.rot[2], px, py, pz);
+    const float dz = camz - f.near_z, mz = 1.001f * b.z;
+    if (!(dz > mz)) return dz <= -mz ? 0 : -1;
+    if (!(f.flags & FLAG_RADIAL)) return 1;
+    const float zl = camz - b.z;   // <= camz on both paths
+    if (!(zl > 0.0f)) return -1;
+    const float camx = dot3(f.rot[0], px, py, pz), camy = dot3(f.rot[1], px, py, pz);
+    const float iz = rcp(camz), izl = rcp(zl);
+    const float t1x = f.intr[0] * camx, t1y = f.intr[1] * camy;
+    const float t2x = t1x * iz, t2y = t1y * iz;
+    float pixx = t2x + f.intr[2];
+    float pixy = t2y + f.intr[3];
+    float jb = 0.0f;
+    if (f.flags & FLAG_JITTER) {
+        const float jy = r3 * sin_turn(u4);
+        const float u5 = fmax(r.next(), 5.96e-8f), u6 = r.next();
+        const float jx = bm_radius(u5) * cos_turn(u6);
+        pixx = pixx + f.jitter * jx;
+        pixy = pixy + f.jitter * jy;
+        jb = f.jitter * (DELTA_N + 6.0f * EPS * U_MAX);
+    }
+    const float4 a = v[0], d = v[3];
+    // |dv - emit_pbvr's dv|: the projection bound, then 8 EPS |t2| + E_RCP |t2| + 6 EPS |c| + 2 EPS |a| for the
+    // roundings of the pixel, the jittered sum and dv on both paths and v_rcp_f32, and the jitter's part
+    const float bx = ((fabsf(f.intr[0]) * b.x + 2.0f * EPS * fabsf(t1x) + fabsf(t2x) * b.z) * izl +
+                      (8.0f * EPS + E_RCP) * fabsf(t2x) + 6.0f * EPS * fabsf(f.intr[2]) +
+                      2.0f * EPS * fabsf(a.x) + jb) * 1.01f;
+    const float by = ((fabsf(f.intr[1]) * b.y + 2.0f * EPS * fabsf(t1y) + fabsf(t2y) * b.z) * izl +
+                      (8.0f * EPS + E_RCP) * fabsf(t2y) + 6.0f * EPS * fabsf(f.intr[3]) +
+                      2.0f * EPS * fabsf(a.y) + jb) * 1.01f;
+    const float dvx = pixx - a.x, dvy = pixy - a.y;
+    const float r2 = c.w * dvx * dvx + 2.0f * d.x * dvx * dvy + d.y * dvy * dvy;
+    const float ex = fabsf(dvx), ey = fabsf(dvy), q0 = fabsf(c.w), q1 = fabsf(d.x), q2 = fabsf(d.y);
+    const float gx = q0 * ex + q1 * ey, gy = q1 * ex + q2 * ey;
+    const float cq = q0 * bx * bx + 2.0f * q1 * bx * by + q2 * by * by;
+    const float br2 = (2.0f * (gx * bx + gy * by) + cq + 10.0f * EPS * (gx * ex + gy * ey)) * 1.01f;
+    if (!(br2 <= 1.0f)) return -1;
+    const float oo = fmin(a.w, 1.0f - 1e-6f);
+    const float t = exp_half_neg(fmax(r2, 0.0f));
+    const float y = oo * t;
+    const float tf = (t + TAU_F) * (1.0f + 2.0f * ETA_TF);   // >= e^(-r2/2) at this path's r2
+    const float te = tf * (1.0f + 0.65f * br2);               // >= e^(-r2/2) at emit_pbvr's r2 (br2 <= 1)
+    const float dt = te * (0.5f * br2 + ETA_TE) + ETA_TF * tf + (TAU_E + TAU_F);
+    const float dy = (oo * dt + 2.0f * EPS * y) * 1.01f;     // >= |y - emit_pbvr's y|
+    const float x = fmin(y, 1.0f - 1e-6f);
+    const float xmax = fmin(y + dy, 1.0f - 1e-6f), xmin = fmax(x - dy, 0.0f);
+    const float h = extinction_over_linear(x), rdz = rcp(d.z);
+    const float kf = clamp(h * rdz, 0.0f, 1.0f);
+    const float dh = dy * rcp(1.0f - xmax) * 1.001f;         // >= |h(x) - h(emit_pbvr's x)|
+    const float rx = rcp(xmin);                               // +inf for xmin = 0
+    const float eta = fmin(ETA_HE, ETA_HE_X * rx) + fmin(ETA_HF, ETA_HF_X * rx);
+    const float hu = h * (1.0f + 2.0f * ETA_HF) + dh;        // >= h at both points
+    const float dk = (dh + (eta + 2.0f * EPS + E_RCP) * hu) * rdz * 1.01f + 4.0f * EPS;
+    const float uk = r.next();
+    if (uk < kf - dk) return 1;
+    if (uk >= kf + dk) return 0;
+    return -1;
+}
+
 // OUT = 0: packed 64-bit minimum into the depth|id buffer of the particle's ensemble.
 // OUT = 1: (single ensemble) per-particle key (depth key << 32 | particle index) and subpixel into pkeys /
 //          pidx at the particle's index, for the depth-sorted resolve.
@@ -750,13 +901,22 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
                                                           uint32_t first_seed, uint64_t *__restrict__ depth,
                                                           uint64_t sub_count, const float *__restrict__ lut,
                                                           uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys,
-                                                          uint32_t *__restrict__ pidx) {
+                                                          uint32_t *__restrict__ pidx,
+                                                          const uint32_t *__restrict__ perm) {
     constexpr uint32_t CH = PART_BLOCK * PPT;
+    // PBVR packed-minimum path: particles of splats marked off-screen go through count_only; the undecided
+    // ones are queued as (splat, index within the splat | ensemble << 24) and re-evaluated with emit_pbvr by
+    // the whole block once the queue may not hold another chunk's worth, and at the end
+    constexpr bool COUNT_ONLY = MK == 1 && OUT == 0;
+    constexpr uint32_t QCAP = COUNT_ONLY ? 2u * CH : 1u;
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
-    __shared__ uint32_t s_lo, s_hi;
+    __shared__ uint32_t s_lo, s_hi, s_qn;
     __shared__ uint64_t s_pts[PART_BLOCK / 64];
+    __shared__ uint2 s_q[QCAP];
     static_assert(PPT == 1 || PPT == 2 || PPT == 4 || PPT == 8, "particles per thread");
+    const bool count_only_on = COUNT_ONLY && (f.flags & FLAG_FAST) != 0u;
     if (threadIdx.x == 0) {
+        s_qn = 0u;
         uint32_t acc = 0;
         for (uint32_t b = 0; b < nens; b++) {
             s_cstart[b] = acc;
@@ -768,6 +928,17 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
     const uint32_t nchunks = s_cstart[nens];
     const uint32_t N = f.N;
     uint64_t pts = 0;
+    auto flush = [&](uint32_t nq) {
+        for (uint32_t t = threadIdx.x; t < nq; t += PART_BLOCK) {
+            const uint2 e = s_q[t];
+            const uint32_t k = e.y & 0xFFFFFFu, b = e.y >> 24;
+            Rng r(particle_seed_mixed(e.x, k + 1u, (first_seed + b) * 2654435761u + 1u));
+            const Hit h = emit_pbvr(f, r, view + (size_t)VIEW_VEC4 * e.x);
+            pts += h.counted;
+            if (h.idx != NO_HIT)
+                write_min(depth + (size_t)b * sub_count + h.idx, ((uint64_t)h.key << 32) | (uint64_t)e.x);
+        }
+    };
     for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
         uint32_t b = 0;
         while (b + 1u < nens && s_cstart[b + 1u] <= chunk) b++;
@@ -784,7 +955,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
         __syncthreads();
         const uint32_t pfirst = p0 + threadIdx.x * PPT;
         if (pfirst < pend) {
-            uint32_t i = owner_search(offs, s_lo, s_hi, pfirst);
+            uint32_t i = owner_search(offs, s_lo, s_hi, pfirst);   // position in the splat order of the offsets
             uint32_t cur = offs[i];
             uint32_t nxt = (i + 1u < N) ? offs[i + 1u] : total;
 #pragma unroll 1
@@ -796,16 +967,23 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
                     cur = nxt;
                     nxt = (i + 1u < N) ? offs[i + 1u] : total;
                 }
+                const uint32_t sp = perm ? perm[i] : i;   // the splat
                 const uint32_t k = p - cur;
-                const float4 *v = view + (size_t)VIEW_VEC4 * i;
-                Rng r(particle_seed_mixed(i, k + 1u, seed_mix));
+                const float4 *v = view + (size_t)VIEW_VEC4 * sp;
+                Rng r(particle_seed_mixed(sp, k + 1u, seed_mix));
+                if (COUNT_ONLY && count_only_on && v[3].w != 0.0f) {
+                    const int c = count_only(f, r, v);
+                    if (c >= 0) pts += (uint32_t)c;
+                    else s_q[atomicAdd(&s_qn, 1u)] = make_uint2(sp, k | (b << 24));
+                    continue;
+                }
                 Hit h;
                 if (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
                 else if (MK == 1) h = emit_pbvr(f, r, v);
                 else h = emit_occupancy(f, r, k, v);
                 pts += h.counted;
                 if (OUT == 0) {
-                    if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)i);
+                    if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)sp);
                 } else {
                     pkeys[p] = ((uint64_t)h.key << 32) | (uint64_t)p;
                     pidx[p] = h.idx;
@@ -813,6 +991,18 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
             }
         }
         __syncthreads();
+        if (COUNT_ONLY) {
+            const uint32_t nq = s_qn;   // the same for every thread: read after the barrier
+            if (nq > QCAP - CH) {       // the next chunk could overflow the queue
+                flush(nq);
+                __syncthreads();        // every thread has read s_qn; the next chunk appends after its first barrier
+                if (threadIdx.x == 0) s_qn = 0u;
+            }
+        }
+    }
+    if (COUNT_ONLY) {
+        __syncthreads();
+        flush(s_qn);
     }
     if (MK != 0) {
         uint64_t s = wave_sum(pts);
@@ -1065,24 +1255,27 @@ __global__ void k_math(int fn, const float *__restrict__ x, const float *__restr
 static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 void launch_view(hipStream_t s, const GpsFrame &f, const float4 *splats, const float *sh, float4 *view, uint2 *cinfo,
-                 float4 *colour) {
+                 float4 *colour, uint32_t *perm) {
     if (f.N == 0) return;
     const size_t lds = 256 * (4 * sizeof(float4) + (f.sh_degree > 0 ? 3 * f.sh_stride * sizeof(float) : 0));
-    hipLaunchKernelGGL(k_view, dim3(blocks_for(f.N, 256)), dim3(256), lds, s, f, splats, sh, view, cinfo, colour);
+    hipLaunchKernelGGL(k_view, dim3(blocks_for(f.N, 256)), dim3(256), lds, s, f, splats, sh, view, cinfo, colour,
+                       perm);
 }
 
 void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uint32_t first_seed, uint32_t nens,
-                       uint32_t *offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch, uint64_t *stats) {
+                       uint32_t *offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch, uint64_t *stats,
+                       const uint32_t *perm) {
     hipLaunchKernelGGL(k_count_scan, dim3(tiles, nens), dim3(SCAN_BLOCK), 0, s, f, cinfo, first_seed, offsets, bs,
-                       tiles, epoch, stats);
+                       tiles, epoch, stats, perm);
 }
 
 void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
                       const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
-                      const float *lut, uint64_t *stats, int grid, uint64_t *pkeys, uint32_t *pidx, int ppt) {
+                      const float *lut, uint64_t *stats, int grid, uint64_t *pkeys, uint32_t *pidx, int ppt,
+                      const uint32_t *perm) {
 #define GPS_LAUNCH(MK, PPT, OUT)                                                                                   \
     hipLaunchKernelGGL((k_particles<MK, PPT, OUT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals, \
-                       nens, first_seed, depth, sub_count, lut, stats, pkeys, pidx)
+                       nens, first_seed, depth, sub_count, lut, stats, pkeys, pidx, OUT ? nullptr : perm)
 #define GPS_LAUNCH_PPT(MK, OUT)          \
     do {                                 \
         if (ppt == 1) GPS_LAUNCH(MK, 1, OUT); \

@@ -40,7 +40,7 @@ void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *fir
 namespace {
 
 thread_local std::string g_create_error;
-GpsTuningDev g_tuning = {8u, 4u, 64u, 0u, 64u};
+GpsTuningDev g_tuning = {8u, 4u, 64u, 0u, 64u, 1u};
 
 constexpr uint64_t MAX_SPLATS = 1ull << 30;
 constexpr uint32_t LUT_O = 65, LUT_V = 129;
@@ -130,6 +130,7 @@ struct GpsRenderer {
     float *sh = nullptr;
     float *lut = nullptr;
     float4 *view = nullptr;
+    uint32_t *perm = nullptr;   // [n + 2]: count-only splat order (marked splats first), then its two counters
     uint2 *cinfo = nullptr;
     float4 *colour = nullptr;
     // batch
@@ -303,6 +304,9 @@ void pack_frame(const GpsRenderer *r, const GpsCamera *cam, const GpsRenderParam
     if (p->pbvr.centre_depth) flags |= FLAG_CENTRE;
     if (p->pbvr.low_pass_jitter) flags |= FLAG_JITTER;
     if (r->cull) flags |= FLAG_CULL;
+    const bool pbvr = p->method == M_PROPORTIONAL || p->method == M_EXTINCTION || p->method == M_VIEW_CONDITIONED;
+    // the count-only queue packs a particle's index within its splat into 24 bits
+    if (g_tuning.count_only && pbvr && f.cap <= (1u << 24)) flags |= FLAG_FAST;
     f.flags = flags;
     f.density = p->density_scale;
     f.near_z = p->near;
@@ -513,6 +517,7 @@ void gps_renderer_destroy(GpsRenderer *r) {
     dfree(r->sh);
     dfree(r->lut);
     dfree(r->view);
+    dfree(r->perm);
     dfree(r->cinfo);
     dfree(r->colour);
     dfree(r->offsets);
@@ -545,6 +550,7 @@ int gps_renderer_set_gaussians(GpsRenderer *r, const GpsGaussian *g, uint64_t n,
     float4 *splats = nullptr, *view = nullptr, *colour = nullptr;
     float *sh = nullptr;
     uint2 *cinfo = nullptr;
+    uint32_t *perm = nullptr;
     uint64_t nn = n ? n : 1;
     auto cleanup = [&]() {
         dfree(splats);
@@ -552,9 +558,11 @@ int gps_renderer_set_gaussians(GpsRenderer *r, const GpsGaussian *g, uint64_t n,
         dfree(colour);
         dfree(sh);
         dfree(cinfo);
+        dfree(perm);
     };
     if (hipMalloc(&splats, nn * 64) != hipSuccess || hipMalloc(&view, nn * VIEW_VEC4 * sizeof(float4)) != hipSuccess ||
         hipMalloc(&colour, nn * sizeof(float4)) != hipSuccess || hipMalloc(&cinfo, nn * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&perm, (nn + 2) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&sh, (stride ? nn * 3 * stride : 16) * sizeof(float)) != hipSuccess) {
         cleanup();
         return fail(r, "out of device memory for the scene");
@@ -572,8 +580,10 @@ int gps_renderer_set_gaussians(GpsRenderer *r, const GpsGaussian *g, uint64_t n,
     dfree(r->colour);
     dfree(r->sh);
     dfree(r->cinfo);
+    dfree(r->perm);
     r->splats = splats;
     r->view = view;
+    r->perm = perm;
     r->colour = colour;
     r->sh = sh;
     r->cinfo = cinfo;
@@ -637,15 +647,20 @@ static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *ca
     pack_frame(r, cam, p, f);
     const uint64_t S = sub_count(r);
     const bool use_particles = (p->path != GPS_PATH_PRIMITIVE || p->method != GPS_METHOD_GPS) && r->n > 0;
+    const bool keep = (r->debug & GPS_DEBUG_KEEP_BUFFERS) != 0;
+    const bool sorted = use_particles && p->path == GPS_PATH_PARTICLE_SORTED;
+    // count-only splat order: the packed-minimum path only (the sorted path keys particles by their index in
+    // splat order), and not when the offsets are kept for inspection
+    uint32_t *perm = ((f.flags & FLAG_FAST) && use_particles && !sorted && !keep) ? r->perm : nullptr;
     if (r->n > 0) {
         Prof pr(r, "view");
-        launch_view(st, f, r->splats, r->sh, r->view, r->cinfo, r->colour);
+        if (perm) launch_zero_u32(st, perm + r->n, 2);
+        launch_view(st, f, r->splats, r->sh, r->view, r->cinfo, r->colour, perm);
     }
-    if (use_particles && p->path == GPS_PATH_PARTICLE_SORTED) {
+    if (sorted) {
         if (st != r->stream) return fail(r, "the sorted particle path cannot be recorded into a batch graph");
         return render_sorted_particles(r, f, first_seed, count);
     }
-    const bool keep = (r->debug & GPS_DEBUG_KEEP_BUFFERS) != 0;
     const int grid = grid_for_particles(r);
     uint32_t done = 0;
     const uint32_t cap = ensembles_per_launch(r);
@@ -662,14 +677,14 @@ static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *ca
         if (use_particles) {
             {
                 Prof pr(r, "count_scan");
-                launch_count_scan(st, f, r->cinfo, seed0, B, r->offsets, r->bs, r->tiles, r->epoch, r->stats);
+                launch_count_scan(st, f, r->cinfo, seed0, B, r->offsets, r->bs, r->tiles, r->epoch, r->stats, perm);
             }
             r->epoch = (r->epoch + 1u) & ((1u << 26) - 1u);
             if (r->epoch == 0) r->epoch = 1;
             {
                 Prof pr(r, "particles");
                 launch_particles(st, f, r->view, r->offsets, r->bs, B, seed0, r->depth, S, r->lut, r->stats,
-                                 grid, nullptr, nullptr, (int)g_tuning.particles_per_thread);
+                                 grid, nullptr, nullptr, (int)g_tuning.particles_per_thread, perm);
             }
         } else if (r->n > 0) {
             Prof pr(r, "splat_pass");
@@ -1151,6 +1166,7 @@ void gps_tuning_get(GpsTuning *out) {
     out->sort_items_per_thread = g_tuning.sort_items_per_thread;
     out->batch_ensembles = g_tuning.batch_ensembles;
     out->depth_window_mib = g_tuning.depth_window_mib;
+    out->count_only = g_tuning.count_only;
 }
 
 int gps_tuning_set(const GpsTuning *t) {
@@ -1161,11 +1177,13 @@ int gps_tuning_set(const GpsTuning *t) {
     if (t->particle_waves_per_cu < 1 || t->particle_waves_per_cu > 64) return -1;
     if (t->batch_ensembles > MAX_BATCH) return -1;
     if (t->depth_window_mib < 1 || t->depth_window_mib > (1u << 20)) return -1;
+    if (t->count_only > 1) return -1;
     g_tuning.particle_waves_per_cu = t->particle_waves_per_cu;
     g_tuning.particles_per_thread = ppt;
     g_tuning.sort_items_per_thread = t->sort_items_per_thread;
     g_tuning.batch_ensembles = t->batch_ensembles;
     g_tuning.depth_window_mib = t->depth_window_mib;
+    g_tuning.count_only = t->count_only;
     return 0;
 }
 

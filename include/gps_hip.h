@@ -301,6 +301,10 @@ typedef struct GpsTuning {
     uint32_t depth_window_mib;       /* with batch_ensembles 0: as many ensembles per launch as keep their
                                         depth|id buffers (8 bytes per subpixel each) within this many MiB,
                                         at least 1; default 64 */
+    uint32_t count_only;             /* 1 (default): PBVR particles of splats whose particles provably all land
+                                        off-screen are evaluated for the generated-points count on the certified
+                                        count-only path (undecided particles on the full path); 0: every particle
+                                        on the full path.  Images and counters are the same either way */
 } GpsTuning;
 void gps_tuning_get(GpsTuning *out);
 int gps_tuning_set(const GpsTuning *t);

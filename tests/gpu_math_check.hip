@@ -2,6 +2,8 @@
 // rounded operations they replace, over every argument the particle kernels pass them.  Built by
 // __graft_entry__.build() into tests/libgps_mathcheck.so; tests/test_gpu_math.py calls it.
 #include "../gps-pbvr-web_amd/csrc/gps_devmath.h"
+#include "../gps-pbvr-web_amd/csrc/gps_fastcount.h"
+#include <string.h>
 
 namespace {
 
@@ -79,6 +81,65 @@ __global__ __launch_bounds__(BLOCK) void k_sincos(uint32_t hi, unsigned long lon
     }
 }
 
+// ---------------------------------------------------------------- error maxima of the count-only forms
+constexpr float PI_F = 3.14159265358979f;   // the particle kernels' pi
+
+__device__ void max_into(double v, unsigned long long *out) {
+    if (!(v == v)) v = 1e300;
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+// which: 0 Box-Muller radius (|fast - exact| over the 2^24 clamped uniforms; out[1], out[2] = max radius
+// exact / fast), 1 sin/cos of 2 pi u (|fast - exact| over the 2^24 uniforms), 2 / 3 e^(-r2/2) exact / fast
+// against f64 over every f32 r2 >= 0 (out[0] relative error where e^(-r2/2) >= 2^-126, out[1] absolute error
+// below), 4 / 5 extinction ratio exact / fast against f64 over every f32 x in [0, 1 - 1e-6] (out[0] relative
+// error, out[1] relative error times x),
+// 6 v_rcp_f32 against f64 over every f32 z whose reciprocal is normal (relative)
+__global__ __launch_bounds__(BLOCK) void k_measure(int which, unsigned long long *out) {
+    uint64_t lo = 0, n = 0;
+    switch (which) {
+    case 0: case 1: n = 1u << 24; break;
+    case 2: case 3: n = 0x7f800001u; break;
+    case 4: case 5: n = (uint64_t)__float_as_uint(1.0f - 1e-6f) + 1u; break;
+    default: lo = 0x00800000u; n = 0x7e800000u - lo; break;
+    }
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; i < n; i += (uint64_t)GRID * BLOCK) {
+        const uint32_t b = (uint32_t)(lo + i);
+        if (which == 0) {
+            float u = gpsm::fmax((float)b * (1.0f / 16777216.0f), 5.96e-8f);
+            float re = gpsm::sqrt_normal(-2.0f * gpsm::log_pos(u)), rf = gpsf::bm_radius(u);
+            m0 = fmax(m0, fabs((double)rf - (double)re));
+            m1 = fmax(m1, (double)re);
+            m2 = fmax(m2, (double)rf);
+        } else if (which == 1) {
+            float u = (float)b * (1.0f / 16777216.0f);
+            float th = 2.0f * PI_F * u, se, ce;
+            gpsm::sincos_fin(th, se, ce);
+            m0 = fmax(m0, fmax(fabs((double)gpsf::sin_turn(u) - se), fabs((double)gpsf::cos_turn(u) - ce)));
+        } else if (which == 2 || which == 3) {
+            float r2 = __uint_as_float(b);
+            float t = which == 2 ? gpsm::exp_nonpos(-0.5f * gpsm::fmax(r2, 0.0f)) : gpsf::exp_half_neg(r2);
+            double e = exp(-0.5 * (double)r2);
+            if (e >= 0x1p-126) m0 = fmax(m0, fabs((double)t - e) / e);
+            else m1 = fmax(m1, fabs((double)t - e));
+        } else if (which == 4 || which == 5) {
+            float x = __uint_as_float(b);
+            float h = which == 4 ? gpsm::extinction_over_linear(x) : gpsf::extinction_over_linear(x);
+            double xd = (double)x, e = x == 0.0f ? 1.0 : -log1p(-xd) / xd, rel = fabs((double)h - e) / e;
+            m0 = fmax(m0, rel);
+            m1 = fmax(m1, rel * xd);
+        } else {
+            float z = __uint_as_float(b);
+            m0 = fmax(m0, fabs((double)gpsf::rcp(z) * (double)z - 1.0));
+        }
+    }
+    max_into(m0, out);
+    max_into(m1, out + 1);
+    max_into(m2, out + 2);
+}
+
 template <typename F>
 int run(unsigned long long *host, F launch) {
     unsigned long long *d = nullptr;
@@ -96,7 +157,33 @@ int run(unsigned long long *host, F launch) {
 
 }  // namespace
 
+// The count-only path's error constants, in the order of k_measure's quantities: E_R, E_T, ETA_TE, TAU_E,
+// ETA_TF, TAU_F, ETA_HE, ETA_HE_X, ETA_HF, ETA_HF_X, E_RCP, U_MAX, DELTA_N.
+extern "C" int mathcheck_constants(double *out, int n) {
+    const float c[] = {gpsf::E_R, gpsf::E_T, gpsf::ETA_TE, gpsf::TAU_E, gpsf::ETA_TF, gpsf::TAU_F, gpsf::ETA_HE,
+                       gpsf::ETA_HE_X, gpsf::ETA_HF, gpsf::ETA_HF_X, gpsf::E_RCP, gpsf::U_MAX, gpsf::DELTA_N};
+    const int m = (int)(sizeof(c) / sizeof(c[0]));
+    for (int k = 0; k < n && k < m; k++) out[k] = (double)c[k];
+    return m;
+}
+
 // Each returns 0 and the number of mismatching arguments in *bad, or -1 on a HIP error.
+extern "C" int mathcheck_measure(int which, double *out3) {
+    if (which < 0 || which > 6) return -1;
+    unsigned long long *d = nullptr;
+    if (hipMalloc(&d, 3 * sizeof(*d)) != hipSuccess) return -1;
+    int rc = hipMemset(d, 0, 3 * sizeof(*d)) == hipSuccess ? 0 : -1;
+    if (!rc) {
+        hipLaunchKernelGGL(k_measure, dim3(GRID), dim3(BLOCK), 0, 0, which, d);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -1;
+    }
+    unsigned long long h[3] = {0, 0, 0};
+    if (!rc && hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    (void)hipFree(d);
+    for (int k = 0; k < 3; k++) memcpy(&out3[k], &h[k], sizeof(double));
+    return rc;
+}
+
 extern "C" int mathcheck_sqrt(uint32_t lo_bits, uint32_t hi_bits, unsigned long long *bad) {
     if (hi_bits < lo_bits) return -1;
     return run(bad, [=](unsigned long long *d) {

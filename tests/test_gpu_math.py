@@ -52,3 +52,32 @@ def test_div_normal_extinction_quotient(lib):
 def test_sincos_selects(lib):
     # the particle kernels take sin/cos of 2 pi u, u in [0, 1): every float in [0, 6.2832]
     assert _bad(lib.mathcheck_sincos, ctypes.c_uint32(fbits(6.2832))) == 0
+
+
+NAMES = ["E_R", "E_T", "ETA_TE", "TAU_E", "ETA_TF", "TAU_F", "ETA_HE", "ETA_HE_X", "ETA_HF", "ETA_HF_X", "E_RCP",
+         "U_MAX", "DELTA_N"]
+
+
+def test_count_only_error_constants_hold(lib):
+    """Every error constant of the count-only path (gps_fastcount.h) is at least the maximum error measured
+    exhaustively over its argument set, and DELTA_N / U_MAX follow from them."""
+    c = (ctypes.c_double * len(NAMES))()
+    assert lib.mathcheck_constants(c, len(NAMES)) == len(NAMES)
+    k = dict(zip(NAMES, c))
+    out = (ctypes.c_double * 3)()
+    meas = {}
+    for which in range(7):
+        assert lib.mathcheck_measure(which, out) == 0
+        meas[which] = tuple(out)
+        print(f"measure {which}: {tuple(out)}", flush=True)
+    eps = 2.0 ** -24
+    r_max = meas[0][1]
+    assert meas[0][0] <= k["E_R"] and meas[0][2] <= r_max * (1 + 1e-6)
+    assert meas[1][0] <= k["E_T"]
+    assert meas[2][0] <= k["ETA_TE"] and meas[2][1] <= k["TAU_E"]
+    assert meas[3][0] <= k["ETA_TF"] and meas[3][1] <= k["TAU_F"]
+    assert meas[4][0] <= k["ETA_HE"] and meas[4][1] <= k["ETA_HE_X"]
+    assert meas[5][0] <= k["ETA_HF"] and meas[5][1] <= k["ETA_HF_X"]
+    assert meas[6][0] <= k["E_RCP"]
+    assert (r_max + k["E_R"]) * (1 + k["E_T"]) * (1 + eps) <= k["U_MAX"]
+    assert r_max * k["E_T"] + k["E_R"] * (1 + k["E_T"]) + 2 * eps * r_max * (1 + k["E_T"]) <= k["DELTA_N"]
