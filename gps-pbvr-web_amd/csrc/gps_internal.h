@@ -60,9 +60,10 @@ struct BatchState {
 };
 
 // kernel launchers (gps_kernels.hip)
-// perm (optional, [N + 2]): the view kernel writes the splat order with the count-only splats first (the two
-// words after it count them, zeroed before); the count scan and the packed-minimum particle kernel then work
-// in that order (results do not depend on the order)
+// perm (optional, [N + 3]): the view kernel writes the splat order with the count-only splats first into
+// perm[0, N) and counts them in the 8-byte word at perm + perm_counter(N) (zeroed before); the count scan and
+// the packed-minimum particle kernel then work in that order (results do not depend on the order)
+__host__ __device__ inline uint32_t perm_counter(uint32_t n) { return (n + 1u) & ~1u; }
 void launch_view(hipStream_t s, const GpsFrame &f, const float4 *splats, const float *sh, float4 *view,
                  uint2 *cinfo, float4 *colour, uint32_t *perm = nullptr);
 void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uint32_t first_seed, uint32_t nens,

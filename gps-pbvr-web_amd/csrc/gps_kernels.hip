@@ -508,7 +508,7 @@ __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restri
         }
         cinfo[i] = ci;
     }
-    if (perm) {   // marked splats from the front, the others from the back; one pair of atomics per block
+    if (perm) {   // marked splats from the front, the others from the back; one 64-bit atomic per block
         __shared__ uint32_t s_cnt[2][4], s_base[2];
         const uint64_t bm = __ballot(live && marked), bu = __ballot(live && !marked);
         const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -518,14 +518,16 @@ __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restri
             s_cnt[1][wave] = (uint32_t)__popcll(bu);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0) {   // both counters in one word: marked in the low half, the others in the high
             uint32_t tm = 0, tu = 0;
             for (uint32_t w = 0; w < blockDim.x / 64u; w++) {
                 tm += s_cnt[0][w];
                 tu += s_cnt[1][w];
             }
-            s_base[0] = tm ? atomicAdd(&perm[f.N], tm) : 0u;
-            s_base[1] = tu ? atomicAdd(&perm[f.N + 1u], tu) : 0u;
+            unsigned long long *ctr = reinterpret_cast<unsigned long long *>(perm + perm_counter(f.N));
+            const unsigned long long old = atomicAdd(ctr, ((unsigned long long)tu << 32) | tm);
+            s_base[0] = (uint32_t)old;
+            s_base[1] = (uint32_t)(old >> 32);
         }
         __syncthreads();
         uint32_t om = s_base[0], ou = s_base[1];

@@ -130,7 +130,7 @@ struct GpsRenderer {
     float *sh = nullptr;
     float *lut = nullptr;
     float4 *view = nullptr;
-    uint32_t *perm = nullptr;   // [n + 2]: count-only splat order (marked splats first), then its two counters
+    uint32_t *perm = nullptr;   // [n + 3]: count-only splat order (marked splats first), then its counter word
     uint2 *cinfo = nullptr;
     float4 *colour = nullptr;
     // batch
@@ -562,7 +562,7 @@ int gps_renderer_set_gaussians(GpsRenderer *r, const GpsGaussian *g, uint64_t n,
     };
     if (hipMalloc(&splats, nn * 64) != hipSuccess || hipMalloc(&view, nn * VIEW_VEC4 * sizeof(float4)) != hipSuccess ||
         hipMalloc(&colour, nn * sizeof(float4)) != hipSuccess || hipMalloc(&cinfo, nn * sizeof(uint2)) != hipSuccess ||
-        hipMalloc(&perm, (nn + 2) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&perm, (nn + 3) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&sh, (stride ? nn * 3 * stride : 16) * sizeof(float)) != hipSuccess) {
         cleanup();
         return fail(r, "out of device memory for the scene");
@@ -654,7 +654,7 @@ static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *ca
     uint32_t *perm = ((f.flags & FLAG_FAST) && use_particles && !sorted && !keep) ? r->perm : nullptr;
     if (r->n > 0) {
         Prof pr(r, "view");
-        if (perm) launch_zero_u32(st, perm + r->n, 2);
+        if (perm) launch_zero_u32(st, perm + perm_counter((uint32_t)r->n), 2);
         launch_view(st, f, r->splats, r->sh, r->view, r->cinfo, r->colour, perm);
     }
     if (sorted) {
