@@ -2,8 +2,9 @@
 //
 // Sort (8-bit digits, stable):
 //   k_os_hist   one read of the keys builds the histograms of every digit position the sort uses at once
-//   k_os_pass   per digit pass, one kernel: ticketed 4096-key tiles rank their keys with wave64 ballots,
-//               publish per-digit counts and resolve their per-digit global offsets by decoupled look-back
+//   k_os_pass   per digit pass, one kernel: ticketed tiles (16,384 keys by default) rank their keys with
+//               wave64 ballots, publish per-digit counts and resolve their per-digit global offsets by decoupled
+//               look-back
 //               (one digit per thread), stage the tile in LDS in digit order, then write it out in
 //               contiguous runs.  Passes whose digit is the same for every key are skipped.
 //   Traffic per executed pass: 12 B read + 12 B written per key.
@@ -31,9 +32,8 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict
     for (int i = threadIdx.x; i < 8 * 256; i += OS_BLOCK) (&h[0][0])[i] = 0u;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * OS_BLOCK;
-    for (uint64_t i = (uint64_t)blockIdx.x * OS_BLOCK + threadIdx.x; i < n; i += stride) {
-        uint64_t k = keys[i];
-        const bool full = i - (threadIdx.x & 63) + 64 <= n;   // every lane of this wave holds a key
+    const uint32_t lane = threadIdx.x & 63u;
+    auto add = [&](uint64_t k, bool full) {   // full: every lane of this wave holds a key
 #pragma unroll
         for (int d = 0; d < 8; d++) {
             if (d < dlo) continue;
@@ -41,12 +41,22 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict
             // a wave whose 64 keys share the digit adds 64 once instead of 64 serialised LDS atomics
             uint32_t first = __builtin_amdgcn_readfirstlane(dg);
             if (full && __ballot(dg != first) == 0ull) {
-                if ((threadIdx.x & 63) == 0) atomicAdd(&h[d][dg], 64u);
+                if (lane == 0) atomicAdd(&h[d][dg], 64u);
             } else {
                 atomicAdd(&h[d][dg], 1u);
             }
         }
+    };
+    uint64_t i = (uint64_t)blockIdx.x * OS_BLOCK + threadIdx.x;
+    // four loads in flight per thread while all four of the wave's keys are in range
+    for (; i - lane + 3 * stride + 64 <= n; i += 4 * stride) {
+        uint64_t kk[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) kk[u] = keys[i + (uint64_t)u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; u++) add(kk[u], true);
     }
+    for (; i < n; i += stride) add(keys[i], i - lane + 64 <= n);
     __syncthreads();
     for (int i = threadIdx.x; i < 8 * 256; i += OS_BLOCK) {
         uint32_t c = (&h[0][0])[i];
@@ -98,12 +108,18 @@ __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ ki
     const uint64_t lt = (1ull << lane) - 1ull;
     uint64_t key[OS_ITEMS];
     uint32_t val[OS_ITEMS], rank[OS_ITEMS];  // rank, then the key's position in the staged tile (~0: none)
+    // every load of the tile is issued before the ranking: the ranking's wave barriers would otherwise hold
+    // each key's load back until the previous key is ranked
+#pragma unroll
+    for (int j = 0; j < OS_ITEMS; j++) {
+        const uint64_t i = tile * OS_TILE + (uint64_t)w * (64 * OS_ITEMS) + (uint64_t)j * 64 + lane;
+        key[j] = i < n ? kin[i] : 0ull;
+        val[j] = i < n ? vin[i] : 0u;
+    }
 #pragma unroll
     for (int j = 0; j < OS_ITEMS; j++) {
         uint64_t i = tile * OS_TILE + (uint64_t)w * (64 * OS_ITEMS) + (uint64_t)j * 64 + lane;
         bool valid = i < n;
-        key[j] = valid ? kin[i] : 0ull;
-        val[j] = valid ? vin[i] : 0u;
         uint32_t d = valid ? (uint32_t)((key[j] >> shift) & 255ull) : 256u;
         uint64_t peers;
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
