@@ -911,6 +911,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
     // the whole block once the queue may not hold another chunk's worth, and at the end
     constexpr bool COUNT_ONLY = MK == 1 && OUT == 0;
     constexpr uint32_t QCAP = COUNT_ONLY ? 2u * CH : 1u;
+    constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
     __shared__ uint32_t s_lo, s_hi, s_qn;
     __shared__ uint64_t s_pts[PART_BLOCK / 64];
@@ -960,28 +961,39 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
             uint32_t i = owner_search(offs, s_lo, s_hi, pfirst);   // position in the splat order of the offsets
             uint32_t cur = offs[i];
             uint32_t nxt = (i + 1u < N) ? offs[i + 1u] : total;
+            // the splat's view record stays in registers while its particles run (loaded when the splat changes,
+            // so its latency overlaps the seed hashing instead of stalling each particle)
+            uint32_t sp = perm ? perm[i] : i;
+            float4 v[NV];
+#pragma unroll
+            for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
 #pragma unroll 1
             for (uint32_t j = 0; j < PPT; j++) {
                 const uint32_t p = pfirst + j;
                 if (p >= pend) break;
-                while (p >= nxt) {
-                    i++;
-                    cur = nxt;
-                    nxt = (i + 1u < N) ? offs[i + 1u] : total;
+                if (p >= nxt) {
+                    do {
+                        i++;
+                        cur = nxt;
+                        nxt = (i + 1u < N) ? offs[i + 1u] : total;
+                    } while (p >= nxt);
+                    sp = perm ? perm[i] : i;
+#pragma unroll
+                    for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
                 }
-                const uint32_t sp = perm ? perm[i] : i;   // the splat
                 const uint32_t k = p - cur;
-                const float4 *v = view + (size_t)VIEW_VEC4 * sp;
                 Rng r(particle_seed_mixed(sp, k + 1u, seed_mix));
-                if (COUNT_ONLY && count_only_on && v[3].w != 0.0f) {
-                    const int c = count_only(f, r, v);
-                    if (c >= 0) pts += (uint32_t)c;
-                    else s_q[atomicAdd(&s_qn, 1u)] = make_uint2(sp, k | (b << 24));
-                    continue;
+                if constexpr (COUNT_ONLY) {
+                    if (count_only_on && v[3].w != 0.0f) {
+                        const int c = count_only(f, r, v);
+                        if (c >= 0) pts += (uint32_t)c;
+                        else s_q[atomicAdd(&s_qn, 1u)] = make_uint2(sp, k | (b << 24));
+                        continue;
+                    }
                 }
                 Hit h;
-                if (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
-                else if (MK == 1) h = emit_pbvr(f, r, v);
+                if constexpr (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
+                else if constexpr (MK == 1) h = emit_pbvr(f, r, v);
                 else h = emit_occupancy(f, r, k, v);
                 pts += h.counted;
                 if (OUT == 0) {
