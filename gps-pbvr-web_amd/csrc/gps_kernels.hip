@@ -822,6 +822,9 @@ GPS_DEV Hit emit_occupancy(const GpsFrame &f, Rng &r, uint32_t k, const float4 *
 // roundings of both evaluations), its effect on e^(-r2/2) and on h (h' <= 1/(1-x)), and the relative errors of
 // both evaluations of e^(-r2/2), h and the divisions.
 GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) {
+    // Fused multiply-adds only remove roundings from this path's arithmetic, which every bound below counts
+    // as separate operations; the measured gpsf functions are defined outside this scope and stay unfused.
+#pragma clang fp contract(fast)
     using namespace gpsf;
     const float u1 = fmax(r.next(), 5.96e-8f), u2 = r.next();
     const float u3 = fmax(r.next(), 5.96e-8f), u4 = r.next();
