@@ -464,6 +464,15 @@ def main():
                 leg["denoised"] = D.denoised_time_to_psnr(r, den, cam, p, ms, ref, fallback_ratio=4.0)
                 if leg["plain"]["gpu_ms"] and leg["denoised"]["gpu_ms"]:
                     leg["speedup"] = round(leg["plain"]["gpu_ms"] / leg["denoised"]["gpu_ms"], 2)
+            # the same ensemble counts priced at the culled renders' GPU time (bit-identical images; only the
+            # generated-points counters differ), for callers that need the image alone
+            cms = per_view[v].get("ms_per_ensemble_cull")
+            if cms is not None:
+                if leg["plain"]["ensembles"]:
+                    leg["plain"]["gpu_ms_cull"] = round(leg["plain"]["ensembles"] * cms, 3)
+                dn = leg["denoised"]
+                if dn is not None and dn["ensembles"]:
+                    dn["gpu_ms_cull"] = round(dn["ensembles"] * cms + dn["denoise_ms"], 3)
             ttp[v] = leg
             log(f"[psnr {v}] {leg}")
         r.set_features(False)

@@ -135,7 +135,9 @@ GPS_DEV float radius_lut(const float *__restrict__ lut, float o, float u) {
     return mix(r0, r1, a);
 }
 
-GPS_DEV float corrected_radius(float o, float u) {
+// out of line: the exact-radius option's series and Newton steps would otherwise be unrolled into every GPS
+// particle loop (about 25k instructions), while the default table path is a handful
+__device__ __noinline__ float corrected_radius(float o, float u) {
     if (o <= 0.0f) return 0.0f;
     float d = dilog(o);
     float tgt = (1.0f - u) * d;
