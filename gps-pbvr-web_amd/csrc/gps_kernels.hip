@@ -875,12 +875,14 @@ GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) 
     const float gx = q0 * ex + q1 * ey, gy = q1 * ex + q2 * ey;
     const float cq = q0 * bx * bx + 2.0f * q1 * bx * by + q2 * by * by;
     const float br2 = (2.0f * (gx * bx + gy * by) + cq + 10.0f * EPS * (gx * ex + gy * ey)) * 1.01f;
-    if (!(br2 <= 1.0f)) return -1;
+    if (!(br2 <= 200.0f)) return -1;   // (and NaN)
     const float oo = fmin(a.w, 1.0f - 1e-6f);
     const float t = exp_half_neg(fmax(r2, 0.0f));
     const float y = oo * t;
     const float tf = (t + TAU_F) * (1.0f + 2.0f * ETA_TF);   // >= e^(-r2/2) at this path's r2
-    const float te = tf * (1.0f + 0.65f * br2);               // >= e^(-r2/2) at emit_pbvr's r2 (br2 <= 1)
+    // e^(br2/2) <= 1 + 0.65 br2 for br2 <= 1, and <= 2^ceil(0.7214 br2) (0.7214 >= log2(e) / 2) above
+    const float grow = br2 <= 1.0f ? 1.0f + 0.65f * br2 : __builtin_amdgcn_ldexpf(1.0f, (int)ceilf(0.7214f * br2));
+    const float te = tf * grow;                               // >= e^(-r2/2) at emit_pbvr's r2
     const float dt = te * (0.5f * br2 + ETA_TE) + ETA_TF * tf + (TAU_E + TAU_F);
     const float dy = (oo * dt + 2.0f * EPS * y) * 1.01f;     // >= |y - emit_pbvr's y|
     const float x = fmin(y, 1.0f - 1e-6f);
