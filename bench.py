@@ -310,7 +310,7 @@ def main():
     ap.add_argument("--no-cull", action="store_true", help="skip the timing of the culled renders")
     ap.add_argument("--heldout", type=int, default=10, help="held-out scenes (seeds 200..) for the denoiser")
     ap.add_argument("--tuning", default=None,
-                    help="waves,ppt,items,batch,window[,count_only]: fixed launch tuning (implies --no-autotune)")
+                    help="waves,ppt,items,batch,window[,count_only[,block]]: fixed launch tuning (implies --no-autotune)")
     ap.add_argument("--ply", default=None, help="bench-ply: the 3DGS PLY file (default: the test fixture)")
     ap.add_argument("--size", default="480x270", help="bench-ply: WxH")
     ap.add_argument("--spp", type=int, default=1, help="bench-ply: SPP side")
@@ -331,7 +331,7 @@ def main():
         v = [int(x) for x in a.tuning.split(",")]
         gh.set_tuning(particle_waves_per_cu=v[0], particles_per_thread=v[1], sort_items_per_thread=v[2],
                       batch_ensembles=v[3], depth_window_mib=v[4],
-                      count_only=v[5] if len(v) > 5 else 1)
+                      count_only=v[5] if len(v) > 5 else 1, particle_block=v[6] if len(v) > 6 else 256)
         a.no_autotune = True
     if a.mode == "calib":
         x = np.ones(64 << 20, np.uint32)

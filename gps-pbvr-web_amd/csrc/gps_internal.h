@@ -37,7 +37,6 @@ constexpr int VIEW_VEC4 = 6;          // float4 per splat in the view cache
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
-constexpr int PART_BLOCK = 256;
 constexpr int MAX_BATCH = 64;         // ensembles per batch (one depth|id buffer each)
 constexpr uint64_t EMPTY64 = ~0ull;
 
@@ -49,6 +48,7 @@ struct GpsTuningDev {
     uint32_t batch_ensembles;         // ensembles per particle launch, 0 = from depth_window_mib
     uint32_t depth_window_mib;        // depth|id bytes one launch may span when batch_ensembles is 0
     uint32_t count_only;              // 1: count-only path for the particles of off-screen PBVR splats
+    uint32_t particle_block;          // particle kernel workgroup size: 128, 256 or 512 threads
 };
 GpsTuningDev gps_tuning_current();
 
@@ -72,7 +72,7 @@ void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uin
 void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
                       const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
                       const float *lut, uint64_t *stats, int grid, uint64_t *pkeys = nullptr,
-                      uint32_t *pidx = nullptr, int ppt = 4, const uint32_t *perm = nullptr);
+                      uint32_t *pidx = nullptr, int ppt = 4, const uint32_t *perm = nullptr, int block = 256);
 void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,
                     uint32_t nens, const float4 *colour, float4 *accum, int clear, uint64_t *stats,
                     float4 *mom = nullptr, float *zmin = nullptr);

@@ -903,8 +903,8 @@ GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) 
 // OUT = 0: packed 64-bit minimum into the depth|id buffer of the particle's ensemble.
 // OUT = 1: (single ensemble) per-particle key (depth key << 32 | particle index) and subpixel into pkeys /
 //          pidx at the particle's index, for the depth-sorted resolve.
-template <int MK, int PPT, int OUT>
-__global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const float4 *__restrict__ view,
+template <int MK, int PPT, int OUT, int BS>
+__global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__restrict__ view,
                                                           const uint32_t *__restrict__ offsets,
                                                           const uint32_t *__restrict__ totals, uint32_t nens,
                                                           uint32_t first_seed, uint64_t *__restrict__ depth,
@@ -912,7 +912,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
                                                           uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys,
                                                           uint32_t *__restrict__ pidx,
                                                           const uint32_t *__restrict__ perm) {
-    constexpr uint32_t CH = PART_BLOCK * PPT;
+    constexpr uint32_t CH = BS * PPT;
     // PBVR packed-minimum path: particles of splats marked off-screen go through count_only; the undecided
     // ones are queued as (splat, index within the splat | ensemble << 24) and re-evaluated with emit_pbvr by
     // the whole block once the queue may not hold another chunk's worth, and at the end
@@ -921,9 +921,10 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
     constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
     __shared__ uint32_t s_lo, s_hi, s_qn;
-    __shared__ uint64_t s_pts[PART_BLOCK / 64];
+    __shared__ uint64_t s_pts[BS / 64];
     __shared__ uint2 s_q[QCAP];
     static_assert(PPT == 1 || PPT == 2 || PPT == 4 || PPT == 8, "particles per thread");
+    static_assert(BS == 128 || BS == 256 || BS == 512, "workgroup size");
     const bool count_only_on = COUNT_ONLY && (f.flags & FLAG_FAST) != 0u;
     if (threadIdx.x == 0) {
         s_qn = 0u;
@@ -939,7 +940,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
     const uint32_t N = f.N;
     uint64_t pts = 0;
     auto flush = [&](uint32_t nq) {
-        for (uint32_t t = threadIdx.x; t < nq; t += PART_BLOCK) {
+        for (uint32_t t = threadIdx.x; t < nq; t += BS) {
             const uint2 e = s_q[t];
             const uint32_t k = e.y & 0xFFFFFFu, b = e.y >> 24;
             Rng r(particle_seed_mixed(e.x, k + 1u, (first_seed + b) * 2654435761u + 1u));
@@ -1031,7 +1032,7 @@ __global__ __launch_bounds__(PART_BLOCK) void k_particles(GpsFrame f, const floa
         __syncthreads();
         if (threadIdx.x == 0) {
             uint64_t t = 0;
-            for (int w = 0; w < PART_BLOCK / 64; w++) t += s_pts[w];
+            for (int w = 0; w < BS / 64; w++) t += s_pts[w];
             if (t) atomicAdd((unsigned long long *)&stats[ST_POINTS], (unsigned long long)t);
         }
     }
@@ -1293,10 +1294,16 @@ void launch_count_scan(hipStream_t s, const GpsFrame &f, const uint2 *cinfo, uin
 void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, const uint32_t *offsets,
                       const BatchState &bs, uint32_t nens, uint32_t first_seed, uint64_t *depth, uint64_t sub_count,
                       const float *lut, uint64_t *stats, int grid, uint64_t *pkeys, uint32_t *pidx, int ppt,
-                      const uint32_t *perm) {
-#define GPS_LAUNCH(MK, PPT, OUT)                                                                                   \
-    hipLaunchKernelGGL((k_particles<MK, PPT, OUT>), dim3(grid), dim3(PART_BLOCK), 0, s, f, view, offsets, bs.totals, \
+                      const uint32_t *perm, int block) {
+#define GPS_LAUNCH_BS(MK, PPT, OUT, BS)                                                                            \
+    hipLaunchKernelGGL((k_particles<MK, PPT, OUT, BS>), dim3(grid), dim3(BS), 0, s, f, view, offsets, bs.totals,   \
                        nens, first_seed, depth, sub_count, lut, stats, pkeys, pidx, OUT ? nullptr : perm)
+#define GPS_LAUNCH(MK, PPT, OUT)                                         \
+    do {                                                                 \
+        if (block == 128) GPS_LAUNCH_BS(MK, PPT, OUT, 128);              \
+        else if (block == 512) GPS_LAUNCH_BS(MK, PPT, OUT, 512);         \
+        else GPS_LAUNCH_BS(MK, PPT, OUT, 256);                           \
+    } while (0)
 #define GPS_LAUNCH_PPT(MK, OUT)          \
     do {                                 \
         if (ppt == 1) GPS_LAUNCH(MK, 1, OUT); \
@@ -1316,6 +1323,7 @@ void launch_particles(hipStream_t s, const GpsFrame &f, const float4 *view, cons
     }
 #undef GPS_LAUNCH_PPT
 #undef GPS_LAUNCH
+#undef GPS_LAUNCH_BS
 }
 
 void launch_resolve(hipStream_t s, const GpsFrame &f, uint64_t *depth, uint64_t sub_count, const uint32_t *skip,

@@ -40,7 +40,7 @@ void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *fir
 namespace {
 
 thread_local std::string g_create_error;
-GpsTuningDev g_tuning = {8u, 4u, 64u, 0u, 64u, 1u};
+GpsTuningDev g_tuning = {8u, 4u, 64u, 0u, 64u, 1u, 256u};
 
 constexpr uint64_t MAX_SPLATS = 1ull << 30;
 constexpr uint32_t LUT_O = 65, LUT_V = 129;
@@ -398,7 +398,8 @@ int render_sorted_particles(GpsRenderer *r, const GpsFrame &f, uint32_t first_se
             {
                 Prof pr(r, "particles_keys");
                 launch_particles(r->stream, f, r->view, r->offsets, r->bs, 1, seed, r->depth, S, r->lut, r->stats,
-                                 grid, r->pkeys[0], r->pidx[0], (int)g_tuning.particles_per_thread);
+                                 grid, r->pkeys[0], r->pidx[0], (int)g_tuning.particles_per_thread, nullptr,
+                                 (int)g_tuning.particle_block);
             }
             int res = 0;
             // keys are written at their particle index, so the low word is already in order: 4 passes
@@ -684,7 +685,8 @@ static int enqueue_ensembles(GpsRenderer *r, hipStream_t st, const GpsCamera *ca
             {
                 Prof pr(r, "particles");
                 launch_particles(st, f, r->view, r->offsets, r->bs, B, seed0, r->depth, S, r->lut, r->stats,
-                                 grid, nullptr, nullptr, (int)g_tuning.particles_per_thread, perm);
+                                 grid, nullptr, nullptr, (int)g_tuning.particles_per_thread, perm,
+                                 (int)g_tuning.particle_block);
             }
         } else if (r->n > 0) {
             Prof pr(r, "splat_pass");
@@ -1167,6 +1169,7 @@ void gps_tuning_get(GpsTuning *out) {
     out->batch_ensembles = g_tuning.batch_ensembles;
     out->depth_window_mib = g_tuning.depth_window_mib;
     out->count_only = g_tuning.count_only;
+    out->particle_block = g_tuning.particle_block;
 }
 
 int gps_tuning_set(const GpsTuning *t) {
@@ -1178,12 +1181,15 @@ int gps_tuning_set(const GpsTuning *t) {
     if (t->batch_ensembles > MAX_BATCH) return -1;
     if (t->depth_window_mib < 1 || t->depth_window_mib > (1u << 20)) return -1;
     if (t->count_only > 1) return -1;
+    const uint32_t pb = t->particle_block;
+    if (!(pb == 128 || pb == 256 || pb == 512)) return -1;
     g_tuning.particle_waves_per_cu = t->particle_waves_per_cu;
     g_tuning.particles_per_thread = ppt;
     g_tuning.sort_items_per_thread = t->sort_items_per_thread;
     g_tuning.batch_ensembles = t->batch_ensembles;
     g_tuning.depth_window_mib = t->depth_window_mib;
     g_tuning.count_only = t->count_only;
+    g_tuning.particle_block = pb;
     return 0;
 }
 
@@ -1220,6 +1226,7 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
     const uint32_t waves[] = {4u, 8u, 16u};
     const uint32_t ppts[] = {1u, 2u, 4u, 8u};
     const uint32_t items[] = {8u, 16u, 32u, 64u};
+    const uint32_t blocks[] = {128u, 256u, 512u};
     const bool sorted_path = p->path == GPS_PATH_PARTICLE_SORTED;
     hipEvent_t e0, e1;
     HIP_TRY(r, hipEventCreate(&e0));
@@ -1227,25 +1234,28 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
     int rc = 0;
     // one untimed call first so that lazily sized buffers exist before timing
     rc = gps_renderer_render_ensembles(r, cam, p, first_seed, 1);
-    for (uint32_t wv : waves)
-        for (uint32_t ppt : ppts)
-            for (uint32_t it : items) {
-                if (rc != 0) break;
-                if (!sorted_path && it != start.sort_items_per_thread) continue;
-                g_tuning.particle_waves_per_cu = wv;
-                g_tuning.particles_per_thread = ppt;
-                g_tuning.sort_items_per_thread = it;
-                (void)hipEventRecord(e0, r->stream);
-                rc = gps_renderer_render_ensembles(r, cam, p, first_seed, trial_ensembles);
-                (void)hipEventRecord(e1, r->stream);
-                (void)hipEventSynchronize(e1);
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (rc == 0 && ms < best_ms) {
-                    best_ms = ms;
-                    best = g_tuning;
+    for (uint32_t bl : blocks)
+        for (uint32_t wv : waves)
+            for (uint32_t ppt : ppts)
+                for (uint32_t it : items) {
+                    if (rc != 0) break;
+                    if (!sorted_path && it != start.sort_items_per_thread) continue;
+                    // the same threads per CU: waves x 256 threads of grid, whatever the workgroup size
+                    g_tuning.particle_block = bl;
+                    g_tuning.particle_waves_per_cu = wv * 256u / bl;
+                    g_tuning.particles_per_thread = ppt;
+                    g_tuning.sort_items_per_thread = it;
+                    (void)hipEventRecord(e0, r->stream);
+                    rc = gps_renderer_render_ensembles(r, cam, p, first_seed, trial_ensembles);
+                    (void)hipEventRecord(e1, r->stream);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (rc == 0 && ms < best_ms) {
+                        best_ms = ms;
+                        best = g_tuning;
+                    }
                 }
-            }
     double best_ms_per = best_ms / trial_ensembles;
     // second stage: ensembles per particle launch, through the depth window
     if (rc == 0 && !sorted_path && start.batch_ensembles == 0) {

@@ -305,10 +305,12 @@ typedef struct GpsTuning {
                                         off-screen are evaluated for the generated-points count on the certified
                                         count-only path (undecided particles on the full path); 0: every particle
                                         on the full path.  Images and counters are the same either way */
+    uint32_t particle_block;         /* particle kernel workgroup size: 128, 256 (default) or 512 threads */
 } GpsTuning;
 void gps_tuning_get(GpsTuning *out);
 int gps_tuning_set(const GpsTuning *t);
-/* Autotuner: times every candidate (waves per CU 4/8/16 x particles per thread 1/2/4/8, and sort tiles 8/16/32/64
+/* Autotuner: times every candidate (workgroup size 128/256/512 x 4/8/16 256-thread groups' worth of threads per CU x
+ * particles per thread 1/2/4/8, and sort tiles 8/16/32/64
  * on the sorted particle path) on trial_ensembles ensembles of this scene and view, then the depth window
  * (8 .. 4096 MiB, with batch_ensembles 0) on max(trial_ensembles, 2 x the ensembles per launch it gives),
  * keeps the fastest as the process tuning, and restores the accumulation buffer, denoiser features, counters

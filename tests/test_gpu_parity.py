@@ -509,10 +509,11 @@ def test_tunings_are_bit_identical():
     saved = gh.get_tuning()
     ref = None
     try:
-        for waves, ppt, items, batch in ((8, 4, 16, 0), (4, 1, 8, 3), (16, 8, 32, 1), (1, 2, 8, 7)):
-            say(f"[tuning] waves={waves} ppt={ppt} items={items} batch={batch}")
+        for waves, ppt, items, batch, block in ((8, 4, 16, 0, 256), (4, 1, 8, 3, 128), (16, 8, 32, 1, 512),
+                                                (1, 2, 8, 7, 256), (2, 8, 64, 0, 128), (3, 4, 16, 2, 512)):
+            say(f"[tuning] waves={waves} ppt={ppt} items={items} batch={batch} block={block}")
             gh.set_tuning(particle_waves_per_cu=waves, particles_per_thread=ppt, sort_items_per_thread=items,
-                          batch_ensembles=batch)
+                          batch_ensembles=batch, particle_block=block)
             for path in (1, 2):
                 r = gh.GpsRenderer(128, 128, 2)
                 r.set_gaussians(scene)
