@@ -42,7 +42,7 @@ constexpr uint64_t EMPTY64 = ~0ull;
 
 // process-wide launch tuning (include/gps_hip.h GpsTuning); results never depend on it
 struct GpsTuningDev {
-    uint32_t particle_waves_per_cu;   // persistent particle grid = CUs * this
+    uint32_t particle_waves_per_cu;   // particle grid = CUs * this workgroups (chunks are grid-strided)
     uint32_t particles_per_thread;    // 1, 2, 4 or 8
     uint32_t sort_items_per_thread;   // 8 or 16
     uint32_t batch_ensembles;         // ensembles per particle launch, 0 = from depth_window_mib

@@ -10,7 +10,7 @@
 //   k_count_scan  per batch of ensembles: Poisson counts fused with a
 //                 single-pass exclusive scan (decoupled look-back, ticketed
 //                 tiles), particle totals and the u32-overflow skip
-//   k_particles   per batch: persistent grid-stride kernel over every
+//   k_particles   per batch: grid-stride kernel over every
 //                 particle of every ensemble of the batch; one packed
 //                 64-bit atomicMin((depth key << 32) | splat id) per particle
 //                 replaces the reference's two regenerating passes

@@ -40,7 +40,7 @@ void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *fir
 namespace {
 
 thread_local std::string g_create_error;
-GpsTuningDev g_tuning = {8u, 4u, 64u, 0u, 64u, 1u, 256u};
+GpsTuningDev g_tuning = {64u, 4u, 64u, 0u, 64u, 1u, 256u};
 
 constexpr uint64_t MAX_SPLATS = 1ull << 30;
 constexpr uint32_t LUT_O = 65, LUT_V = 129;
@@ -1177,7 +1177,7 @@ int gps_tuning_set(const GpsTuning *t) {
     if (!(ppt == 1 || ppt == 2 || ppt == 4 || ppt == 8)) return -1;
     const uint32_t si = t->sort_items_per_thread;
     if (!(si == 8 || si == 16 || si == 32 || si == 64)) return -1;
-    if (t->particle_waves_per_cu < 1 || t->particle_waves_per_cu > 64) return -1;
+    if (t->particle_waves_per_cu < 1 || t->particle_waves_per_cu > 1024) return -1;
     if (t->batch_ensembles > MAX_BATCH) return -1;
     if (t->depth_window_mib < 1 || t->depth_window_mib > (1u << 20)) return -1;
     if (t->count_only > 1) return -1;
@@ -1223,7 +1223,7 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
     const GpsTuningDev start = g_tuning;
     GpsTuningDev best = start;
     double best_ms = 1e30;
-    const uint32_t waves[] = {4u, 8u, 16u};
+    const uint32_t waves[] = {16u, 32u, 64u, 128u, 256u};
     const uint32_t ppts[] = {1u, 2u, 4u, 8u};
     const uint32_t items[] = {8u, 16u, 32u, 64u};
     const uint32_t blocks[] = {128u, 256u, 512u};

@@ -293,7 +293,8 @@ void gps_ply_to_gpu(const GpsPly *g, GpsGaussian *out);
 /* ---- launch tuning (process-wide; no counterpart in the reference).  Rendered bits never depend on it:
  * it only changes how the work is split into launches.  Renderers pick it up on their next call. */
 typedef struct GpsTuning {
-    uint32_t particle_waves_per_cu;  /* persistent particle grid = CUs x this (1..64), default 8 */
+    uint32_t particle_waves_per_cu;  /* particle grid = CUs x this workgroups (1..1024), default 64; grids
+                                        larger than the resident workgroups balance uneven chunks better */
     uint32_t particles_per_thread;   /* 1, 2, 4 (default) or 8 */
     uint32_t sort_items_per_thread;  /* radix sort tile of 256 x this keys: 8, 16, 32 or 64 (default; run
                                         as 512 or 1024 threads x 16) */
@@ -309,8 +310,8 @@ typedef struct GpsTuning {
 } GpsTuning;
 void gps_tuning_get(GpsTuning *out);
 int gps_tuning_set(const GpsTuning *t);
-/* Autotuner: times every candidate (workgroup size 128/256/512 x 4/8/16 256-thread groups' worth of threads per CU x
- * particles per thread 1/2/4/8, and sort tiles 8/16/32/64
+/* Autotuner: times every candidate (workgroup size 128/256/512 x 16/32/64/128/256 256-thread groups' worth of
+ * threads per CU x particles per thread 1/2/4/8, and sort tiles 8/16/32/64
  * on the sorted particle path) on trial_ensembles ensembles of this scene and view, then the depth window
  * (8 .. 4096 MiB, with batch_ensembles 0) on max(trial_ensembles, 2 x the ensembles per launch it gives),
  * keeps the fastest as the process tuning, and restores the accumulation buffer, denoiser features, counters
