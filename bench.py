@@ -80,6 +80,33 @@ def time_to_psnr(r, cam, p, ms_per_ens, ref, target_db=40.0):
     return {"ensembles": None, "gpu_ms": None, "psnr_db": curve[-1][1], "curve": curve}
 
 
+def bench_atomics(a):
+    """64-bit atomicMin throughput (gps_atomic_bench.hip) against which the particle kernel's L2 atomic rate is
+    read: buffers of 8 MiB (one 1280x720 ensemble's depth|id buffer), 32 MiB (a four-ensemble launch), 256 MiB
+    and 2 GiB; address windows per wave and iteration of 1 (all lanes on one address), 64, 1024, 16384 entries
+    and the whole buffer; plain atomics and write_min's load-then-atomic form."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "gps-pbvr-web_amd", "libgps_atomicbench.so"))
+    lib.gps_atomic_bench.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    blocks, per_thread, reps = 4096, 128, 5
+    n_atomics = blocks * 256 * per_thread
+    rows = []
+    for log2_n in (20, 22, 25, 28):
+        for log2_w in (0, 6, 10, 14, log2_n):
+            for mode in (0, 1):
+                ms = ctypes.c_float()
+                rc = lib.gps_atomic_bench(log2_n, log2_w, mode, blocks, per_thread, reps, ctypes.byref(ms))
+                if rc != 0:
+                    raise RuntimeError(f"gps_atomic_bench({log2_n}, {log2_w}, {mode}) returned {rc}")
+                row = dict(buffer_mib=(8 << log2_n) >> 20, window=1 << log2_w, mode=("atomic", "load+atomic")[mode],
+                           ms=round(ms.value, 4), g_atomics_per_s=round(n_atomics / ms.value / 1e6, 2))
+                log(f"[atomics] {row}")
+                rows.append(row)
+    print(json.dumps({"metric": "64-bit atomicMin throughput", "unit": "G atomics/s", "atomics_per_launch": n_atomics,
+                      "launch": f"{blocks} x 256 threads x {per_thread} atomics", "reps": reps, "rows": rows}),
+          flush=True)
+
+
 def bench_sort3m(a):
     """3,000,000 clustered Gaussians (seed 2, SH3), 1920x1080, SPP side 2, GPS, cap 65,536: the packed-minimum
     resolve and the depth-sorted resolve (64-bit onesweep sort of every particle) per view."""
@@ -315,9 +342,10 @@ def main():
     ap.add_argument("--size", default="480x270", help="bench-ply: WxH")
     ap.add_argument("--spp", type=int, default=1, help="bench-ply: SPP side")
     ap.add_argument("--mode", default="bench",
-                    choices=["bench", "calib", "sort3m", "batch8", "denoise", "bench-ply"],
+                    choices=["bench", "calib", "atomics", "sort3m", "batch8", "denoise", "bench-ply"],
                     help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
-                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters; atomics: the 64-bit "
+                         "atomicMin throughput microbenchmark")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
@@ -340,6 +368,8 @@ def main():
         print(json.dumps({"calib_bytes_read": 4 * x.size, "calib_bytes_written": 4 * x.size}), flush=True)
         return
 
+    if a.mode == "atomics":
+        return bench_atomics(a)
     if a.mode == "sort3m":
         return bench_sort3m(a)
     if a.mode == "batch8":
@@ -440,6 +470,18 @@ def main():
         b = kernel_bytes(k, n, n_vis, 45, particles, Sx, Sx, ens, c)
         kernels[k] = {"launches": c, "avg_ms": round(t / c, 4),
                       "GBps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 and b else None}
+
+    # 64-bit atomic requests per ensemble counted by rocprofv3 (TCC_ATOMIC) on this workload, against the
+    # atomicMin ceiling that gps_atomic_bench.hip measured
+    apath = os.path.join(ROOT, "profiles", "particles_atomics.json")
+    if head == "near" and "particles" in kernels and os.path.exists(apath):
+        at = json.load(open(apath))
+        kp = kernels["particles"]
+        req = at["l2_atomic_requests_per_ensemble"] * ens / kp["launches"]
+        per_s = req / (kp["avg_ms"] * 1e-3)
+        ceil = at["ceiling"]["g_requests_per_s"] * 1e9
+        kp["u64_atomics"] = {"requests_per_launch": int(req), "per_s": round(per_s), "ceiling_per_s": round(ceil),
+                             "frac": round(per_s / ceil, 5), "source": "profiles/particles_atomics.json"}
 
     ttp, heldout = None, None
     if not a.no_psnr:
