@@ -702,6 +702,25 @@ GPS_DEV uint32_t owner_search(const uint32_t *__restrict__ offs, uint32_t lo, ui
     return lo;
 }
 
+// owner_search over [0, N - 1] by one whole wave (every lane calls it with the same arguments): each step samples
+// 64 evenly spaced positions of the remaining range, one per lane, and keeps the interval after the last sample
+// <= p, so N = 560k takes 4 dependent loads instead of 20.  offs[0] = 0 <= p.
+GPS_DEV uint32_t owner_search_wave(const uint32_t *__restrict__ offs, uint32_t N, uint32_t p) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t lo = 0, hi = N - 1u;   // offs[lo] <= p, and the owner is in [lo, hi]
+    while (hi - lo >= 64u) {
+        const uint32_t step = (hi - lo) / 64u + 1u;   // 64 samples cover hi - lo + 1 positions
+        const uint32_t q = lo + lane * step;
+        const uint64_t m = __ballot(q <= hi && offs[q] <= p);   // lane 0 (q = lo) is always set
+        const uint32_t last = 63u - (uint32_t)__clzll(m);
+        lo += last * step;
+        hi = min(hi, lo + step - 1u);
+    }
+    const uint32_t q = lo + lane;
+    const uint64_t m = __ballot(q <= hi && offs[q] <= p);
+    return lo + 63u - (uint32_t)__clzll(m);
+}
+
 // A generated particle: the subpixel it writes (NO_HIT if none) and its ordered depth key.
 constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
 struct Hit {
@@ -959,9 +978,10 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
         const uint32_t *offs = offsets + (size_t)b * N;
         uint64_t *dep = depth + (size_t)b * sub_count;
         const uint32_t seed_mix = (first_seed + b) * 2654435761u + 1u;
-        if (threadIdx.x < 2u) {
-            uint32_t s = owner_search(offs, 0u, N - 1u, threadIdx.x == 0 ? p0 : pend - 1u);
-            if (threadIdx.x == 0) s_lo = s; else s_hi = s;
+        if (threadIdx.x < 128u) {   // waves 0 and 1: the owners of the chunk's first and last particles
+            const uint32_t s = owner_search_wave(offs, N, threadIdx.x < 64u ? p0 : pend - 1u);
+            if (threadIdx.x == 0) s_lo = s;
+            else if (threadIdx.x == 64u) s_hi = s;
         }
         __syncthreads();
         const uint32_t pfirst = p0 + threadIdx.x * PPT;
