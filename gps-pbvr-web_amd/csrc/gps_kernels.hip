@@ -939,7 +939,9 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
     constexpr uint32_t QCAP = COUNT_ONLY ? 2u * CH : 1u;
     constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
+    constexpr uint32_t OCAP = 512u;   // chunk splat ranges up to this many positions are staged in LDS
     __shared__ uint32_t s_lo, s_hi, s_qn;
+    __shared__ uint32_t s_off[OCAP + 1], s_prm[OCAP];
     __shared__ uint64_t s_pts[BS / 64];
     __shared__ uint2 s_q[QCAP];
     static_assert(PPT == 1 || PPT == 2 || PPT == 4 || PPT == 8, "particles per thread");
@@ -984,14 +986,40 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
             else if (threadIdx.x == 64u) s_hi = s;
         }
         __syncthreads();
+        // the chunk's positions s_lo..s_hi: their first particles (and the next position's, or the total) and
+        // splats, staged in LDS when a search among them would take more than a few steps and they fit
+        const uint32_t lo = s_lo, R = s_hi - lo + 1u;
+        const bool staged = R > 16u && R <= OCAP;
+        if (staged) {
+            for (uint32_t t = threadIdx.x; t <= R; t += BS) {
+                const uint32_t gi = lo + t;
+                s_off[t] = gi < N ? offs[gi] : total;
+                if (t < R) s_prm[t] = perm ? perm[gi] : gi;
+            }
+            __syncthreads();
+        }
         const uint32_t pfirst = p0 + threadIdx.x * PPT;
         if (pfirst < pend) {
-            uint32_t i = owner_search(offs, s_lo, s_hi, pfirst);   // position in the splat order of the offsets
-            uint32_t cur = offs[i];
-            uint32_t nxt = (i + 1u < N) ? offs[i + 1u] : total;
+            // position in the splat order of the offsets, its first particle and the next position's
+            uint32_t i, cur, nxt, sp;
+            if (staged) {
+                uint32_t a = 0, z = R - 1u;
+                while (a < z) {
+                    const uint32_t mid = (a + z + 1u) >> 1;
+                    if (s_off[mid] <= pfirst) a = mid; else z = mid - 1u;
+                }
+                i = lo + a;
+                cur = s_off[a];
+                nxt = s_off[a + 1u];
+                sp = s_prm[a];
+            } else {
+                i = owner_search(offs, lo, s_hi, pfirst);
+                cur = offs[i];
+                nxt = (i + 1u < N) ? offs[i + 1u] : total;
+                sp = perm ? perm[i] : i;
+            }
             // the splat's view record stays in registers while its particles run (loaded when the splat changes,
             // so its latency overlaps the seed hashing instead of stalling each particle)
-            uint32_t sp = perm ? perm[i] : i;
             float4 v[NV];
 #pragma unroll
             for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
@@ -999,13 +1027,13 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
             for (uint32_t j = 0; j < PPT; j++) {
                 const uint32_t p = pfirst + j;
                 if (p >= pend) break;
-                if (p >= nxt) {
+                if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
                     do {
                         i++;
                         cur = nxt;
-                        nxt = (i + 1u < N) ? offs[i + 1u] : total;
+                        nxt = staged ? s_off[i + 1u - lo] : ((i + 1u < N) ? offs[i + 1u] : total);
                     } while (p >= nxt);
-                    sp = perm ? perm[i] : i;
+                    sp = staged ? s_prm[i - lo] : (perm ? perm[i] : i);
 #pragma unroll
                     for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
                 }
