@@ -702,16 +702,30 @@ GPS_DEV uint32_t owner_search(const uint32_t *__restrict__ offs, uint32_t lo, ui
     return lo;
 }
 
-// owner_search over [0, N - 1] by one whole wave (every lane calls it with the same arguments): each step samples
-// 64 evenly spaced positions of the remaining range, one per lane, and keeps the interval after the last sample
-// <= p, so N = 560k takes 4 dependent loads instead of 20.  offs[0] = 0 <= p.
-GPS_DEV uint32_t owner_search_wave(const uint32_t *__restrict__ offs, uint32_t N, uint32_t p) {
+// owner_search over [lo, hi] (offs[lo] <= p) by one whole wave (every lane calls it with the same arguments):
+// each step samples 64 evenly spaced positions of the remaining range, one per lane, and keeps the interval after
+// the last sample <= p, so N = 560k takes 4 dependent loads instead of 20.  With gallop, when the owner is
+// expected just after lo, the first steps sample the 64 positions from lo, then every 64th of the next 4096.
+GPS_DEV uint32_t owner_search_wave(const uint32_t *__restrict__ offs, uint32_t lo, uint32_t hi, uint32_t p,
+                                   bool gallop) {
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t lo = 0, hi = N - 1u;   // offs[lo] <= p, and the owner is in [lo, hi]
-    while (hi - lo >= 64u) {
+    if (gallop) {
+        for (uint32_t s = 1u; s <= 64u; s *= 64u) {
+            const uint32_t q = lo + lane * s;
+            const uint64_t m = __ballot(q <= hi && offs[q] <= p);   // lane 0 (q = lo) is always set
+            const uint32_t last = 63u - (uint32_t)__clzll(m);
+            if (last < 63u || hi - lo <= 63u * s) {   // the owner is in [lo + last s, lo + last s + s - 1]
+                lo += last * s;
+                hi = min(hi, lo + s - 1u);
+                break;
+            }
+            lo += 63u * s;
+        }
+    }
+    while (hi - lo >= 64u) {   // offs[lo] <= p, and the owner is in [lo, hi]
         const uint32_t step = (hi - lo) / 64u + 1u;   // 64 samples cover hi - lo + 1 positions
         const uint32_t q = lo + lane * step;
-        const uint64_t m = __ballot(q <= hi && offs[q] <= p);   // lane 0 (q = lo) is always set
+        const uint64_t m = __ballot(q <= hi && offs[q] <= p);
         const uint32_t last = 63u - (uint32_t)__clzll(m);
         lo += last * step;
         hi = min(hi, lo + step - 1u);
@@ -940,6 +954,7 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
     constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
     constexpr uint32_t OCAP = 512u;   // chunk splat ranges up to this many positions are staged in LDS
+    constexpr uint32_t CRUN = 8u;     // consecutive chunks per block turn
     __shared__ uint32_t s_lo, s_hi, s_qn;
     __shared__ uint32_t s_off[OCAP + 1], s_prm[OCAP];
     __shared__ uint64_t s_pts[BS / 64];
@@ -971,7 +986,10 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
                 write_min(depth + (size_t)b * sub_count + h.idx, ((uint64_t)h.key << 32) | (uint64_t)e.x);
         }
     };
-    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    uint32_t prev = 0xFFFFFFFFu, prev_hi = 0;   // the block's previous chunk and the owner of its last particle
+    // runs of CRUN consecutive chunks per block: a chunk's owners are searched from the previous chunk's last
+    for (uint32_t chunk = blockIdx.x * CRUN; chunk < nchunks;
+         chunk = (chunk % CRUN == CRUN - 1u) ? chunk + (gridDim.x - 1u) * CRUN + 1u : chunk + 1u) {
         uint32_t b = 0;
         while (b + 1u < nens && s_cstart[b + 1u] <= chunk) b++;
         const uint32_t total = totals[b];
@@ -980,12 +998,16 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
         const uint32_t *offs = offsets + (size_t)b * N;
         uint64_t *dep = depth + (size_t)b * sub_count;
         const uint32_t seed_mix = (first_seed + b) * 2654435761u + 1u;
+        const bool follows = prev + 1u == chunk && chunk > s_cstart[b];   // the same ensemble's next chunk
         if (threadIdx.x < 128u) {   // waves 0 and 1: the owners of the chunk's first and last particles
-            const uint32_t s = owner_search_wave(offs, N, threadIdx.x < 64u ? p0 : pend - 1u);
+            const uint32_t s = owner_search_wave(offs, follows ? prev_hi : 0u, N - 1u,
+                                                 threadIdx.x < 64u ? p0 : pend - 1u, follows);
             if (threadIdx.x == 0) s_lo = s;
             else if (threadIdx.x == 64u) s_hi = s;
         }
         __syncthreads();
+        prev = chunk;
+        prev_hi = s_hi;
         // the chunk's positions s_lo..s_hi: their first particles (and the next position's, or the total) and
         // splats, staged in LDS when a search among them would take more than a few steps and they fit
         const uint32_t lo = s_lo, R = s_hi - lo + 1u;
