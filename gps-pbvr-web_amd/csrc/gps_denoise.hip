@@ -32,7 +32,11 @@
 
 namespace {
 
-constexpr int NL = 5;                 // layers
+#ifndef GPS_DN_LAYERS
+#define GPS_DN_LAYERS 5
+#endif
+constexpr int NL = GPS_DN_LAYERS;     // layers (the first 8 -> 32 channels, the last 32 -> 3)
+static_assert(NL >= 3, "at least one hidden 32 -> 32 layer");
 constexpr int HID = 32;               // hidden channels
 constexpr int IN_CH = 8;              // feature channels as stored (7 used)
 constexpr int TW = 64, TH = 4;        // conv tile: 64 x 4 pixels, one row per wave
@@ -40,8 +44,18 @@ constexpr int HALO_W = TW + 2, HALO_H = TH + 2;
 constexpr int FB_TILE = 16;           // fallback decision tile
 
 // layer shapes: cin (as stored), cout (real)
-constexpr int L_CIN[NL] = {IN_CH, HID, HID, HID, HID};
-constexpr int L_COUT[NL] = {HID, HID, HID, HID, 3};
+struct LayerShapes {
+    int cin[NL], cout[NL];
+    constexpr LayerShapes() : cin(), cout() {
+        for (int l = 0; l < NL; l++) {
+            cin[l] = l == 0 ? IN_CH : HID;
+            cout[l] = l == NL - 1 ? 3 : HID;
+        }
+    }
+};
+constexpr LayerShapes SHAPES{};
+constexpr const int (&L_CIN)[NL] = SHAPES.cin;
+constexpr const int (&L_COUT)[NL] = SHAPES.cout;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -610,11 +624,12 @@ void backward(GpsDenoiser *d, uint32_t W, uint32_t H, uint32_t imgs) {
         hipLaunchKernelGGL(k_dn_reduce, dim3(cdiv(n, 256)), dim3(256), 0, d->stream, d->part, blocks, n,
                            d->g + L.w[l]);
     };
-    // layer 4: dy (8 ch, 3 used) with input act[4]
-    hipLaunchKernelGGL((k_dn_wgrad<HID, 3, IN_CH>), grid, dim3(256), 0, d->stream, d->act[4], d->dy8, W, H, d->part);
-    reduce(4);
-    conv<IN_CH, 2, EPI_MASK>(d, d->dy8, W, H, imgs, d->bwd[4], nullptr, d->gA, d->act[4], nullptr, nullptr, nullptr,
-                             nullptr, 0.0f, nullptr);
+    // the last layer: dy (8 ch, 3 used) with input act[NL - 1]
+    hipLaunchKernelGGL((k_dn_wgrad<HID, 3, IN_CH>), grid, dim3(256), 0, d->stream, d->act[NL - 1], d->dy8, W, H,
+                       d->part);
+    reduce(NL - 1);
+    conv<IN_CH, 2, EPI_MASK>(d, d->dy8, W, H, imgs, d->bwd[NL - 1], nullptr, d->gA, d->act[NL - 1], nullptr, nullptr,
+                             nullptr, nullptr, 0.0f, nullptr);
     uint16_t *cur = d->gA, *nxt = d->gB;
     for (int l = NL - 2; l >= 1; l--) {
         hipLaunchKernelGGL((k_dn_wgrad<HID, HID, HID>), grid, dim3(256), 0, d->stream, d->act[l], cur, W, H, d->part);

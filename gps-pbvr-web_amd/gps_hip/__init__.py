@@ -581,11 +581,21 @@ def frame_sphere_distance(radius, fov_y=0.9):
     return (max(radius, 1e-6) / math.sin(0.5 * fov_y)) * 1.25
 
 
-DENOISER_LAYERS = ((8, 32), (32, 32), (32, 32), (32, 32), (32, 3))  # (cin as stored, cout)
-
-
 def denoiser_param_count():
     return int(lib.gps_denoiser_param_count())
+
+
+def _denoiser_layers():
+    """(cin as stored, cout) per layer: 8 -> 32, then 32 -> 32 layers, then 32 -> 3; the library's parameter count
+    fixes how many hidden layers it was built with."""
+    first, hidden, last = 32 * 9 * 8 + 32, 32 * 9 * 32 + 32, 3 * 9 * 32 + 3
+    k, rem = divmod(denoiser_param_count() - first - last, hidden)
+    if rem or k < 1:
+        raise GpsError(f"unexpected denoiser parameter count {denoiser_param_count()}")
+    return ((8, 32),) + ((32, 32),) * k + ((32, 3),)
+
+
+DENOISER_LAYERS = _denoiser_layers()
 
 
 def denoiser_split(params):
