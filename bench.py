@@ -548,6 +548,20 @@ def main():
         t0 = time.perf_counter()
         O.mirror_render(g[:m_mt], sh[:m_mt * 45], 3, oc, mp, W, H, 1, 1)
         dtn = time.perf_counter() - t0
+        # the f64 port of the reference's own CPU particle renderer (oracle.rs:426-494), single-threaded like it:
+        # PBVR Extinction C3 with the radial correction and per-particle seeds, the same splats without their SH
+        # bands (it shades with the base colour) and without the per-splat cap, on 1/(16 frac) of the splats
+        frac64 = 16 * frac
+        m64 = max(1, n // frac64)
+        sub = g[:m64]
+        scene = [dict(pos=tuple(float(t) for t in x["pos_opacity"][:3]), log_scale=tuple(float(t) for t in x["log_scale"][:3]),
+                      rot_xyzw=(float(x["rot_wxyz"][1]), float(x["rot_wxyz"][2]), float(x["rot_wxyz"][3]),
+                                float(x["rot_wxyz"][0])),
+                      opacity=float(x["pos_opacity"][3]), color=tuple(float(t) for t in x["color"][:3])) for x in sub]
+        popt = O.default_particle_options(level=3, rule=1, radial_correction=1, per_particle_seed=1)
+        t0 = time.perf_counter()
+        O.render_particles_3d(scene, oc, W, H, 1, 1, tuple(float(t) for t in p.background), 1, popt)
+        dt64 = time.perf_counter() - t0
         model = ""
         try:
             model = subprocess.run(["bash", "-c", "lscpu | grep 'Model name' | head -1"], capture_output=True,
@@ -560,7 +574,12 @@ def main():
                          f"host {model}, {os.cpu_count()} cpus visible",
                "sample_seconds": round(dtn, 2),
                "single_thread": {"value": round(dt1 * 1000.0 * frac, 1), "cores": 1,
-                                 "sample": f"first {m} splats (1/{frac}), scaled x{frac}", "sample_seconds": round(dt1, 2)}}
+                                 "sample": f"first {m} splats (1/{frac}), scaled x{frac}", "sample_seconds": round(dt1, 2)},
+               "f64_reference_port": {"value": round(dt64 * 1000.0 * frac64, 1), "cores": 1,
+                                      "sample": f"f64 port of the reference's CPU particle renderer (render_particles_3d, "
+                                                f"per-particle seeds, base colour, no per-splat cap), first {m64} "
+                                                f"splats (1/{frac64}), scaled x{frac64}",
+                                      "sample_seconds": round(dt64, 2)}}
         log(f"[cpu] {cpu}")
 
     v = per_view[head]
