@@ -527,6 +527,43 @@ def test_tunings_are_bit_identical():
         gh.set_tuning(**saved)
 
 
+def sparse_visible_scene(seed=3):
+    """Mostly zero-particle splats (opacity 0: no particles) with visible ones at gaps of 1 to 12,000 positions,
+    so the offsets hold long runs of equal values for the chunk-owner searches to cross."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.choice([1, 3, 40, 70, 700, 5000, 12000], size=160)
+    idx = np.cumsum(gaps)
+    vis = S.small_uniform(len(idx), seed=seed)
+    g = np.zeros(int(idx[-1]) + 50, vis.dtype)
+    g["pos_opacity"][:, :3] = rng.uniform(-1.0, 1.0, (len(g), 3))
+    g["log_scale"][:, :3] = -3.5
+    g["rot_wxyz"][:, 0] = 1.0
+    g["pos_opacity"][:, 3] = 0.0
+    g[idx] = vis
+    return g
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_owner_search_across_zero_particle_runs(method):
+    """Chunk owners found across long runs of zero-particle splats, with tiny chunks (one particle per thread,
+    128-thread blocks) and runs of chunks that cross ensembles, bit-exact against the oracle."""
+    scene = sparse_visible_scene()
+    cam = S.small_uniform_camera(96, 80)
+    p = gh.RenderParams(spp_side=2, background=BG, method=method, max_points_per_splat=1 << 20,
+                        pbvr=dict(radial_correction=1))
+    saved = gh.get_tuning()
+    try:
+        for waves, ppt, block, batch, keep in ((1, 1, 128, 3, True), (2, 4, 256, 0, False)):
+            say(f"[owners] n={len(scene)} method={method} waves={waves} ppt={ppt} block={block} batch={batch}")
+            gh.set_tuning(particle_waves_per_cu=waves, particles_per_thread=ppt, particle_block=block,
+                          batch_ensembles=batch)
+            gpu, m = run_pair(scene, None, 0, cam, p, 96, 80, 3, 5, keep=keep)
+            assert gpu["stats"].visible_splats > 0
+            assert_exact(gpu, m)
+    finally:
+        gh.set_tuning(**saved)
+
+
 def test_autotune_keeps_history():
     scene = S.small_uniform(4096, seed=0)
     cam = S.small_uniform_camera(128, 128)
