@@ -55,7 +55,7 @@ def psnr(a, b):
 
 
 def reference_image(r, cam, p, ref_ensembles=1024, ref_seed=1_000_000):
-    """The judge of the PSNR legs: an independent ref_ensembles-ensemble plain render of the view."""
+    """The yardstick of the PSNR legs: an independent ref_ensembles-ensemble plain render of the view."""
     r.reset_accum()
     r.render_ensembles(cam, p, ref_seed, ref_ensembles)
     acc = r.read_accum()
@@ -348,7 +348,7 @@ def main():
                          "atomicMin throughput microbenchmark")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); it has no multi-GPU mode",
               file=sys.stderr)
         sys.exit(2)
 
