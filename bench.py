@@ -107,6 +107,42 @@ def bench_atomics(a):
           flush=True)
 
 
+def bench_alpha(a):
+    """The sorted alpha path (render_sorted: project, 24-bit radix sort of the splat keys, tile binning and its
+    sort, per-tile f32 blending) on the clustered 560k-Gaussian SH3 scene at 1280x720, at the three orbit
+    views: host wall time per frame (it includes the path's two host reads of the tile-pair count), the summed
+    HIP-event time of its profiled kernels, and the splat and tile-pair sorts' share."""
+    import gps_hip as gh
+    from gps_hip import scenes as S
+    W, H = 1280, 720
+    g, sh = S.clustered(560_000, seed=1, sh_degree=3)
+    p = gh.RenderParams(spp_side=1)
+    r = gh.GpsRenderer(W, H, 1)
+    r.set_gaussians_sh(g, sh, 3)
+    frames = max(a.steps, 1) * 4
+    out = {}
+    for v in [x for x in a.views.split(",") if x]:
+        cam = S.clustered_camera(v, W, H)
+        for _ in range(max(a.warmup, 1)):
+            r.render_sorted(cam, p)
+        r.wait_idle()
+        r.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            r.render_sorted(cam, p)
+        r.wait_idle()
+        wall = (time.perf_counter() - t0) * 1000.0 / frames
+        prof = r.profile_read()
+        r.profile(False)
+        out[v] = {"wall_ms_per_frame": round(wall, 3),
+                  "profiled_gpu_ms_per_frame": round(sum(t for _, t in prof.values()) / frames, 3),
+                  "kernels": {k: {"launches": c, "avg_ms": round(t / c, 4)} for k, (c, t) in prof.items()}}
+        log(f"[alpha {v}] {out[v]}")
+    print(json.dumps({"mode": "alpha", "workload": "clustered 560k Gaussians SH3 seed 1, 1280x720, sorted alpha path",
+                      "frames_per_view": frames, "views": out}), flush=True)
+    r.close()
+
+
 def bench_sort3m(a):
     """3,000,000 clustered Gaussians (seed 2, SH3), 1920x1080, SPP side 2, GPS, cap 65,536: the packed-minimum
     resolve and the depth-sorted resolve (64-bit onesweep sort of every particle) per view."""
@@ -342,7 +378,7 @@ def main():
     ap.add_argument("--size", default="480x270", help="bench-ply: WxH")
     ap.add_argument("--spp", type=int, default=1, help="bench-ply: SPP side")
     ap.add_argument("--mode", default="bench",
-                    choices=["bench", "calib", "atomics", "sort3m", "batch8", "denoise", "bench-ply"],
+                    choices=["bench", "calib", "atomics", "alpha", "sort3m", "batch8", "denoise", "bench-ply"],
                     help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters; atomics: the 64-bit "
                          "atomicMin throughput microbenchmark")
@@ -370,6 +406,8 @@ def main():
 
     if a.mode == "atomics":
         return bench_atomics(a)
+    if a.mode == "alpha":
+        return bench_alpha(a)
     if a.mode == "sort3m":
         return bench_sort3m(a)
     if a.mode == "batch8":
