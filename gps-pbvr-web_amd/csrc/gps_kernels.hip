@@ -950,7 +950,7 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
     // ones are queued as (splat, index within the splat | ensemble << 24) and re-evaluated with emit_pbvr by
     // the whole block once the queue may not hold another chunk's worth, and at the end
     constexpr bool COUNT_ONLY = MK == 1 && OUT == 0;
-    constexpr uint32_t QCAP = COUNT_ONLY ? 2u * CH : 1u;
+    constexpr uint32_t QCAP = COUNT_ONLY ? CH + BS : 1u;   // a chunk appends at most CH; flushed above BS
     constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
     constexpr uint32_t OCAP = 512u;   // chunk splat ranges up to this many positions are staged in LDS
