@@ -71,23 +71,23 @@ extern "C" int gps_atomic_bench(uint32_t log2_n, uint32_t log2_window, int mode,
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = -2;
     for (int it = 0; rc == 0 && it <= reps; it++) {   // iteration 0 warms up
         k_ab_fill<<<2048, 256>>>(buf, n);
-        hipEventRecord(e0, 0);
+        (void)hipEventRecord(e0, 0);
         if (mode == 0)
             k_ab_atomic_min<0><<<blocks, 256>>>(buf, log2_n, log2_window, per_thread, 0x1234567u * (it + 1));
         else
             k_ab_atomic_min<1><<<blocks, 256>>>(buf, log2_n, log2_window, per_thread, 0x1234567u * (it + 1));
-        hipEventRecord(e1, 0);
+        (void)hipEventRecord(e1, 0);
         if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) {
             rc = -2;
             break;
         }
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         if (it > 0) total += ms;
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipFree(buf);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
     if (rc == 0) *out_ms = total / (float)reps;
     return rc;
 }
