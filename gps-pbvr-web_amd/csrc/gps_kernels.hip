@@ -494,6 +494,11 @@ __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restri
                                 v1.y = bx.bnd[1];
                                 v1.z = bx.bnd[2];
                                 v3.w = 1.0f;
+                                // every particle in front of the near plane: the marker is minus a lower bound
+                                // of the radial keep ratio clamp(h(y) / h(oo), 0, 1) over y in [0, oo], since the
+                                // exact h >= (1 - ETA_HE) h(0) = 1 - ETA_HE and the division rounds correctly
+                                if (bx.cc[2] - bx.ext[2] > f.near_z + 1e-6f * (fabsf(bx.cc[2]) + 1.0f))
+                                    v3.w = -fmin((1.0f - gpsf::ETA_HE) / v3.z * (1.0f - 1e-6f), 1.0f);
                                 marked = true;
                             }
                         }
@@ -933,11 +938,29 @@ GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) 
     return -1;
 }
 
+// The count of a particle of a splat whose particles all pass the near plane (v[3].w < 0) from its draws alone,
+// without the arithmetic: 0 when the view-conditioned draw rejects; 1 when there is no radial correction, or
+// the keep draw is below -v[3].w, a lower bound of every particle's keep ratio; -1 when count_only must decide.
+GPS_DEV int keep_draw(const GpsFrame &f, uint32_t seed, const float4 *__restrict__ v) {
+    uint32_t c = pcg(pcg(pcg(pcg(seed))));   // the two normal pairs
+    if (f.method == M_VIEW_CONDITIONED) {
+        c = pcg(c);
+        if ((float)(c >> 8u) * (1.0f / 16777216.0f) >= v[1].w) return 0;
+    }
+    if (!(f.flags & FLAG_RADIAL)) return 1;
+    if (f.flags & FLAG_JITTER) c = pcg(pcg(c));
+    c = pcg(c);
+    return (float)(c >> 8u) * (1.0f / 16777216.0f) < -v[3].w ? 1 : -1;
+}
+
 // OUT = 0: packed 64-bit minimum into the depth|id buffer of the particle's ensemble.
 // OUT = 1: (single ensemble) per-particle key (depth key << 32 | particle index) and subpixel into pkeys /
 //          pidx at the particle's index, for the depth-sorted resolve.
+// Occupancy target: seven waves per SIMD at 256 threads (the LDS below, 22.3 KB at 4 particles per thread, lets a
+// CU hold seven blocks; the register limit costs a few bytes of spills on the PBVR packed-minimum path, which the
+// count-only views repay), six at 128 and 512 threads, where the exact path's spills would cost more (measured).
 template <int MK, int PPT, int OUT, int BS>
-__global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__restrict__ view,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 7 : 6))) void k_particles(GpsFrame f, const float4 *__restrict__ view,
                                                           const uint32_t *__restrict__ offsets,
                                                           const uint32_t *__restrict__ totals, uint32_t nens,
                                                           uint32_t first_seed, uint64_t *__restrict__ depth,
@@ -946,19 +969,23 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
                                                           uint32_t *__restrict__ pidx,
                                                           const uint32_t *__restrict__ perm) {
     constexpr uint32_t CH = BS * PPT;
-    // PBVR packed-minimum path: particles of splats marked off-screen go through count_only; the undecided
-    // ones are queued as (splat, index within the splat | ensemble << 24) and re-evaluated with emit_pbvr by
-    // the whole block once the queue may not hold another chunk's worth, and at the end
+    // PBVR packed-minimum path: a particle of a splat marked off-screen is settled by keep_draw when its draws
+    // alone decide; otherwise (splat, index within the splat | ensemble << 24) goes to its wave's buffer, which
+    // after each chunk runs count_only on whole waves of 64 entries.  Entries count_only cannot decide, and the
+    // buffer's last remainder, are queued for emit_pbvr, which the whole block runs once the queue may not hold
+    // another chunk's worth, and at the end.
     constexpr bool COUNT_ONLY = MK == 1 && OUT == 0;
     constexpr uint32_t QCAP = COUNT_ONLY ? CH + BS : 1u;   // a chunk appends at most CH; flushed above BS
     constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
-    constexpr uint32_t OCAP = 512u;   // chunk splat ranges up to this many positions are staged in LDS
+    constexpr uint32_t OCAP = 256u;   // chunk splat ranges up to this many positions are staged in LDS
     constexpr uint32_t CRUN = 8u;     // consecutive chunks per block turn
     __shared__ uint32_t s_lo, s_hi, s_qn;
     __shared__ uint32_t s_off[OCAP + 1], s_prm[OCAP];
     __shared__ uint64_t s_pts[BS / 64];
     __shared__ uint2 s_q[QCAP];
+    constexpr uint32_t RING = 64u * (PPT + 1u);   // per-wave count_only work: fewer than 64 left + PPT x 64 appended
+    __shared__ uint2 s_ring[COUNT_ONLY ? BS / 64 : 1][COUNT_ONLY ? RING : 1];
     static_assert(PPT == 1 || PPT == 2 || PPT == 4 || PPT == 8, "particles per thread");
     static_assert(BS == 128 || BS == 256 || BS == 512, "workgroup size");
     const bool count_only_on = COUNT_ONLY && (f.flags & FLAG_FAST) != 0u;
@@ -986,6 +1013,8 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
                 write_min(depth + (size_t)b * sub_count + h.idx, ((uint64_t)h.key << 32) | (uint64_t)e.x);
         }
     };
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t rn = 0;   // this wave's pending count_only work: entries [0, rn) of its buffer (the same in every lane)
     uint32_t prev = 0xFFFFFFFFu, prev_hi = 0;   // the block's previous chunk and the owner of its last particle
     // runs of CRUN consecutive chunks per block: a chunk's owners are searched from the previous chunk's last
     for (uint32_t chunk = blockIdx.x * CRUN; chunk < nchunks;
@@ -1021,9 +1050,9 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
             __syncthreads();
         }
         const uint32_t pfirst = p0 + threadIdx.x * PPT;
+        // position in the splat order of the offsets, its first particle, the next position's and the splat
+        uint32_t i = 0, cur = 0, nxt = 0, sp = 0;
         if (pfirst < pend) {
-            // position in the splat order of the offsets, its first particle and the next position's
-            uint32_t i, cur, nxt, sp;
             if (staged) {
                 uint32_t a = 0, z = R - 1u;
                 while (a < z) {
@@ -1040,15 +1069,24 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
                 nxt = (i + 1u < N) ? offs[i + 1u] : total;
                 sp = perm ? perm[i] : i;
             }
-            // the splat's view record stays in registers while its particles run (loaded when the splat changes,
-            // so its latency overlaps the seed hashing instead of stalling each particle)
-            float4 v[NV];
+        }
+        // the view record of splat vsp stays in registers while its particles run
+        float4 v[NV];
+        uint32_t vsp = 0xFFFFFFFFu;
+        auto load_view = [&](uint32_t s2) {
+            if (s2 != vsp) {
 #pragma unroll
-            for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
+                for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * s2 + q];
+                vsp = s2;
+            }
+        };
+        if (pfirst < pend) load_view(sp);
 #pragma unroll 1
-            for (uint32_t j = 0; j < PPT; j++) {
-                const uint32_t p = pfirst + j;
-                if (p >= pend) break;
+        for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the ring needs whole waves)
+            const uint32_t p = pfirst + j;
+            bool push = false;
+            uint32_t kb = 0;
+            if (p < pend) {
                 if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
                     do {
                         i++;
@@ -1056,30 +1094,68 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
                         nxt = staged ? s_off[i + 1u - lo] : ((i + 1u < N) ? offs[i + 1u] : total);
                     } while (p >= nxt);
                     sp = staged ? s_prm[i - lo] : (perm ? perm[i] : i);
-#pragma unroll
-                    for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
                 }
+                load_view(sp);
                 const uint32_t k = p - cur;
-                Rng r(particle_seed_mixed(sp, k + 1u, seed_mix));
+                const uint32_t seed = particle_seed_mixed(sp, k + 1u, seed_mix);
+                bool done = false;
                 if constexpr (COUNT_ONLY) {
                     if (count_only_on && v[3].w != 0.0f) {
-                        const int c = count_only(f, r, v);
-                        if (c >= 0) pts += (uint32_t)c;
-                        else s_q[atomicAdd(&s_qn, 1u)] = make_uint2(sp, k | (b << 24));
-                        continue;
+                        const int c = v[3].w < 0.0f ? keep_draw(f, seed, v) : -1;
+                        if (c >= 0) {
+                            pts += (uint32_t)c;
+                        } else {
+                            push = true;
+                            kb = k | (b << 24);
+                        }
+                        done = true;
                     }
                 }
-                Hit h;
-                if constexpr (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
-                else if constexpr (MK == 1) h = emit_pbvr(f, r, v);
-                else h = emit_occupancy(f, r, k, v);
-                pts += h.counted;
-                if (OUT == 0) {
-                    if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)sp);
-                } else {
-                    pkeys[p] = ((uint64_t)h.key << 32) | (uint64_t)p;
-                    pidx[p] = h.idx;
+                if (!done) {
+                    Rng r(seed);
+                    Hit h;
+                    if constexpr (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
+                    else if constexpr (MK == 1) h = emit_pbvr(f, r, v);
+                    else h = emit_occupancy(f, r, k, v);
+                    pts += h.counted;
+                    if (OUT == 0) {
+                        if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)sp);
+                    } else {
+                        pkeys[p] = ((uint64_t)h.key << 32) | (uint64_t)p;
+                        pidx[p] = h.idx;
+                    }
                 }
+            }
+            if constexpr (COUNT_ONLY) {
+                if (count_only_on) {   // every lane of the wave is here
+                    const uint64_t m = __ballot(push);
+                    if (push)
+                        s_ring[wave][rn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+                            make_uint2(sp, kb);
+                    rn += (uint32_t)__popcll(m);
+                }
+            }
+        }
+        if constexpr (COUNT_ONLY) {
+            if (count_only_on && rn >= 64u) {   // whole-wave rounds of count_only, then the rest to the front
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                uint32_t base = 0;
+                for (; base + 64u <= rn; base += 64u) {
+                    const uint2 e = s_ring[wave][base + lane];
+                    load_view(e.x);
+                    const uint32_t k2 = e.y & 0xFFFFFFu, b2 = e.y >> 24;
+                    Rng r(particle_seed_mixed(e.x, k2 + 1u, (first_seed + b2) * 2654435761u + 1u));
+                    const int c = count_only(f, r, v);
+                    if (c >= 0) pts += (uint32_t)c;
+                    else s_q[atomicAdd(&s_qn, 1u)] = e;
+                }
+                rn -= base;   // fewer than 64 left, at [base, base + rn); base >= 64 > rn, so the copy does not overlap
+                if (lane < rn) s_ring[wave][lane] = s_ring[wave][base + lane];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
         __syncthreads();
@@ -1093,6 +1169,12 @@ __global__ __launch_bounds__(BS) void k_particles(GpsFrame f, const float4 *__re
         }
     }
     if (COUNT_ONLY) {
+        if (count_only_on && rn) {   // the ring's remainder (fewer than 64), one entry per lane
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < rn) s_q[atomicAdd(&s_qn, 1u)] = s_ring[wave][lane];   // to emit_pbvr
+        }
         __syncthreads();
         flush(s_qn);
     }
