@@ -956,11 +956,13 @@ GPS_DEV int keep_draw(const GpsFrame &f, uint32_t seed, const float4 *__restrict
 // OUT = 0: packed 64-bit minimum into the depth|id buffer of the particle's ensemble.
 // OUT = 1: (single ensemble) per-particle key (depth key << 32 | particle index) and subpixel into pkeys /
 //          pidx at the particle's index, for the depth-sorted resolve.
-// Occupancy target: seven waves per SIMD at 256 threads (the LDS below, 22.3 KB at 4 particles per thread, lets a
-// CU hold seven blocks; the register limit costs a few bytes of spills on the PBVR packed-minimum path, which the
+// Occupancy target of the PBVR packed-minimum path: seven waves per SIMD at 256 threads (the LDS below, 22.3 KB at
+// 4 particles per thread, lets a CU hold seven blocks; the register limit costs a few bytes of spills, which the
 // count-only views repay), six at 128 and 512 threads, where the exact path's spills would cost more (measured).
+// The other paths keep the compiler's choice.
 template <int MK, int PPT, int OUT, int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS == 256 ? 7 : 6))) void k_particles(GpsFrame f, const float4 *__restrict__ view,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MK == 1 && OUT == 0 ? (BS == 256 ? 7 : 6) : 1)))
+void k_particles(GpsFrame f, const float4 *__restrict__ view,
                                                           const uint32_t *__restrict__ offsets,
                                                           const uint32_t *__restrict__ totals, uint32_t nens,
                                                           uint32_t first_seed, uint64_t *__restrict__ depth,
