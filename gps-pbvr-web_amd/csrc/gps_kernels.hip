@@ -972,10 +972,10 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
                                                           const uint32_t *__restrict__ perm) {
     constexpr uint32_t CH = BS * PPT;
     // PBVR packed-minimum path: a particle of a splat marked off-screen is settled by keep_draw when its draws
-    // alone decide; otherwise (splat, index within the splat | ensemble << 24) goes to its wave's buffer, which
-    // after each chunk runs count_only on whole waves of 64 entries.  Entries count_only cannot decide, and the
-    // buffer's last remainder, are queued for emit_pbvr, which the whole block runs once the queue may not hold
-    // another chunk's worth, and at the end.
+    // alone decide; otherwise (splat, particle seed) goes to its wave's buffer, which after each chunk runs
+    // count_only on whole waves of 64 entries.  Entries count_only cannot decide, and the buffer's last
+    // remainder, are queued for emit_pbvr, which the whole block runs once the queue may not hold another
+    // chunk's worth, and at the end.
     constexpr bool COUNT_ONLY = MK == 1 && OUT == 0;
     constexpr uint32_t QCAP = COUNT_ONLY ? CH + BS : 1u;   // a chunk appends at most CH; flushed above BS
     constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
@@ -1004,15 +1004,13 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
     const uint32_t nchunks = s_cstart[nens];
     const uint32_t N = f.N;
     uint64_t pts = 0;
+    // queued (splat, particle seed) entries, all of splats marked off-screen: emit_pbvr decides their count, and
+    // none of their particles can reach a subpixel (the box test that marked the splat), so nothing is written
     auto flush = [&](uint32_t nq) {
         for (uint32_t t = threadIdx.x; t < nq; t += BS) {
             const uint2 e = s_q[t];
-            const uint32_t k = e.y & 0xFFFFFFu, b = e.y >> 24;
-            Rng r(particle_seed_mixed(e.x, k + 1u, (first_seed + b) * 2654435761u + 1u));
-            const Hit h = emit_pbvr(f, r, view + (size_t)VIEW_VEC4 * e.x);
-            pts += h.counted;
-            if (h.idx != NO_HIT)
-                write_min(depth + (size_t)b * sub_count + h.idx, ((uint64_t)h.key << 32) | (uint64_t)e.x);
+            Rng r(e.y);
+            pts += emit_pbvr(f, r, view + (size_t)VIEW_VEC4 * e.x).counted;
         }
     };
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -1087,7 +1085,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
         for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the ring needs whole waves)
             const uint32_t p = pfirst + j;
             bool push = false;
-            uint32_t kb = 0;
+            uint32_t pseed = 0;
             if (p < pend) {
                 if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
                     do {
@@ -1108,7 +1106,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
                             pts += (uint32_t)c;
                         } else {
                             push = true;
-                            kb = k | (b << 24);
+                            pseed = seed;
                         }
                         done = true;
                     }
@@ -1133,7 +1131,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
                     const uint64_t m = __ballot(push);
                     if (push)
                         s_ring[wave][rn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
-                            make_uint2(sp, kb);
+                            make_uint2(sp, pseed);
                     rn += (uint32_t)__popcll(m);
                 }
             }
@@ -1147,8 +1145,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
                 for (; base + 64u <= rn; base += 64u) {
                     const uint2 e = s_ring[wave][base + lane];
                     load_view(e.x);
-                    const uint32_t k2 = e.y & 0xFFFFFFu, b2 = e.y >> 24;
-                    Rng r(particle_seed_mixed(e.x, k2 + 1u, (first_seed + b2) * 2654435761u + 1u));
+                    Rng r(e.y);
                     const int c = count_only(f, r, v);
                     if (c >= 0) pts += (uint32_t)c;
                     else s_q[atomicAdd(&s_qn, 1u)] = e;
