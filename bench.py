@@ -433,8 +433,17 @@ def main():
     for v in views:
         cam = S.clustered_camera(v, W, H)
         if not a.no_autotune:
-            t, tms = r.autotune(cam, p, trial_ensembles=4)
-            log(f"[autotune {v}] {t} {tms:.3f} ms/ens")
+            # trials of about 4 ms of GPU work each (4 to 64 ensembles): light views get enough work per trial
+            # for launch timing noise not to pick their tuning
+            r.reset_accum()
+            r.profile(True)
+            r.render_ensembles(cam, p, 1, 4)
+            r.wait_idle()
+            est = sum(t for _, t in r.profile_read().values()) / 4.0
+            r.profile(False)
+            trial = int(min(64, max(4, math.ceil(4.0 / max(est, 1e-3)))))
+            t, tms = r.autotune(cam, p, trial_ensembles=trial)
+            log(f"[autotune {v}] {trial} ensembles per trial: {t} {tms:.3f} ms/ens")
         tuning[v] = gh.get_tuning()
         seed = 1
         for _ in range(a.warmup):
