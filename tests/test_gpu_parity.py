@@ -812,3 +812,34 @@ def test_count_only_equals_full_path_560k(view, method, radial, jitter):
     assert stats_tuple(s0) == stats_tuple(s1)
     assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
     assert np.array_equal(d0, d1)
+
+
+@pytest.mark.parametrize("view", ["near", "mid"])
+def test_count_only_under_tunings(view):
+    """The keep-draw shortcut and the per-wave count_only buffer (64 x (PPT + 1) entries) under every particles-
+    per-thread value and workgroup size, with and without the splat order: identical to the path turned off."""
+    g, sh = S.clustered(560_000, seed=1, sh_degree=3)
+    cam = S.clustered_camera(view)
+    p = gh.RenderParams(spp_side=1, method=2, max_points_per_splat=16384,
+                        pbvr=dict(calibration=3, radial_correction=1, low_pass_jitter=1))
+    a0, d0, s0 = _render_with(0, g, sh, 3, cam, p, 1280, 720, 21, 2)
+    saved = gh.get_tuning()
+    try:
+        for waves, ppt, block, keep in ((8, 1, 128, False), (16, 2, 512, True), (64, 8, 256, False),
+                                        (256, 4, 256, True), (32, 4, 128, False)):
+            say(f"[count-only tunings {view}] waves={waves} ppt={ppt} block={block} keep={keep}")
+            gh.set_tuning(particle_waves_per_cu=waves, particles_per_thread=ppt, particle_block=block, count_only=1)
+            r = gh.GpsRenderer(1280, 720, 1)
+            r.set_gaussians_sh(g, sh, 3)
+            if keep:
+                r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+            r.render_ensembles(cam, p, 21, 2)
+            a1, s1 = r.read_accum(), r.read_stats()
+            d1 = r.read_depth_id() if keep else None
+            r.close()
+            assert stats_tuple(s1) == stats_tuple(s0)
+            assert np.array_equal(a1.view(np.uint32), a0.view(np.uint32))
+            if keep:
+                assert np.array_equal(d1, d0)
+    finally:
+        gh.set_tuning(**saved)
