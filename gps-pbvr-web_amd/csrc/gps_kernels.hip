@@ -962,14 +962,11 @@ GPS_DEV int keep_draw(const GpsFrame &f, uint32_t seed, const float4 *__restrict
 // The other paths keep the compiler's choice.
 template <int MK, int PPT, int OUT, int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MK == 1 && OUT == 0 ? (BS == 256 ? 7 : 6) : 1)))
-void k_particles(GpsFrame f, const float4 *__restrict__ view,
-                                                          const uint32_t *__restrict__ offsets,
-                                                          const uint32_t *__restrict__ totals, uint32_t nens,
-                                                          uint32_t first_seed, uint64_t *__restrict__ depth,
-                                                          uint64_t sub_count, const float *__restrict__ lut,
-                                                          uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys,
-                                                          uint32_t *__restrict__ pidx,
-                                                          const uint32_t *__restrict__ perm) {
+void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__restrict__ offsets,
+                 const uint32_t *__restrict__ totals, uint32_t nens, uint32_t first_seed,
+                 uint64_t *__restrict__ depth, uint64_t sub_count, const float *__restrict__ lut,
+                 uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys, uint32_t *__restrict__ pidx,
+                 const uint32_t *__restrict__ perm) {
     constexpr uint32_t CH = BS * PPT;
     // PBVR packed-minimum path: a particle of a splat marked off-screen is settled by keep_draw when its draws
     // alone decide; otherwise (splat, particle seed) goes to its wave's buffer, which after each chunk runs
@@ -986,8 +983,8 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
     __shared__ uint32_t s_off[OCAP + 1], s_prm[OCAP];
     __shared__ uint64_t s_pts[BS / 64];
     __shared__ uint2 s_q[QCAP];
-    constexpr uint32_t RING = 64u * (PPT + 1u);   // per-wave count_only work: fewer than 64 left + PPT x 64 appended
-    __shared__ uint2 s_ring[COUNT_ONLY ? BS / 64 : 1][COUNT_ONLY ? RING : 1];
+    constexpr uint32_t WCAP = 64u * (PPT + 1u);   // per-wave count_only work: fewer than 64 left + PPT x 64 appended
+    __shared__ uint2 s_work[COUNT_ONLY ? BS / 64 : 1][COUNT_ONLY ? WCAP : 1];
     static_assert(PPT == 1 || PPT == 2 || PPT == 4 || PPT == 8, "particles per thread");
     static_assert(BS == 128 || BS == 256 || BS == 512, "workgroup size");
     const bool count_only_on = COUNT_ONLY && (f.flags & FLAG_FAST) != 0u;
@@ -1082,7 +1079,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
         };
         if (pfirst < pend) load_view(sp);
 #pragma unroll 1
-        for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the ring needs whole waves)
+        for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the work buffer needs whole waves)
             const uint32_t p = pfirst + j;
             bool push = false;
             uint32_t pseed = 0;
@@ -1129,9 +1126,11 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
             if constexpr (COUNT_ONLY) {
                 if (count_only_on) {   // every lane of the wave is here
                     const uint64_t m = __ballot(push);
-                    if (push)
-                        s_ring[wave][rn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
-                            make_uint2(sp, pseed);
+                    if (push) {
+                        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        s_work[wave][rn + rank] = make_uint2(sp, pseed);
+                    }
                     rn += (uint32_t)__popcll(m);
                 }
             }
@@ -1143,7 +1142,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 uint32_t base = 0;
                 for (; base + 64u <= rn; base += 64u) {
-                    const uint2 e = s_ring[wave][base + lane];
+                    const uint2 e = s_work[wave][base + lane];
                     load_view(e.x);
                     Rng r(e.y);
                     const int c = count_only(f, r, v);
@@ -1151,7 +1150,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
                     else s_q[atomicAdd(&s_qn, 1u)] = e;
                 }
                 rn -= base;   // fewer than 64 left, at [base, base + rn); base >= 64 > rn, so the copy does not overlap
-                if (lane < rn) s_ring[wave][lane] = s_ring[wave][base + lane];
+                if (lane < rn) s_work[wave][lane] = s_work[wave][base + lane];
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1168,11 +1167,11 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view,
         }
     }
     if (COUNT_ONLY) {
-        if (count_only_on && rn) {   // the ring's remainder (fewer than 64), one entry per lane
+        if (count_only_on && rn) {   // the work buffer's remainder (fewer than 64), one entry per lane
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane < rn) s_q[atomicAdd(&s_qn, 1u)] = s_ring[wave][lane];   // to emit_pbvr
+            if (lane < rn) s_q[atomicAdd(&s_qn, 1u)] = s_work[wave][lane];   // to emit_pbvr
         }
         __syncthreads();
         flush(s_qn);
