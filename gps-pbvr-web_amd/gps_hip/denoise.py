@@ -43,9 +43,9 @@ def mean_image(acc):
     return acc[..., :3] / np.maximum(acc[..., 3:4], 1.0)
 
 
-def random_view(rng, w=FULL_W, h=FULL_H):
-    """Orbit camera around the origin: distance log-uniform in [2, 15], yaw uniform, pitch in [-0.35, 0.7]."""
-    dist = math.exp(rng.uniform(math.log(2.0), math.log(15.0)))
+def random_view(rng, w=FULL_W, h=FULL_H, dist_range=(2.0, 15.0)):
+    """Orbit camera around the origin: distance log-uniform in dist_range, yaw uniform, pitch in [-0.35, 0.7]."""
+    dist = math.exp(rng.uniform(math.log(dist_range[0]), math.log(dist_range[1])))
     return orbit_camera((0.0, 0.0, 0.0), dist, rng.uniform(0.0, 2.0 * math.pi), rng.uniform(-0.35, 0.7),
                         (0.0, 0.0, 1.0), S.FOV_60, w, h)
 
@@ -62,7 +62,7 @@ def window_camera(cam, x0, y0):
 
 
 def build_pairs(d, seeds, targets, crop=128, in_ens=(4, 16, 64, 256, 768), tgt_ens=2048, n_splats=560_000, rng_seed=0,
-                log=print):
+                log=print, dist_range=(2.0, 15.0)):
     """Renders `targets` target windows spread over the scenes `seeds` and, per target, one input per count in
     `in_ens`; adds every (input, target) pair to the denoiser.  Returns the GPU-side render count."""
     rng = np.random.default_rng(rng_seed)
@@ -82,7 +82,7 @@ def build_pairs(d, seeds, targets, crop=128, in_ens=(4, 16, 64, 256, 768), tgt_e
         inp.set_gaussians_sh(g, sh, 3)
         tgt.set_gaussians_sh(g, sh, 3)
         for _ in range(per_scene[si]):
-            full = random_view(rng)
+            full = random_view(rng, dist_range=dist_range)
             cam = window_camera(full, int(rng.integers(0, FULL_W - crop + 1)), int(rng.integers(0, FULL_H - crop + 1)))
             tgt.reset_accum()
             tgt.render_ensembles(cam, p, 1_000_000 + 4096 * k, tgt_ens)
@@ -188,6 +188,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--init", default=None, help="start from this parameter file")
+    ap.add_argument("--dist", default="2,15", help="orbit distance range of the training views, lo,hi")
+    ap.add_argument("--rng-seed", type=int, default=0, help="seed of the view and window draws")
     a = ap.parse_args(argv)
     lo, hi = (int(x) for x in a.seeds.split("-"))
     log = lambda *m: print(*m, file=sys.stderr, flush=True)  # noqa: E731
@@ -195,8 +197,9 @@ def main(argv=None):
     if a.init:
         d.load(a.init)
     t0 = time.perf_counter()
+    dist = tuple(float(x) for x in a.dist.split(","))
     build_pairs(d, list(range(lo, hi + 1)), a.targets, a.crop, tuple(int(x) for x in a.in_ens.split(",")), a.tgt_ens,
-                a.splats, log=log)
+                a.splats, rng_seed=a.rng_seed, log=log, dist_range=dist)
     t1 = time.perf_counter()
     losses = train(d, a.steps, a.batch, a.lr, log=log)
     t2 = time.perf_counter()
