@@ -942,7 +942,7 @@ GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) 
 // without the arithmetic: 0 when the view-conditioned draw rejects; 1 when there is no radial correction, or
 // the keep draw is below -v[3].w, a lower bound of every particle's keep ratio; -1 when count_only must decide.
 // The draws follow emit_particle_pbvr's order (gps_splat.wgsl:490-532): two normal pairs, the view-conditioned
-// draw, the jitter pair (its first normal is the second pair's spare), the radial keep draw.
+// draw, a third pair for the jitter (jy is the second pair's spare, jx starts the third), the radial keep draw.
 GPS_DEV int keep_draw(const GpsFrame &f, uint32_t seed, const float4 *__restrict__ v) {
     uint32_t c = pcg(pcg(pcg(pcg(seed))));   // the two normal pairs
     if (f.method == M_VIEW_CONDITIONED) {
