@@ -1069,36 +1069,75 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
                 sp = perm ? perm[i] : i;
             }
         }
-        // the view record of splat vsp stays in registers while its particles run
-        float4 v[NV];
-        uint32_t vsp = 0xFFFFFFFFu;
-        auto load_view = [&](uint32_t s2) {
-            if (s2 != vsp) {
-#pragma unroll
-                for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * s2 + q];
-                vsp = s2;
+        // one particle's emit and output, the splat's view record in v
+        auto emit_out = [&](Rng &r, uint32_t p, uint32_t k, uint32_t s2, const float4 *v) {
+            Hit h;
+            if constexpr (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
+            else if constexpr (MK == 1) h = emit_pbvr(f, r, v);
+            else h = emit_occupancy(f, r, k, v);
+            pts += h.counted;
+            if (OUT == 0) {
+                if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)s2);
+            } else {
+                pkeys[p] = ((uint64_t)h.key << 32) | (uint64_t)p;
+                pidx[p] = h.idx;
             }
         };
-        if (pfirst < pend) load_view(sp);
+        if constexpr (!COUNT_ONLY) {
+            if (pfirst < pend) {
+                // the splat's view record stays in registers while its particles run (loaded when the splat
+                // changes, so its latency overlaps the seed hashing instead of stalling each particle)
+                float4 v[NV];
+#pragma unroll
+                for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
 #pragma unroll 1
-        for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the work buffer needs whole waves)
-            const uint32_t p = pfirst + j;
-            bool push = false;
-            uint32_t pseed = 0;
-            if (p < pend) {
-                if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
-                    do {
-                        i++;
-                        cur = nxt;
-                        nxt = staged ? s_off[i + 1u - lo] : ((i + 1u < N) ? offs[i + 1u] : total);
-                    } while (p >= nxt);
-                    sp = staged ? s_prm[i - lo] : (perm ? perm[i] : i);
+                for (uint32_t j = 0; j < PPT; j++) {
+                    const uint32_t p = pfirst + j;
+                    if (p >= pend) break;
+                    if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
+                        do {
+                            i++;
+                            cur = nxt;
+                            nxt = staged ? s_off[i + 1u - lo] : ((i + 1u < N) ? offs[i + 1u] : total);
+                        } while (p >= nxt);
+                        sp = staged ? s_prm[i - lo] : (perm ? perm[i] : i);
+#pragma unroll
+                        for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
+                    }
+                    const uint32_t k = p - cur;
+                    Rng r(particle_seed_mixed(sp, k + 1u, seed_mix));
+                    emit_out(r, p, k, sp, v);
                 }
-                load_view(sp);
-                const uint32_t k = p - cur;
-                const uint32_t seed = particle_seed_mixed(sp, k + 1u, seed_mix);
-                bool done = false;
-                if constexpr (COUNT_ONLY) {
+            }
+        } else {
+            // the view record of splat vsp stays in registers while its particles run
+            float4 v[NV];
+            uint32_t vsp = 0xFFFFFFFFu;
+            auto load_view = [&](uint32_t s2) {
+                if (s2 != vsp) {
+#pragma unroll
+                    for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * s2 + q];
+                    vsp = s2;
+                }
+            };
+            if (pfirst < pend) load_view(sp);
+#pragma unroll 1
+            for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the work buffer needs whole waves)
+                const uint32_t p = pfirst + j;
+                bool push = false;
+                uint32_t pseed = 0;
+                if (p < pend) {
+                    if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
+                        do {
+                            i++;
+                            cur = nxt;
+                            nxt = staged ? s_off[i + 1u - lo] : ((i + 1u < N) ? offs[i + 1u] : total);
+                        } while (p >= nxt);
+                        sp = staged ? s_prm[i - lo] : (perm ? perm[i] : i);
+                    }
+                    load_view(sp);
+                    const uint32_t k = p - cur;
+                    const uint32_t seed = particle_seed_mixed(sp, k + 1u, seed_mix);
                     if (count_only_on && v[3].w != 0.0f) {
                         const int c = v[3].w < 0.0f ? keep_draw(f, seed, v) : -1;
                         if (c >= 0) {
@@ -1107,25 +1146,11 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
                             push = true;
                             pseed = seed;
                         }
-                        done = true;
-                    }
-                }
-                if (!done) {
-                    Rng r(seed);
-                    Hit h;
-                    if constexpr (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
-                    else if constexpr (MK == 1) h = emit_pbvr(f, r, v);
-                    else h = emit_occupancy(f, r, k, v);
-                    pts += h.counted;
-                    if (OUT == 0) {
-                        if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)sp);
                     } else {
-                        pkeys[p] = ((uint64_t)h.key << 32) | (uint64_t)p;
-                        pidx[p] = h.idx;
+                        Rng r(seed);
+                        emit_out(r, p, k, sp, v);
                     }
                 }
-            }
-            if constexpr (COUNT_ONLY) {
                 if (count_only_on) {   // every lane of the wave is here
                     const uint64_t m = __ballot(push);
                     if (push) {
@@ -1136,8 +1161,6 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
                     rn += (uint32_t)__popcll(m);
                 }
             }
-        }
-        if constexpr (COUNT_ONLY) {
             if (count_only_on && rn >= 64u) {   // whole-wave rounds of count_only, then the rest to the front
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
