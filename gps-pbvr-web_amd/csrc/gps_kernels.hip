@@ -1192,6 +1192,9 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
         }
     }
     if (COUNT_ONLY) {
+        // the last chunk's flush may have ended with thread 0 resetting s_qn after its barrier: the remainder's
+        // appends below must follow that reset
+        __syncthreads();
         if (count_only_on && rn) {   // the work buffer's remainder (fewer than 64), one entry per lane
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

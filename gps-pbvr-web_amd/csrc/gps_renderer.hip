@@ -305,8 +305,8 @@ void pack_frame(const GpsRenderer *r, const GpsCamera *cam, const GpsRenderParam
     if (p->pbvr.low_pass_jitter) flags |= FLAG_JITTER;
     if (r->cull) flags |= FLAG_CULL;
     const bool pbvr = p->method == M_PROPORTIONAL || p->method == M_EXTINCTION || p->method == M_VIEW_CONDITIONED;
-    // the count-only queue packs a particle's index within its splat into 24 bits
-    if (g_tuning.count_only && pbvr && f.cap <= (1u << 24)) flags |= FLAG_FAST;
+    // the count-only work entries carry (splat, particle seed), so the path holds for every cap
+    if (g_tuning.count_only && pbvr) flags |= FLAG_FAST;
     f.flags = flags;
     f.density = p->density_scale;
     f.near_z = p->near;

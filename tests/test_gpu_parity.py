@@ -814,6 +814,23 @@ def test_count_only_equals_full_path_560k(view, method, radial, jitter):
     assert np.array_equal(d0, d1)
 
 
+def test_count_only_above_24_bit_cap():
+    """A per-splat cap above 2^24 keeps the count-only path (its entries carry the particle's seed, not its
+    index within the splat): the mid view at 640x360 and cap 2^26 is identical with the path on and off."""
+    g, sh = S.clustered(560_000, seed=1, sh_degree=3)
+    cam = S.clustered_camera("mid", 640, 360)
+    p = gh.RenderParams(spp_side=1, method=2, max_points_per_splat=1 << 26,
+                        pbvr=dict(calibration=3, radial_correction=1, low_pass_jitter=1))
+    a0, d0, s0 = _render_with(0, g, sh, 3, cam, p, 640, 360, 31, 1)
+    a1, d1, s1 = _render_with(1, g, sh, 3, cam, p, 640, 360, 31, 1)
+    say(f"[count-only cap 2^26] points {s0.points} / {s1.points}, truncated {s1.truncated_splats}, "
+        f"skipped {s1.skipped_ensembles}")
+    assert s1.skipped_ensembles == 0 and s1.points > 0
+    assert stats_tuple(s0) == stats_tuple(s1)
+    assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
+    assert np.array_equal(d0, d1)
+
+
 @pytest.mark.parametrize("view", ["near", "mid"])
 def test_count_only_under_tunings(view):
     """The keep-draw shortcut and the per-wave count_only buffer (64 x (PPT + 1) entries) under every particles-
