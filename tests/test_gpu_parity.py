@@ -831,6 +831,28 @@ def test_count_only_above_24_bit_cap():
     assert np.array_equal(d0, d1)
 
 
+def test_count_only_last_chunk_flush_keeps_remainders():
+    """Regression: after a block's last chunk flushes the emit_pbvr queue, the per-wave work buffers' remainders
+    must be appended after the queue counter's reset.  Few resident blocks (4 waves per CU) with one particle
+    per thread give every block many chunks and many queue flushes; over several seeds the mid view's counters
+    and images must equal those of the path turned off."""
+    g, sh = S.clustered(560_000, seed=1, sh_degree=3)
+    cam = S.clustered_camera("mid", 640, 360)
+    p = gh.RenderParams(spp_side=1, method=2, max_points_per_splat=16384,
+                        pbvr=dict(calibration=3, radial_correction=1, low_pass_jitter=1))
+    saved = gh.get_tuning()
+    try:
+        for seed in range(101, 109):
+            gh.set_tuning(particle_waves_per_cu=4, particles_per_thread=1, particle_block=256)
+            a0, _, s0 = _render_with(0, g, sh, 3, cam, p, 640, 360, seed, 2)
+            a1, _, s1 = _render_with(1, g, sh, 3, cam, p, 640, 360, seed, 2)
+            say(f"[count-only last-chunk flush] seed {seed}: points {s0.points} / {s1.points}")
+            assert stats_tuple(s0) == stats_tuple(s1)
+            assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
+    finally:
+        gh.set_tuning(**saved)
+
+
 @pytest.mark.parametrize("view", ["near", "mid"])
 def test_count_only_under_tunings(view):
     """The keep-draw shortcut and the per-wave count_only buffer (64 x (PPT + 1) entries) under every particles-
