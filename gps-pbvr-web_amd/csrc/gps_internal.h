@@ -30,7 +30,12 @@ enum : uint32_t { KIND_NONE = 0, KIND_POISSON = 1, KIND_FIXED = 2, KIND_UNDERCOV
 // device statistics slots (u64 each)
 enum : int {
     ST_POINTS = 0, ST_TRUNC = 1, ST_DROPPED = 2, ST_VISIBLE = 3, ST_ORPHAN = 4, ST_CAND = 5, ST_UNDER = 6,
-    ST_SKIPPED = 7, ST_ENSEMBLES = 8, ST_PROBE = 9, ST_WORDS = 16,
+    ST_SKIPPED = 7, ST_ENSEMBLES = 8, ST_PROBE = 9,
+    // high-water marks of the particle kernel's count-only LDS buffers since the last stats reset: the emit_pbvr
+    // queue's fill (capacity particle_block x (particles_per_thread + 1)) and a wave's count_only work buffer
+    // (capacity 64 x (particles_per_thread + 1))
+    ST_QPEAK = 10, ST_WPEAK = 11,
+    ST_WORDS = 16,
 };
 
 constexpr int VIEW_VEC4 = 6;          // float4 per splat in the view cache

@@ -982,6 +982,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
     constexpr uint32_t OCAP = 256u;   // chunk splat ranges up to this many positions are staged in LDS
     constexpr uint32_t CRUN = 8u;     // consecutive chunks per block turn
     __shared__ uint32_t s_lo, s_hi, s_qn;
+    __shared__ uint32_t s_qpeak, s_wpeak;   // high-water marks of s_qn and of a wave's rn (ST_QPEAK, ST_WPEAK)
     __shared__ uint32_t s_off[OCAP + 1], s_prm[OCAP];
     __shared__ uint64_t s_pts[BS / 64];
     __shared__ uint2 s_q[QCAP];
@@ -992,10 +993,12 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
     const bool count_only_on = COUNT_ONLY && (f.flags & FLAG_FAST) != 0u;
     if (threadIdx.x == 0) {
         s_qn = 0u;
+        s_qpeak = 0u;
+        s_wpeak = 0u;
         uint32_t acc = 0;
         for (uint32_t b = 0; b < nens; b++) {
             s_cstart[b] = acc;
-            acc += (totals[b] + CH - 1u) / CH;
+            acc += (uint32_t)(((uint64_t)totals[b] + (CH - 1u)) / CH);   // 64-bit: totals reach 2^32 - 1
         }
         s_cstart[nens] = acc;
     }
@@ -1022,7 +1025,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
         while (b + 1u < nens && s_cstart[b + 1u] <= chunk) b++;
         const uint32_t total = totals[b];
         const uint32_t p0 = (chunk - s_cstart[b]) * CH;
-        const uint32_t pend = min(p0 + CH, total);
+        const uint32_t pend = (uint32_t)min((uint64_t)p0 + CH, (uint64_t)total);   // p0 + CH may pass 2^32
         const uint32_t *offs = offsets + (size_t)b * N;
         uint64_t *dep = depth + (size_t)b * sub_count;
         const uint32_t seed_mix = (first_seed + b) * 2654435761u + 1u;
@@ -1048,10 +1051,13 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
             }
             __syncthreads();
         }
-        const uint32_t pfirst = p0 + threadIdx.x * PPT;
+        // this thread's particles are p0 + q for q in [q0, min(q0 + PPT, cnt)); the comparisons use q, which
+        // cannot wrap (p0 + q0 can, in the last chunk of an ensemble with more than 2^32 - CH particles)
+        const uint32_t cnt = pend - p0, q0 = threadIdx.x * PPT;
+        const uint32_t pfirst = p0 + q0;
         // position in the splat order of the offsets, its first particle, the next position's and the splat
         uint32_t i = 0, cur = 0, nxt = 0, sp = 0;
-        if (pfirst < pend) {
+        if (q0 < cnt) {
             if (staged) {
                 uint32_t a = 0, z = R - 1u;
                 while (a < z) {
@@ -1084,7 +1090,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
             }
         };
         if constexpr (!COUNT_ONLY) {
-            if (pfirst < pend) {
+            if (q0 < cnt) {
                 // the splat's view record stays in registers while its particles run (loaded when the splat
                 // changes, so its latency overlaps the seed hashing instead of stalling each particle)
                 float4 v[NV];
@@ -1092,8 +1098,8 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
                 for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * sp + q];
 #pragma unroll 1
                 for (uint32_t j = 0; j < PPT; j++) {
+                    if (q0 + j >= cnt) break;
                     const uint32_t p = pfirst + j;
-                    if (p >= pend) break;
                     if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
                         do {
                             i++;
@@ -1120,13 +1126,13 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
                     vsp = s2;
                 }
             };
-            if (pfirst < pend) load_view(sp);
+            if (q0 < cnt) load_view(sp);
 #pragma unroll 1
             for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the work buffer needs whole waves)
                 const uint32_t p = pfirst + j;
                 bool push = false;
                 uint32_t pseed = 0;
-                if (p < pend) {
+                if (q0 + j < cnt) {
                     if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
                         do {
                             i++;
@@ -1161,6 +1167,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
                     rn += (uint32_t)__popcll(m);
                 }
             }
+            if (count_only_on && lane == 0u) atomicMax(&s_wpeak, rn);
             if (count_only_on && rn >= 64u) {   // whole-wave rounds of count_only, then the rest to the front
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -1184,6 +1191,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
         __syncthreads();
         if (COUNT_ONLY) {
             const uint32_t nq = s_qn;   // the same for every thread: read after the barrier
+            if (threadIdx.x == 0 && nq > s_qpeak) s_qpeak = nq;
             if (nq > QCAP - CH) {       // the next chunk could overflow the queue
                 flush(nq);
                 __syncthreads();        // every thread has read s_qn; the next chunk appends after its first barrier
@@ -1202,6 +1210,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
             if (lane < rn) s_q[atomicAdd(&s_qn, 1u)] = s_work[wave][lane];   // to emit_pbvr
         }
         __syncthreads();
+        if (threadIdx.x == 0 && s_qn > s_qpeak) s_qpeak = s_qn;
         flush(s_qn);
     }
     if (MK != 0) {
@@ -1212,6 +1221,12 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
             uint64_t t = 0;
             for (int w = 0; w < BS / 64; w++) t += s_pts[w];
             if (t) atomicAdd((unsigned long long *)&stats[ST_POINTS], (unsigned long long)t);
+            if (COUNT_ONLY) {   // the buffers' high-water marks (an atomic only when above the recorded one)
+                if (s_qpeak > __hip_atomic_load(&stats[ST_QPEAK], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    atomicMax((unsigned long long *)&stats[ST_QPEAK], (unsigned long long)s_qpeak);
+                if (s_wpeak > __hip_atomic_load(&stats[ST_WPEAK], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    atomicMax((unsigned long long *)&stats[ST_WPEAK], (unsigned long long)s_wpeak);
+            }
         }
     }
 }

@@ -877,6 +877,18 @@ int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out) {
     return 0;
 }
 
+int gps_renderer_read_queue_peaks(GpsRenderer *r, uint32_t *out) {
+    uint64_t w[ST_WORDS];
+    HIP_TRY(r, hipMemcpyAsync(w, r->stats, sizeof w, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(r, hipStreamSynchronize(r->stream));
+    const uint32_t ppt = g_tuning.particles_per_thread;
+    out[0] = (uint32_t)w[ST_QPEAK];
+    out[1] = g_tuning.particle_block * (ppt + 1u);
+    out[2] = (uint32_t)w[ST_WPEAK];
+    out[3] = 64u * (ppt + 1u);
+    return 0;
+}
+
 int gps_renderer_read_particle_keys(GpsRenderer *r, uint64_t *keys, uint32_t *subpixels, uint64_t max,
                                     uint64_t *count) {
     if (count) *count = r->last_np;

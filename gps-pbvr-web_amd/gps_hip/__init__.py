@@ -174,6 +174,7 @@ _sig("gps_renderer_set_debug", C.c_int, [_R, u32])
 _sig("gps_renderer_read_counts", C.c_int, [_R, P(u32), P(u32), P(u32), P(u32)])
 _sig("gps_renderer_read_depth_id", C.c_int, [_R, P(u64)])
 _sig("gps_renderer_read_particle_keys", C.c_int, [_R, P(u64), P(u32), u64, P(u64)])
+_sig("gps_renderer_read_queue_peaks", C.c_int, [_R, P(u32)])
 _sig("gps_sort_pairs_u64", C.c_int, [C.c_int, P(u64), P(u32), u64])
 _sig("gps_tuning_get", None, [P(Tuning)])
 _sig("gps_tuning_set", C.c_int, [P(Tuning)])
@@ -225,7 +226,7 @@ EXPORTED = [
     "gps_renderer_composite", "gps_renderer_read_accum", "gps_renderer_read_stats", "gps_renderer_read_sorted",
     "gps_renderer_wait_idle", "gps_renderer_submitted", "gps_renderer_accumulated", "gps_renderer_info",
     "gps_estimate_memory", "gps_radius_lut", "gps_renderer_set_debug", "gps_renderer_read_counts",
-    "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
+    "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_renderer_read_queue_peaks", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
     "gps_renderer_profile", "gps_renderer_profile_read", "gps_tuning_get", "gps_tuning_set",
     "gps_renderer_autotune", "gps_batch_create", "gps_batch_destroy", "gps_batch_render", "gps_batch_read",
     "gps_batch_timing", "gps_renderer_set_features", "gps_renderer_read_features", "gps_renderer_set_cull",
@@ -414,6 +415,13 @@ class GpsRenderer:
         out = np.zeros(w * h * s * s, np.uint64)
         self._check(lib.gps_renderer_read_depth_id(self._h, _ptr(out, u64)))
         return out
+
+    def read_queue_peaks(self):
+        """High-water marks of the particle kernel's count-only LDS buffers since the last reset_stats, with their
+        capacities under the current tuning: (queue peak, queue capacity, work peak, work capacity)."""
+        out = (u32 * 4)()
+        self._check(lib.gps_renderer_read_queue_peaks(self._h, out))
+        return tuple(int(x) for x in out)
 
     def read_particle_keys(self, max_count=None):
         n = u64()

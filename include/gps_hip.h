@@ -162,6 +162,12 @@ int gps_renderer_read_counts(GpsRenderer *r, uint32_t *counts, uint32_t *offsets
  * the high and low halves are the reference's `depth` and `winner` buffers (gps_splat.wgsl:28-29) */
 int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out);
 
+/* High-water marks of the particle kernel's count-only LDS buffers since the last reset_stats (no counterpart in
+ * the reference; a test hook for the buffers' capacity invariants): out[0] the emit_pbvr queue's peak fill and
+ * out[1] its capacity under the current tuning (particle_block x (particles_per_thread + 1)); out[2] a wave's
+ * count_only work buffer's peak fill and out[3] its capacity (64 x (particles_per_thread + 1)). */
+int gps_renderer_read_queue_peaks(GpsRenderer *r, uint32_t *out);
+
 /* GPS_PATH_PARTICLE_SORTED: the sorted keys and subpixels (0xFFFFFFFF = off screen) of the last ensemble;
  * copies min(count, max) entries and stores the particle count in *count. */
 int gps_renderer_read_particle_keys(GpsRenderer *r, uint64_t *keys, uint32_t *subpixels, uint64_t max,

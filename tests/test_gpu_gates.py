@@ -171,3 +171,124 @@ def test_determinism_and_history():
     r.close()
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert not np.array_equal(a, c)
+
+
+# ------------------------------------------------------------------ phase 2: composite, one huge splat, 33.5M subpixels
+def _mirror_params(p):
+    import ctypes
+    m = O.MParams()
+    ctypes.memmove(ctypes.byref(m), ctypes.byref(p), ctypes.sizeof(O.MParams))
+    return m
+
+
+def _ocam(c):
+    import ctypes
+    o = O.OrcCamera()
+    ctypes.memmove(ctypes.byref(o), ctypes.byref(c), ctypes.sizeof(O.OrcCamera))
+    return o
+
+
+STAT_KEYS = ("points", "truncated_splats", "dropped_points", "visible_splats", "orphan_subpixels", "candidates",
+             "undercovered_splats", "skipped_ensembles", "ensembles")
+
+
+def _stats(st):
+    return tuple(int(getattr(st, k)) for k in STAT_KEYS)
+
+
+def test_phase2_composite_gates():
+    """verify.rs:225-291 on the GPU composite: scene 1 after 16 ensembles; the five probes of the same-size composite
+    and every pixel of the 2x upsampled one within 1.01 of the f64 single-OETF expectation (and within the 0.51 an
+    RGBA8 target's rounding to nearest allows), also with exposure and gamma off and at odd sizes."""
+    from composite_gate import REFERENCE_BOUND, STRICT, probe_error, worst_error
+    scene = to_gpu(phase_scenes()[1][1])
+    p = params()
+    r = gh.GpsRenderer(W, H, 2)
+    try:
+        r.set_gaussians(scene)
+        r.render_ensembles(CAM, p, 1, 16)
+        acc = r.read_accum()
+        assert (acc[..., 3] == 16.0).all()
+        same = r.composite(p, W, H)
+        say(f"[composite] probes {probe_error(same, acc):.3f}, all {worst_error(same, acc):.3f}")
+        assert probe_error(same, acc) <= REFERENCE_BOUND
+        assert worst_error(same, acc) <= STRICT
+        up = r.composite(p, 2 * W, 2 * H)
+        say(f"[upsample 2x] {worst_error(up, acc):.3f}")
+        assert worst_error(up, acc) <= REFERENCE_BOUND and worst_error(up, acc) <= STRICT
+        for ow, oh, gamma, expo in ((W, H, 0, 1.7), (50, 70, 1, 0.8), (3 * W, H + 1, 1, 1.2)):
+            img = r.composite(p.copy(srgb_output=gamma, exposure=expo), ow, oh)
+            assert worst_error(img, acc, expo, bool(gamma)) <= STRICT, (ow, oh, gamma, expo)
+            assert (img[..., 3] == 255).all()
+    finally:
+        r.close()
+
+
+def test_phase2_4m_particles_from_one_splat():
+    """verify.rs:192-199: one splat (log-scale ln 0.55, o = 0.9) at density 400 and cap 2^24 gives more than 4M particles
+    in one ensemble.  Particle path, 2 ensembles: >= 60 dB against the f64 Monte Carlo oracle with per-particle seeds,
+    points within 4 sigma, no orphans; and bit-exact against the f32 mirror (image, counters, and the last
+    ensemble's counts, offsets and depth|id buffer through the debug hook)."""
+    s = dict(phase_scenes()[2][1][0], log_scale=(math.log(0.55),) * 3)
+    scene = [s]
+    p = params(path=1, density_scale=400.0, max_points_per_splat=1 << 24)
+    gpu = {}
+    for keep in (False, True):
+        r = gh.GpsRenderer(W, H, 2)
+        try:
+            r.set_gaussians(to_gpu(scene))
+            if keep:
+                r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+            r.render_ensembles(CAM, p, 1, 2)
+            gpu[keep] = dict(accum=r.read_accum(), stats=r.read_stats())
+            if keep:
+                gpu[keep].update(r.read_counts(), depth_id=r.read_depth_id())
+        finally:
+            r.close()
+    st = gpu[False]["stats"]
+    say(f"[4M+ one splat] points {st.points}, truncated {st.truncated_splats}")
+    assert st.points > 4_000_000 and st.truncated_splats == 0 and st.orphan_subpixels == 0
+    img = to_image(gpu[False]["accum"])
+    opt = O.default_mc_options(per_particle_seed=1, density_scale=400.0, max_points_per_splat=1 << 24)
+    cpu, cst = O.render_monte_carlo(scene, OCAM, W, H, 2, 2, BG, 1, opt)
+    assert O.psnr(img, cpu) >= 60.0
+    assert within_4_sigma(st.points, cst.points)
+    m = O.mirror_render(to_gpu(scene), None, 0, OCAM, _mirror_params(p), W, H, 1, 2)
+    last = O.mirror_render(to_gpu(scene), None, 0, OCAM, _mirror_params(p), W, H, 2, 1, want_buffers=True)
+    for keep in (False, True):
+        assert _stats(gpu[keep]["stats"]) == tuple(int(x) for x in m["stats"][:9])
+        assert np.array_equal(gpu[keep]["accum"].view(np.uint32), m["accum"].view(np.uint32))
+    b = gpu[True]
+    assert np.array_equal(b["counts"], last["counts"]) and np.array_equal(b["offsets"], last["offsets"])
+    assert b["total"] == last["total"] > 4_000_000
+    assert np.array_equal(b["depth_id"], last["depth_id"])
+
+
+def test_phase2_2048x1024_spp16():
+    """verify.rs:294-322: 2048 x 1024 at spp side 4 (33.5M subpixels) with one splat and focal 900: the corners keep
+    the background, the centre is covered, no orphan subpixels; bit-exact against the f32 mirror, including the
+    depth|id buffer."""
+    w, h, s = 2048, 1024, 4
+    scene = to_gpu([phase_scenes()[1][1][0]])
+    cam = gh.default_camera(900.0)
+    p = gh.RenderParams(spp_side=s, background=BG)
+    r = gh.GpsRenderer(96, 96, 2)
+    try:
+        r.resize(w, h, s)
+        r.set_gaussians(scene)
+        r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+        r.reset_stats()
+        r.render_ensembles(cam, p, 1, 1)
+        acc = r.read_accum()
+        st = r.read_stats()
+        d = r.read_depth_id()
+    finally:
+        r.close()
+    say(f"[2048x1024 spp16] points {st.points}")
+    assert abs(acc[0, 0, 0] - BG[0]) < 1e-6 and abs(acc[-1, -1, 2] - BG[2]) < 1e-6
+    assert abs(acc[h // 2, w // 2, 0] - BG[0]) > 1e-3
+    assert st.orphan_subpixels == 0 and d.size == w * h * s * s
+    m = O.mirror_render(scene, None, 0, _ocam(cam), _mirror_params(p), w, h, 1, 1, want_buffers=True)
+    assert _stats(st) == tuple(int(x) for x in m["stats"][:9])
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+    assert np.array_equal(d, m["depth_id"])

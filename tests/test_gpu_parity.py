@@ -757,7 +757,7 @@ def test_trajectory_export_matches_single_renders():
 
 
 # ------------------------------------------------------------------ count-only path for off-screen splats
-def _render_with(count_only, scene, sh, deg, cam, p, w, h, seed, count):
+def _render_with(count_only, scene, sh, deg, cam, p, w, h, seed, count, peaks=False):
     old = gh.get_tuning()
     gh.set_tuning(count_only=count_only)
     try:
@@ -766,6 +766,8 @@ def _render_with(count_only, scene, sh, deg, cam, p, w, h, seed, count):
         r.set_debug(gh.DEBUG_KEEP_BUFFERS)
         r.render_ensembles(cam, p, seed, count)
         out = (r.read_accum(), r.read_depth_id(), r.read_stats())
+        if peaks:
+            out = out + (r.read_queue_peaks(),)
         r.close()
     finally:
         gh.set_tuning(**old)
@@ -845,8 +847,10 @@ def test_count_only_last_chunk_flush_keeps_remainders():
         for seed in range(101, 109):
             gh.set_tuning(particle_waves_per_cu=4, particles_per_thread=1, particle_block=256)
             a0, _, s0 = _render_with(0, g, sh, 3, cam, p, 640, 360, seed, 2)
-            a1, _, s1 = _render_with(1, g, sh, 3, cam, p, 640, 360, seed, 2)
-            say(f"[count-only last-chunk flush] seed {seed}: points {s0.points} / {s1.points}")
+            a1, _, s1, pk = _render_with(1, g, sh, 3, cam, p, 640, 360, seed, 2, peaks=True)
+            say(f"[count-only last-chunk flush] seed {seed}: points {s0.points} / {s1.points}, queue peak "
+                f"{pk[0]} / {pk[1]}, work peak {pk[2]} / {pk[3]}")
+            assert pk[0] <= pk[1] and pk[2] < pk[3] and pk[0] > 0
             assert stats_tuple(s0) == stats_tuple(s1)
             assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
     finally:
@@ -875,7 +879,10 @@ def test_count_only_under_tunings(view):
             r.render_ensembles(cam, p, 21, 2)
             a1, s1 = r.read_accum(), r.read_stats()
             d1 = r.read_depth_id() if keep else None
+            pk = r.read_queue_peaks()
             r.close()
+            say(f"    queue peak {pk[0]} / {pk[1]}, work peak {pk[2]} / {pk[3]}")
+            assert pk[0] <= pk[1] and pk[2] < pk[3] and pk[2] > 0
             assert stats_tuple(s1) == stats_tuple(s0)
             assert np.array_equal(a1.view(np.uint32), a0.view(np.uint32))
             if keep:
