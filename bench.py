@@ -33,18 +33,58 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, n, n_vis, sh_floats, particles, S, X, ens, launches):
+def kernel_bytes(name, n, sh_floats, hits, S, X, ens, launches):
     """Algorithmic HBM bytes of all `launches` launches of kernel `name` in the timed region (`ens` ensembles,
-    `particles` generated particles and `n_vis` visible splats per ensemble in total)."""
+    `hits` particles that reached a subpixel, over all of them)."""
     if name == "view":        # per render call: splat + SH read, 6 x float4 cache + colour + count info written
         return launches * n * (64 + 4 * sh_floats + 96 + 16 + 8)
     if name == "count_scan":  # per splat and ensemble: count info read, offset written
         return ens * n * (8 + 4)
-    if name == "particles":   # 8-byte read-modify-write per particle, the splat's cache per visible splat
-        return particles * 16 + ens * n_vis * 100
+    if name == "particles":   # 8-byte read-modify-write per particle that reaches a subpixel (the others write
+        # nothing); per launch the splats' 96-byte view records and splat order, per ensemble their offsets
+        return hits * 16 + launches * n * (96 + 4) + ens * n * 4
     if name == "resolve":     # depth|id read + reset, winner colour per subpixel; accumulation read + write
         return ens * S * (8 + 8 + 16) + launches * 32 * X
     return 0
+
+
+def particle_roofline(dname, dcount, dms, n, ens, particles, hits, Sx, view):
+    """The dominant kernel's roofline.  k_particles is bound by VALU issue: its rate is the lane-instructions per
+    generated particle that rocprofv3 counted (SQ_INSTS_VALU x 64, profiles/particles_valu.json, on this workload
+    and tuning) times this run's particles per launch, over this run's HIP-event launch time, against the fastest
+    VALU rate gps_valu_bench.hip reached (counted the same way).  Its HBM side: bytes per launch measured by
+    FETCH_SIZE / WRITE_SIZE (profiles/particles_traffic.json) and the algorithmic bytes (kernel_bytes), both over
+    the same launch time."""
+    avg_s = dms * 1e-3 / dcount
+    alg = kernel_bytes(dname, n, 45, hits, Sx, Sx, ens, dcount) / dcount
+    out = {"kernel": dname, "avg_launch_ms": round(dms / dcount, 4)}
+    vpath = os.path.join(ROOT, "profiles", "particles_valu.json")
+    tpath = os.path.join(ROOT, "profiles", "particles_traffic.json")
+    if dname != "particles" or not os.path.exists(vpath):
+        out.update(bound="hbm", achieved=round(alg / avg_s / 1e9, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                   frac=round(alg / avg_s / 1e9 / HBM_PEAK_GBS, 5), traffic=None)
+        return out
+    vp = json.load(open(vpath))
+    lane_per_particle = vp["views"][view]["lane_insts_per_particle"]
+    ppl = particles / dcount
+    achieved = lane_per_particle * ppl / avg_s / 1e12
+    peak = vp["ceiling"]["t_lane_insts_per_s"]
+    out.update(bound="valu", achieved=round(achieved, 3), peak=peak, unit="T VALU lane-instructions/s",
+               frac=round(achieved / peak, 4), traffic=None,
+               lane_insts_per_particle=lane_per_particle, particles_per_launch=int(ppl),
+               floor_lane_insts_per_particle=vp["floor_lane_insts_per_particle"],
+               source="profiles/particles_valu.json (SQ_INSTS_VALU per particle; ceiling from bench.py --mode valu)")
+    hbm = {"algorithmic_bytes_per_launch": int(alg), "algorithmic_GBps": round(alg / avg_s / 1e9, 2),
+           "peak": HBM_PEAK_GBS}
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if view in tr.get("views", {}):
+            t = tr["views"][view]["hbm_bytes_per_ensemble"] * ens / dcount
+            out["traffic"] = int(t)
+            hbm.update(measured_GBps=round(t / avg_s / 1e9, 2), frac=round(t / avg_s / 1e9 / HBM_PEAK_GBS, 5),
+                       source="profiles/particles_traffic.json (FETCH_SIZE, WRITE_SIZE, calibrated)")
+    out["hbm"] = hbm
+    return out
 
 
 def psnr(a, b):
@@ -105,6 +145,27 @@ def bench_atomics(a):
     print(json.dumps({"metric": "64-bit atomicMin throughput", "unit": "G atomics/s", "atomics_per_launch": n_atomics,
                       "launch": f"{blocks} x 256 threads x {per_thread} atomics", "reps": reps, "rows": rows}),
           flush=True)
+
+
+def bench_valu(a):
+    """VALU issue throughput (gps_valu_bench.hip): 8 independent chains per thread of f32 FMAs, of the particle
+    streams' PCG hash, and of its add/xor/shift part alone, at full occupancy (65,536 workgroups of 256 threads).
+    Run under rocprofv3 --pmc SQ_INSTS_VALU to count the lane-instructions of each launch; the kernel's rate against
+    the fastest of these is the particle kernel's VALU roofline."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "gps-pbvr-web_amd", "libgps_atomicbench.so"))
+    lib.gps_valu_bench.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_float)]
+    blocks, iters, reps = 65536, 256, 5
+    rows = []
+    for mode, name in ((0, "fma_f32"), (1, "pcg_hash"), (2, "add_xor_shift")):
+        ms = ctypes.c_float()
+        rc = lib.gps_valu_bench(mode, blocks, iters, reps, ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"gps_valu_bench({mode}) returned {rc}")
+        rows.append(dict(mode=name, ms=round(ms.value, 4), chain_steps_per_launch=blocks * 256 * 8 * iters))
+        log(f"[valu] {rows[-1]}")
+    print(json.dumps({"metric": "VALU issue throughput", "launch": f"{blocks} x 256 threads x 8 chains x {iters} steps",
+                      "reps": reps, "rows": rows}), flush=True)
 
 
 def bench_alpha(a):
@@ -378,7 +439,7 @@ def main():
     ap.add_argument("--size", default="480x270", help="bench-ply: WxH")
     ap.add_argument("--spp", type=int, default=1, help="bench-ply: SPP side")
     ap.add_argument("--mode", default="bench",
-                    choices=["bench", "calib", "atomics", "alpha", "sort3m", "batch8", "denoise", "bench-ply"],
+                    choices=["bench", "calib", "atomics", "valu", "alpha", "sort3m", "batch8", "denoise", "bench-ply"],
                     help="calib: one exclusive scan of 64 Mi u32 (reads and writes exactly 256 MiB), the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters; atomics: the 64-bit "
                          "atomicMin throughput microbenchmark")
@@ -406,6 +467,8 @@ def main():
 
     if a.mode == "atomics":
         return bench_atomics(a)
+    if a.mode == "valu":
+        return bench_valu(a)
     if a.mode == "alpha":
         return bench_alpha(a)
     if a.mode == "sort3m":
@@ -464,14 +527,16 @@ def main():
         prof = r.profile_read()
         r.profile(False)
         st = r.read_stats()
+        hits = r.read_particle_counters()["hits"]
         ens = a.steps * a.ensembles
         gpu_ms = sum(t for _, t in prof.values())
         particles = st.candidates - st.dropped_points
         per_view[v] = dict(ms_per_ensemble=gpu_ms / ens, wall_ms_per_ensemble=1000.0 * wall / ens,
                            points_per_ensemble=st.points / ens, particles_per_ensemble=particles / ens,
+                           onscreen_particles_per_ensemble=hits / ens,
                            candidates_per_ensemble=st.candidates / ens, truncated_per_ensemble=st.truncated_splats / ens,
                            skipped=st.skipped_ensembles, visible_splats=st.visible_splats / ens)
-        prof_view[v] = (prof, st, ens, particles)
+        prof_view[v] = (prof, st, ens, particles, hits)
         # the same renders with the conservative cull: bit-identical images, fewer generated particles
         if a.no_cull:
             log(f"[{v}] {per_view[v]}")
@@ -495,26 +560,14 @@ def main():
         log(f"[{v}] kernels: " + ", ".join(f"{k}={t / c:.3f}ms x{c}" for k, (c, t) in prof.items()))
 
     head = a.view
-    prof, st, ens, particles = prof_view[head]
-    n_vis = st.visible_splats / max(ens, 1)
+    prof, st, ens, particles, hits = prof_view[head]
     Sx = W * H
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     dname, (dcount, dms) = dom
-    dbytes = kernel_bytes(dname, n, n_vis, 45, particles, Sx, Sx, ens, dcount)
-    achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "avg_launch_ms": round(dms / dcount, 4), "algorithmic_bytes_per_launch": int(dbytes / dcount)}
-    # HBM bytes per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE, scaled by a known-size scan) on
-    # this workload, scaled to this run's ensembles per launch
-    tpath = os.path.join(ROOT, "profiles", "particles_traffic.json")
-    if dname == "particles" and head == "near" and os.path.exists(tpath):
-        tr = json.load(open(tpath))
-        roofline["traffic"] = int(tr["hbm_bytes_per_ensemble"] * ens / dcount)
-        roofline["traffic_source"] = "profiles/particles_traffic.json"
+    roofline = particle_roofline(dname, dcount, dms, n, ens, particles, hits, Sx, head)
     kernels = {}
     for k, (c, t) in prof.items():
-        b = kernel_bytes(k, n, n_vis, 45, particles, Sx, Sx, ens, c)
+        b = kernel_bytes(k, n, 45, hits, Sx, Sx, ens, c)
         kernels[k] = {"launches": c, "avg_ms": round(t / c, 4),
                       "GBps": round(b / (t * 1e-3) / 1e9, 1) if t > 0 and b else None}
 

@@ -35,6 +35,8 @@ enum : int {
     // queue's fill (capacity particle_block x (particles_per_thread + 1)) and a wave's count_only work buffer
     // (capacity 64 x (particles_per_thread + 1))
     ST_QPEAK = 10, ST_WPEAK = 11,
+    // particles of the particle kernel that reached a subpixel (the packed minimum or a sorted key on screen)
+    ST_HITS = 12,
     ST_WORDS = 16,
 };
 

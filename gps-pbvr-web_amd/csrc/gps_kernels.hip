@@ -550,7 +550,9 @@ __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restri
 }
 
 // ================================================================== decoupled look-back
-// status word: [63:38] epoch, [37:36] flag (1 aggregate, 2 inclusive prefix), [35:0] saturated value
+// status word: [63:38] epoch, [37:36] flag (1 aggregate, 2 inclusive prefix), [35:0] value: saturated at 2^36 - 1
+// for the particle counts (a total above 2^32 - 1 marks the ensemble as skipped), or modulo 2^32 (WRAP32) for the
+// plain u32 scan, whose output wraps like the reference's u32 arithmetic
 constexpr uint64_t ST_VAL_MASK = (1ull << 36) - 1ull;
 constexpr uint32_t ST_EPOCH_MASK = (1u << 26) - 1u;
 
@@ -559,10 +561,18 @@ GPS_DEV uint64_t sat_add(uint64_t a, uint64_t b) {
     return s > ST_VAL_MASK ? ST_VAL_MASK : s;
 }
 
+template <bool WRAP32>
+GPS_DEV uint64_t lb_add(uint64_t a, uint64_t b) {
+    if constexpr (WRAP32) return (a + b) & 0xFFFFFFFFull;
+    else return sat_add(a, b);
+}
+
 // Called by one thread per tile: publishes the aggregate and returns the exclusive prefix of `tile`.
+template <bool WRAP32 = false>
 GPS_DEV uint64_t lookback(uint64_t *__restrict__ status, uint32_t tile, uint64_t agg, uint32_t epoch) {
     const uint64_t tag = (uint64_t)(epoch & ST_EPOCH_MASK) << 38;
-    agg = agg > ST_VAL_MASK ? ST_VAL_MASK : agg;
+    if constexpr (WRAP32) agg &= 0xFFFFFFFFull;
+    else agg = agg > ST_VAL_MASK ? ST_VAL_MASK : agg;
     if (tile == 0u) {
         __hip_atomic_store(&status[0], tag | (2ull << 36) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return 0ull;
@@ -577,11 +587,11 @@ GPS_DEV uint64_t lookback(uint64_t *__restrict__ status, uint32_t tile, uint64_t
             if ((uint32_t)(w >> 38) == (epoch & ST_EPOCH_MASK) && ((w >> 36) & 3ull) != 0ull) break;
             __builtin_amdgcn_s_sleep(1);
         }
-        excl = sat_add(excl, w & ST_VAL_MASK);
+        excl = lb_add<WRAP32>(excl, w & ST_VAL_MASK);
         if (((w >> 36) & 3ull) == 2ull) break;
         t--;
     }
-    __hip_atomic_store(&status[tile], tag | (2ull << 36) | sat_add(excl, agg), __ATOMIC_RELAXED,
+    __hip_atomic_store(&status[tile], tag | (2ull << 36) | lb_add<WRAP32>(excl, agg), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
     return excl;
 }
@@ -984,7 +994,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
     __shared__ uint32_t s_lo, s_hi, s_qn;
     __shared__ uint32_t s_qpeak, s_wpeak;   // high-water marks of s_qn and of a wave's rn (ST_QPEAK, ST_WPEAK)
     __shared__ uint32_t s_off[OCAP + 1], s_prm[OCAP];
-    __shared__ uint64_t s_pts[BS / 64];
+    __shared__ uint64_t s_pts[BS / 64], s_hits[BS / 64];
     __shared__ uint2 s_q[QCAP];
     constexpr uint32_t WCAP = 64u * (PPT + 1u);   // per-wave count_only work: fewer than 64 left + PPT x 64 appended
     __shared__ uint2 s_work[COUNT_ONLY ? BS / 64 : 1][COUNT_ONLY ? WCAP : 1];
@@ -1005,6 +1015,9 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
     __syncthreads();
     const uint32_t nchunks = s_cstart[nens];
     const uint32_t N = f.N;
+    // generated points (PBVR / occupancy statistic) in the low 32 bits, particles that reach the packed minimum or
+    // write a subpixel in the high 32 bits; a thread's share of a launch stays below 2^32 particles (at most
+    // 64 ensembles of fewer than 2^32 particles over at least 32,768 threads)
     uint64_t pts = 0;
     // queued (splat, particle seed) entries, all of splats marked off-screen: emit_pbvr decides their count, and
     // none of their particles can reach a subpixel (the box test that marked the splat), so nothing is written
@@ -1081,7 +1094,7 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
             if constexpr (MK == 0) h = emit_gps(f, r, v[0], v[1], lut);
             else if constexpr (MK == 1) h = emit_pbvr(f, r, v);
             else h = emit_occupancy(f, r, k, v);
-            pts += h.counted;
+            pts += (uint64_t)h.counted + ((uint64_t)(h.idx != NO_HIT) << 32);
             if (OUT == 0) {
                 if (h.idx != NO_HIT) write_min(dep + h.idx, ((uint64_t)h.key << 32) | (uint64_t)s2);
             } else {
@@ -1213,14 +1226,21 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
         if (threadIdx.x == 0 && s_qn > s_qpeak) s_qpeak = s_qn;
         flush(s_qn);
     }
-    if (MK != 0) {
-        uint64_t s = wave_sum(pts);
-        if ((threadIdx.x & 63) == 0) s_pts[threadIdx.x >> 6] = s;
+    {
+        const uint64_t sp = wave_sum(pts & 0xFFFFFFFFull), sh = wave_sum(pts >> 32);
+        if ((threadIdx.x & 63) == 0) {
+            s_pts[threadIdx.x >> 6] = sp;
+            s_hits[threadIdx.x >> 6] = sh;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint64_t t = 0;
-            for (int w = 0; w < BS / 64; w++) t += s_pts[w];
-            if (t) atomicAdd((unsigned long long *)&stats[ST_POINTS], (unsigned long long)t);
+            uint64_t t = 0, th = 0;
+            for (int w = 0; w < BS / 64; w++) {
+                t += s_pts[w];
+                th += s_hits[w];
+            }
+            if (MK != 0 && t) atomicAdd((unsigned long long *)&stats[ST_POINTS], (unsigned long long)t);
+            if (th) atomicAdd((unsigned long long *)&stats[ST_HITS], (unsigned long long)th);
             if (COUNT_ONLY) {   // the buffers' high-water marks (an atomic only when above the recorded one)
                 if (s_qpeak > __hip_atomic_load(&stats[ST_QPEAK], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                     atomicMax((unsigned long long *)&stats[ST_QPEAK], (unsigned long long)s_qpeak);
@@ -1437,9 +1457,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_u32(const uint32_t *__restr
     }
     uint64_t texcl;
     uint64_t agg = block_exclusive_scan(tsum, texcl, s_wave);
-    if (threadIdx.x == 0) s_prefix = lookback(status, tile, agg, epoch);
+    if (threadIdx.x == 0) s_prefix = lookback<true>(status, tile, agg, epoch);
     __syncthreads();
-    uint64_t run = s_prefix + texcl;
+    uint64_t run = s_prefix + texcl;   // the output keeps the low 32 bits
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
         if (base + j < n) out[base + j] = (uint32_t)run;

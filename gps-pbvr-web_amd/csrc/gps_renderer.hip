@@ -148,6 +148,13 @@ struct GpsRenderer {
     bool features = false;
     bool cull = false;      // skip PBVR splats whose particles all miss the image (set_cull)
     uint64_t *stats = nullptr;
+    // non-blocking stats readback (request_stats / latest_stats / stats_issued): a pinned host copy of the
+    // counters, the event that marks its copy done, the issue number it carries, and the last delivered snapshot
+    uint64_t *stats_staging = nullptr;
+    hipEvent_t stats_event = nullptr;
+    bool stats_pending = false;
+    uint32_t stats_issued = 0, stats_pending_issue = 0;
+    GpsStats stats_latest{};
     uint32_t *composite_buf = nullptr;
     uint64_t composite_cap = 0;
     uint32_t accumulated = 0;
@@ -487,6 +494,9 @@ int gps_renderer_create(uint32_t width, uint32_t height, uint32_t spp_side, int 
     if (hipMemcpy(r->lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
         return bail("lut upload failed");
     if (hipMalloc(&r->stats, ST_WORDS * sizeof(uint64_t)) != hipSuccess) return bail("hipMalloc(stats) failed");
+    if (hipHostMalloc((void **)&r->stats_staging, ST_WORDS * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+        return bail("hipHostMalloc(stats staging) failed");
+    if (hipEventCreateWithFlags(&r->stats_event, hipEventDisableTiming) != hipSuccess) return bail("hipEventCreate");
     if (hipMemsetAsync(r->stats, 0, ST_WORDS * sizeof(uint64_t), r->stream) != hipSuccess) return bail("stats clear");
     if (hipMalloc(&r->bs.tickets, MAX_BATCH * sizeof(uint32_t)) != hipSuccess) return bail("hipMalloc failed");
     if (hipMalloc(&r->bs.totals, MAX_BATCH * sizeof(uint32_t)) != hipSuccess) return bail("hipMalloc failed");
@@ -531,6 +541,8 @@ void gps_renderer_destroy(GpsRenderer *r) {
     dfree(r->mom);
     dfree(r->zmin);
     dfree(r->stats);
+    if (r->stats_staging) (void)hipHostFree(r->stats_staging);
+    if (r->stats_event) (void)hipEventDestroy(r->stats_event);
     dfree(r->composite_buf);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
@@ -816,6 +828,33 @@ int gps_renderer_read_stats(GpsRenderer *r, GpsStats *out) {
     return 0;
 }
 
+// Delivers a finished non-blocking readback into stats_latest (no wait).
+static void poll_stats(GpsRenderer *r) {
+    if (!r->stats_pending || hipEventQuery(r->stats_event) != hipSuccess) return;
+    stats_from_words(r->stats_staging, &r->stats_latest);
+    r->stats_latest.issue = r->stats_pending_issue;
+    r->stats_pending = false;
+}
+
+int gps_renderer_request_stats(GpsRenderer *r) {
+    poll_stats(r);
+    if (r->stats_pending) return 0;   // one readback in flight at a time
+    HIP_TRY(r, hipMemcpyAsync(r->stats_staging, r->stats, ST_WORDS * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              r->stream));
+    HIP_TRY(r, hipEventRecord(r->stats_event, r->stream));
+    r->stats_pending = true;
+    r->stats_pending_issue = ++r->stats_issued;
+    return 0;
+}
+
+int gps_renderer_latest_stats(GpsRenderer *r, GpsStats *out) {
+    poll_stats(r);
+    *out = r->stats_latest;
+    return 0;
+}
+
+uint32_t gps_renderer_stats_issued(const GpsRenderer *r) { return r->stats_issued; }
+
 int gps_renderer_wait_idle(GpsRenderer *r) {
     HIP_TRY(r, hipStreamSynchronize(r->stream));
     return 0;
@@ -877,15 +916,16 @@ int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out) {
     return 0;
 }
 
-int gps_renderer_read_queue_peaks(GpsRenderer *r, uint32_t *out) {
+int gps_renderer_read_particle_counters(GpsRenderer *r, uint64_t *out) {
     uint64_t w[ST_WORDS];
     HIP_TRY(r, hipMemcpyAsync(w, r->stats, sizeof w, hipMemcpyDeviceToHost, r->stream));
     HIP_TRY(r, hipStreamSynchronize(r->stream));
-    const uint32_t ppt = g_tuning.particles_per_thread;
-    out[0] = (uint32_t)w[ST_QPEAK];
-    out[1] = g_tuning.particle_block * (ppt + 1u);
-    out[2] = (uint32_t)w[ST_WPEAK];
-    out[3] = 64u * (ppt + 1u);
+    const uint64_t ppt = g_tuning.particles_per_thread;
+    out[0] = w[ST_HITS];
+    out[1] = w[ST_QPEAK];
+    out[2] = g_tuning.particle_block * (ppt + 1u);
+    out[3] = w[ST_WPEAK];
+    out[4] = 64u * (ppt + 1u);
     return 0;
 }
 

@@ -165,6 +165,9 @@ _sig("gps_renderer_read_accum", C.c_int, [_R, P(f32)])
 _sig("gps_renderer_read_stats", C.c_int, [_R, P(GpuStats)])
 _sig("gps_renderer_read_sorted", C.c_int, [_R, P(u32), P(u32)])
 _sig("gps_renderer_wait_idle", C.c_int, [_R])
+_sig("gps_renderer_request_stats", C.c_int, [_R])
+_sig("gps_renderer_latest_stats", C.c_int, [_R, P(GpuStats)])
+_sig("gps_renderer_stats_issued", u32, [_R])
 _sig("gps_renderer_submitted", u32, [_R])
 _sig("gps_renderer_accumulated", u32, [_R])
 _sig("gps_renderer_info", None, [_R, P(u32), P(u32), P(u32), P(u64), P(u32)])
@@ -174,7 +177,7 @@ _sig("gps_renderer_set_debug", C.c_int, [_R, u32])
 _sig("gps_renderer_read_counts", C.c_int, [_R, P(u32), P(u32), P(u32), P(u32)])
 _sig("gps_renderer_read_depth_id", C.c_int, [_R, P(u64)])
 _sig("gps_renderer_read_particle_keys", C.c_int, [_R, P(u64), P(u32), u64, P(u64)])
-_sig("gps_renderer_read_queue_peaks", C.c_int, [_R, P(u32)])
+_sig("gps_renderer_read_particle_counters", C.c_int, [_R, P(u64)])
 _sig("gps_sort_pairs_u64", C.c_int, [C.c_int, P(u64), P(u32), u64])
 _sig("gps_tuning_get", None, [P(Tuning)])
 _sig("gps_tuning_set", C.c_int, [P(Tuning)])
@@ -224,9 +227,9 @@ EXPORTED = [
     "gps_renderer_set_gaussians", "gps_renderer_resize", "gps_renderer_reset_accum", "gps_renderer_reset_stats",
     "gps_renderer_render_ensembles", "gps_renderer_render_sorted", "gps_renderer_probe_demand",
     "gps_renderer_composite", "gps_renderer_read_accum", "gps_renderer_read_stats", "gps_renderer_read_sorted",
-    "gps_renderer_wait_idle", "gps_renderer_submitted", "gps_renderer_accumulated", "gps_renderer_info",
+    "gps_renderer_request_stats", "gps_renderer_latest_stats", "gps_renderer_stats_issued", "gps_renderer_wait_idle", "gps_renderer_submitted", "gps_renderer_accumulated", "gps_renderer_info",
     "gps_estimate_memory", "gps_radius_lut", "gps_renderer_set_debug", "gps_renderer_read_counts",
-    "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_renderer_read_queue_peaks", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
+    "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_renderer_read_particle_counters", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
     "gps_renderer_profile", "gps_renderer_profile_read", "gps_tuning_get", "gps_tuning_set",
     "gps_renderer_autotune", "gps_batch_create", "gps_batch_destroy", "gps_batch_render", "gps_batch_read",
     "gps_batch_timing", "gps_renderer_set_features", "gps_renderer_read_features", "gps_renderer_set_cull",
@@ -363,6 +366,20 @@ class GpsRenderer:
         self._check(lib.gps_renderer_read_stats(self._h, C.byref(s)))
         return s
 
+    def request_stats(self):
+        """gps.rs:1005-1027: non-blocking counter readback (no-op while one is in flight)."""
+        self._check(lib.gps_renderer_request_stats(self._h))
+
+    def latest_stats(self):
+        """gps.rs:1035-1038: the last delivered readback, its issue number in .issue."""
+        s = GpuStats()
+        self._check(lib.gps_renderer_latest_stats(self._h, C.byref(s)))
+        return s
+
+    def stats_issued(self):
+        """gps.rs:1029-1033"""
+        return int(lib.gps_renderer_stats_issued(self._h))
+
     def read_sorted(self):
         n = self.gaussian_count()
         k = np.zeros(max(n, 1), np.uint32)
@@ -416,12 +433,18 @@ class GpsRenderer:
         self._check(lib.gps_renderer_read_depth_id(self._h, _ptr(out, u64)))
         return out
 
+    def read_particle_counters(self):
+        """Particle kernel counters since the last reset_stats: particles that reached a subpixel (hits), and the
+        count-only LDS buffers' high-water marks with their capacities under the current tuning."""
+        out = (u64 * 5)()
+        self._check(lib.gps_renderer_read_particle_counters(self._h, out))
+        return dict(hits=int(out[0]), queue_peak=int(out[1]), queue_cap=int(out[2]), work_peak=int(out[3]),
+                    work_cap=int(out[4]))
+
     def read_queue_peaks(self):
-        """High-water marks of the particle kernel's count-only LDS buffers since the last reset_stats, with their
-        capacities under the current tuning: (queue peak, queue capacity, work peak, work capacity)."""
-        out = (u32 * 4)()
-        self._check(lib.gps_renderer_read_queue_peaks(self._h, out))
-        return tuple(int(x) for x in out)
+        """(queue peak, queue capacity, work peak, work capacity) of read_particle_counters."""
+        c = self.read_particle_counters()
+        return c["queue_peak"], c["queue_cap"], c["work_peak"], c["work_cap"]
 
     def read_particle_keys(self, max_count=None):
         n = u64()

@@ -14,8 +14,9 @@ without its canvas.
   60 ms (frame-time EMA) hand motion over to GPS particles. Only the GPS particle model uses the sorted path.
 
 `frame(now_ms)` is one animation tick. Where the reference presents to its canvas, this composites into
-`Viewer.image` (RGBA8, the output size). The reference reads its GPU counters back asynchronously and checks a
-readback's issue number; here every readback is synchronous, so a request is answered at once.
+`Viewer.image` (RGBA8, the output size). Counters are read back as the reference reads them: asynchronously, one
+readback in flight (`request_stats` / `latest_stats` / `stats_issued`), and a probe's demand is taken only from a
+readback issued after the probe.
 """
 import json
 import math
@@ -128,17 +129,6 @@ class Viewer:
         self.dirty, self.needs_present = True, True
         self.scene_info = ""
         self.image = None
-        self._stats, self._issue = None, 0
-
-    # ---- counters: a readback is synchronous here (the reference's is asynchronous, one in flight)
-    def _request_stats(self):
-        self._stats = self.renderer.read_stats()
-        self._issue += 1
-
-    def _latest_stats(self):
-        if self._stats is None:
-            self._request_stats()
-        return self._stats
 
     def invalidate(self):
         self.dirty = True
@@ -316,20 +306,19 @@ class Viewer:
         if self.dirty:
             r.reset_accum()
             r.reset_stats()
-            self._stats = None
             self.history_epoch = (self.history_epoch + 1) & U32
             self.frames_since_reset = 0
             self.next_seed = self.seed
             self.dirty = False
             self.needs_present = True
         if r.submitted() <= self.target + 8:
-            self._request_stats()
+            r.request_stats()
         acc = r.accumulated()
         # point budget: demand at density factor 1 from the counters of the current history window
         self.frames_since_reset = min(self.frames_since_reset + 1, U32)
         if (self.point_budget > 0.0 and not self.lod_moving and self.frames_since_reset >= 8
-                and self._latest_stats().ensembles >= 2):
-            st = self._latest_stats()
+                and r.latest_stats().ensembles >= 2):
+            st = r.latest_stats()
             demand = (st.points + st.dropped_points) / st.ensembles / self.auto_density
             if demand > 0.0:
                 want = float(np.float32(min(max(self.point_budget / demand, 0.1), 1.0)))
@@ -355,15 +344,16 @@ class Viewer:
                         r.probe_demand(cam, self._params(False))
                         self.probe_epoch = self.history_epoch
                         self.probe_wait = 0
-                        self.probe_issue = self._issue
+                        self.probe_issue = r.stats_issued()
                     except Exception:
                         self.sorted_ok = False
                         self.sorted_active = False
                 if self.probe_epoch == self.history_epoch:
-                    self._request_stats()
+                    r.request_stats()
                     self.probe_wait += 1
-                    if self._issue > self.probe_issue:
-                        demand = float(self._latest_stats().probe_demand)
+                    # a readback issued after the probe (one in flight before it still shows the old window)
+                    if r.latest_stats().issue > self.probe_issue:
+                        demand = float(r.latest_stats().probe_demand)
                         self.last_demand = int(demand)
                         keep = 0.5 if self.sorted_active else 1.0   # hysteresis near the threshold
                         self.sorted_active = demand > keep * self.demand_threshold
@@ -425,9 +415,9 @@ class Viewer:
 
     def stats_json(self):
         """lib.rs:724-760 without the memory estimate."""
-        self._request_stats()
-        s = self._latest_stats()
         r = self.renderer
+        r.request_stats()
+        s = r.latest_stats()
         names = {0: "GPS", 1: "PBVR Proportional", 2: "PBVR Extinction", 3: "PBVR ViewConditioned",
                  4: "Screen occupancy"}
         return json.dumps({

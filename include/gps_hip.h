@@ -135,6 +135,15 @@ int gps_renderer_read_accum(GpsRenderer *r, float *out);
 int gps_renderer_read_stats(GpsRenderer *r, GpsStats *out);
 /* read_sorted, gps.rs:958-963: n keys and n values of the last render_sorted */
 int gps_renderer_read_sorted(GpsRenderer *r, uint32_t *keys, uint32_t *vals);
+/* request_stats, gps.rs:1005-1027: starts a non-blocking readback of the counters on the renderer's stream, after
+ * the work enqueued so far; a no-op while one is in flight.  Each readback gets the next issue number. */
+int gps_renderer_request_stats(GpsRenderer *r);
+/* latest_stats, gps.rs:1035-1038: the most recent counters a readback delivered, with its issue number in
+ * `issue` (all zero before the first); delivers a finished readback first, never waits */
+int gps_renderer_latest_stats(GpsRenderer *r, GpsStats *out);
+/* stats_issued, gps.rs:1029-1033: readbacks issued so far.  A snapshot whose issue is greater than the value read
+ * right after some work was enqueued was copied after that work completed. */
+uint32_t gps_renderer_stats_issued(const GpsRenderer *r);
 /* wait_idle, gps.rs:1055-1057 */
 int gps_renderer_wait_idle(GpsRenderer *r);
 /* submitted / accumulated, gps.rs:669-677 */
@@ -162,11 +171,13 @@ int gps_renderer_read_counts(GpsRenderer *r, uint32_t *counts, uint32_t *offsets
  * the high and low halves are the reference's `depth` and `winner` buffers (gps_splat.wgsl:28-29) */
 int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out);
 
-/* High-water marks of the particle kernel's count-only LDS buffers since the last reset_stats (no counterpart in
- * the reference; a test hook for the buffers' capacity invariants): out[0] the emit_pbvr queue's peak fill and
- * out[1] its capacity under the current tuning (particle_block x (particles_per_thread + 1)); out[2] a wave's
- * count_only work buffer's peak fill and out[3] its capacity (64 x (particles_per_thread + 1)). */
-int gps_renderer_read_queue_peaks(GpsRenderer *r, uint32_t *out);
+/* Particle kernel counters since the last reset_stats (no counterpart in the reference; measurement and test
+ * hooks): out[0] the particles that reached a subpixel (on screen: a packed-minimum update or an on-screen sorted
+ * key), the HBM-writing part of the particle work; the high-water marks of the count-only LDS buffers against
+ * their capacities under the current tuning: out[1] the emit_pbvr queue's peak fill, out[2] its capacity
+ * (particle_block x (particles_per_thread + 1)), out[3] a wave's count_only work buffer's peak fill, out[4] its
+ * capacity (64 x (particles_per_thread + 1)). */
+int gps_renderer_read_particle_counters(GpsRenderer *r, uint64_t *out);
 
 /* GPS_PATH_PARTICLE_SORTED: the sorted keys and subpixels (0xFFFFFFFF = off screen) of the last ensemble;
  * copies min(count, max) entries and stores the particle count in *count. */

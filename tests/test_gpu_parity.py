@@ -105,6 +105,18 @@ def test_scan_exact(n):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("n", [2049, 3_000_017])
+def test_scan_wraps_modulo_2_32(n):
+    """Full-range u32 inputs: prefix sums pass 2^36 within a few tiles, and the output is the exclusive sum modulo
+    2^32 (u32 arithmetic, gps_scan.wgsl:11-36)."""
+    say(f"[scan wrap] n={n}")
+    x = np.random.default_rng(n).integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    got = gh.scan_exclusive_u32(x)
+    ref = np.concatenate([np.zeros(1, np.uint64), np.cumsum(x[:-1], dtype=np.uint64)]) & np.uint64(0xFFFFFFFF)
+    assert int(np.sum(x, dtype=np.uint64)) > (1 << 36) or n < 100_000
+    assert np.array_equal(got, ref.astype(np.uint32))
+
+
 # ------------------------------------------------------------------ render_ensembles vs the mirror
 W = H = 96
 
@@ -816,21 +828,39 @@ def test_count_only_equals_full_path_560k(view, method, radial, jitter):
     assert np.array_equal(d0, d1)
 
 
-def test_count_only_above_24_bit_cap():
-    """A per-splat cap above 2^24 keeps the count-only path (its entries carry the particle's seed, not its
-    index within the splat): the mid view at 640x360 and cap 2^26 is identical with the path on and off."""
-    g, sh = S.clustered(560_000, seed=1, sh_degree=3)
-    cam = S.clustered_camera("mid", 640, 360)
-    p = gh.RenderParams(spp_side=1, method=2, max_points_per_splat=1 << 26,
-                        pbvr=dict(calibration=3, radial_correction=1, low_pass_jitter=1))
-    a0, d0, s0 = _render_with(0, g, sh, 3, cam, p, 640, 360, 31, 1)
-    a1, d1, s1 = _render_with(1, g, sh, 3, cam, p, 640, 360, 31, 1)
-    say(f"[count-only cap 2^26] points {s0.points} / {s1.points}, truncated {s1.truncated_splats}, "
-        f"skipped {s1.skipped_ensembles}")
-    assert s1.skipped_ensembles == 0 and s1.points > 0
-    assert stats_tuple(s0) == stats_tuple(s1)
-    assert np.array_equal(a0.view(np.uint32), a1.view(np.uint32))
-    assert np.array_equal(d0, d1)
+@pytest.mark.parametrize("cap_log2", [26, 24])
+def test_count_only_above_24_bit_cap(cap_log2):
+    """A splat beside the camera (every particle off-screen, so on the count-only path) with about 20M particles:
+    particle indices beyond 2^24 within one splat (its entries carry the particle's seed, not its index), next to an
+    on-screen splat.  At cap 2^26 nothing is truncated; at cap 2^24 the big splat is.  Image, counters, per-splat
+    counts and offsets equal the oracle's, and the count-only buffers were used."""
+    scene = to_gpu([dict(pos=(10.0, 0.0, 3.0), log_scale=(math.log(0.5),) * 3, opacity=0.6, color=(0.2, 0.5, 0.9),
+                         rot_xyzw=(0.0, 0.0, 0.0, 1.0)), phase_scenes()[1][1][0]])
+    p = gh.RenderParams(spp_side=1, method=2, max_points_per_splat=1 << cap_log2, density_scale=900.0,
+                        background=BG, pbvr=dict(calibration=3, radial_correction=1, low_pass_jitter=1))
+    old = gh.get_tuning()
+    gh.set_tuning(count_only=1)
+    try:
+        r = gh.GpsRenderer(W, H, 1)
+        r.set_gaussians(scene)
+        r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+        r.render_ensembles(CAM, p, 3, 1)
+        gpu = dict(accum=r.read_accum(), stats=r.read_stats(), bufs=r.read_counts())
+        gpu["bufs"]["depth_id"] = r.read_depth_id()
+        pk = r.read_queue_peaks()
+        r.close()
+    finally:
+        gh.set_tuning(**old)
+    st = gpu["stats"]
+    say(f"[count-only cap 2^{cap_log2}] counts {gpu['bufs']['counts']}, points {st.points}, truncated "
+        f"{st.truncated_splats}, work peak {pk[2]} / {pk[3]}")
+    m = O.mirror_render(scene, None, 0, ocam(CAM), mparams(p), W, H, 3, 1, want_buffers=True)
+    assert_exact(gpu, m)
+    assert pk[2] > 0 and pk[2] < pk[3] and pk[0] <= pk[1]
+    if cap_log2 == 26:
+        assert gpu["bufs"]["counts"][0] > (1 << 24) and st.truncated_splats == 0
+    else:
+        assert gpu["bufs"]["counts"][0] == 1 << 24 and st.truncated_splats == 1
 
 
 def test_count_only_last_chunk_flush_keeps_remainders():
@@ -889,3 +919,28 @@ def test_count_only_under_tunings(view):
                 assert np.array_equal(d1, d0)
     finally:
         gh.set_tuning(**saved)
+
+
+def test_async_stats_readback():
+    """gps.rs:1005-1038: request_stats starts a non-blocking readback (one in flight), latest_stats delivers the
+    last finished one with its issue number, stats_issued counts the readbacks; a snapshot whose issue is above the
+    count read right after some work was enqueued shows that work."""
+    p = gh.RenderParams(spp_side=2, background=BG, method=2, pbvr=dict(radial_correction=1))
+    r = gh.GpsRenderer(W, H, 2)
+    try:
+        r.set_gaussians(to_gpu(phase_scenes()[3][1]))
+        assert r.stats_issued() == 0 and r.latest_stats().issue == 0 and r.latest_stats().points == 0
+        for k in range(3):
+            r.render_ensembles(CAM, p, 1 + 4 * k, 4)
+            before = r.stats_issued()
+            r.request_stats()
+            r.request_stats()             # no-op while the first is in flight, or a second readback
+            assert before + 1 <= r.stats_issued() <= before + 2
+            r.wait_idle()
+            s = r.latest_stats()
+            ref = r.read_stats()
+            say(f"[async stats] round {k}: issue {s.issue} of {r.stats_issued()}, points {s.points}")
+            assert s.issue > before and s.issue <= r.stats_issued()
+            assert stats_tuple(s) == stats_tuple(ref) and s.ensembles == 4 * (k + 1)
+    finally:
+        r.close()

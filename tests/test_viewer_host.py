@@ -11,7 +11,9 @@ from gps_hip import viewer as V  # noqa: E402
 
 
 class FakeRenderer:
-    """Records calls; each rendered ensemble adds `demand` points (and `dropped`) to the counters."""
+    """Records calls; each rendered ensemble adds `demand` points (and `dropped`) to the counters.  Counter
+    readbacks are asynchronous like the renderer's (gps.rs:1005-1038): one in flight, delivered at the second poll
+    after its request."""
 
     def __init__(self, n=1000, demand=1e6, dropped=0.0, probe=0.0, skip_all=False):
         self.n, self.demand, self.dropped, self.probe, self.skip_all = n, demand, dropped, probe, skip_all
@@ -19,6 +21,8 @@ class FakeRenderer:
         self.acc = self.sub = 0
         self.points = self.drop = self.ens = 0
         self.probe_value = 0
+        self.issued, self.pending, self.ready, self.latest = 0, None, False, self._snapshot(0)
+        self.latest.ensembles = 0
 
     def reset_accum(self):
         self.calls.append(("reset_accum",))
@@ -45,11 +49,34 @@ class FakeRenderer:
     def set_gaussians(self, g):
         self.calls.append(("set_gaussians", len(g)))
 
-    def read_stats(self):
+    def _snapshot(self, issue):
         return types.SimpleNamespace(points=self.points, dropped_points=self.drop, ensembles=self.ens,
                                      probe_demand=self.probe_value, visible_splats=0, truncated_splats=0,
                                      orphan_subpixels=0, candidates=0, undercovered_splats=0,
-                                     skipped_ensembles=self.ens if self.skip_all else 0)
+                                     skipped_ensembles=self.ens if self.skip_all else 0, issue=issue)
+
+    def read_stats(self):
+        return self._snapshot(0)
+
+    def _poll(self):
+        if self.pending is not None:
+            if self.ready:
+                self.latest, self.pending = self.pending, None
+            else:
+                self.ready = True
+
+    def request_stats(self):
+        self._poll()
+        if self.pending is None:
+            self.issued += 1
+            self.pending, self.ready = self._snapshot(self.issued), False
+
+    def latest_stats(self):
+        self._poll()
+        return self.latest
+
+    def stats_issued(self):
+        return self.issued
 
     def render_ensembles(self, cam, p, seed, n):
         self.calls.append(("render", seed, n, float(p.density_scale), int(p.max_points_per_splat)))
@@ -160,6 +187,11 @@ def test_adaptive_lod_one_while_moving_with_the_moving_cap_then_grows_when_still
     t += 300.0                                        # still again: the cap returns, so the history restarts
     v.frame(t)
     assert ("reset_accum",) in r.calls[n_before:]
+    for _ in range(3):   # Auto path: the still view's demand probe is read back asynchronously (a frame or two)
+        if renders(r)[-1][4] == 16384:
+            break
+        t += 16.7
+        v.frame(t)
     assert renders(r)[-1][4] == 16384
     for _ in range(200):                              # fast frames: one more ensemble per 30 quiet frames
         t += 16.7
@@ -200,20 +232,35 @@ def test_point_budget_factor_is_clamped_at_a_tenth():
 
 
 # ------------------------------------------------------------------ render path
+def until_decided(v, t, dt=16.7):
+    """Frames until the Auto path has decided the current history window: the probe's demand arrives with a
+    readback issued after the probe, a frame or two later (lib.rs:578-600)."""
+    v.frame(t)
+    for _ in range(3):
+        if v.decided_epoch == v.history_epoch:
+            break
+        t += dt
+        v.frame(t)
+    assert v.decided_epoch == v.history_epoch
+    return t
+
+
 def test_auto_path_takes_the_sorted_path_above_the_threshold_with_hysteresis():
     r = FakeRenderer(probe=50e6)
     v = V.Viewer(64, 48, renderer=r)
     v.configure(64, 48, 1, 1.0, per_frame=1, target=8, max_points=16384)
     v.set_render_path(2, 30.0)
     v.frame(16.7)
+    assert not renders(r) and ("probe",) in r.calls   # waiting for the probe's readback: nothing rendered yet
+    t = until_decided(v, 33.4)
     assert v.sorted_active and ("sorted", 8) in r.calls
     r.probe = 20e6                                    # above half the threshold: stays sorted
     v.reset()
-    v.frame(33.4)
+    t = until_decided(v, t + 16.7)
     assert v.sorted_active
     r.probe = 10e6                                    # below half: back to GPS particles
     v.reset()
-    v.frame(50.1)
+    until_decided(v, t + 16.7)
     assert not v.sorted_active and renders(r)
 
 
@@ -230,8 +277,7 @@ def test_slow_sorted_motion_hands_over_to_gps_particles():
     r = FakeRenderer(probe=50e6)
     v = V.Viewer(64, 48, renderer=r)
     v.configure(64, 48, 1, 1.0, per_frame=1, target=100000, max_points=16384)
-    t = 16.7
-    v.frame(t)
+    t = until_decided(v, 16.7)
     assert v.sorted_active
     for _ in range(40):                               # moving, 100 ms frames
         v.orbit(1.0, 0.0)
