@@ -37,11 +37,23 @@ def test_auto_path_probes_and_the_sorted_path_renders():
         v.load_demo()
         v.configure(640, 480, 2, 1.0, per_frame=4, target=8, max_points=16384)
         v.set_render_path(2, 0.1)          # a threshold of 0.1M points: the demo's demand (2.5M) is above it
-        v.frame(16.7)
+        # the probe's demand arrives with a readback issued after it (asynchronous, gps.rs:1005-1038): the GPU
+        # idles between frames, as a frame's time would let it, and the first or second frame decides
+        t = 16.7
+        for _ in range(4):
+            v.frame(t)
+            t += 16.7
+            v.renderer.wait_idle()
+            if v.decided_epoch == v.history_epoch:
+                break
         assert v.last_demand > 0 and v.sorted_active and v.accumulated() == 8
         v.set_render_path(2, 100000.0)     # far above it: GPS particles
-        for k in range(3):
-            v.frame(33.4 + 16.7 * k)
+        for _ in range(6):
+            v.renderer.wait_idle()
+            v.frame(t)
+            t += 16.7
+            if v.accumulated() == 8 and not v.sorted_active:
+                break
         assert not v.sorted_active and v.accumulated() == 8
     finally:
         v.renderer.close()
