@@ -217,6 +217,7 @@ _sig("gps_batch_composite", C.c_int, [C.c_void_p, u32, C.c_void_p, u32, u32, P(C
 _sig("gps_batch_timing", None, [C.c_void_p, P(f64), P(f64), P(u64)])
 _sig("gps_phase0_atomic", C.c_int, [C.c_int, u32, u32, u32, P(u32), P(u32)])
 _sig("gps_scan_exclusive_u32", C.c_int, [C.c_int, P(u32), P(u32), u64])
+_sig("gps_scan_exclusive_u32_noalloc", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, u64, C.c_void_p])
 _sig("gps_math_eval", C.c_int, [C.c_int, C.c_int, P(f32), P(f32), P(f32), u64])
 _sig("gps_renderer_profile", C.c_int, [_R, C.c_int])
 _sig("gps_renderer_profile_read", C.c_int, [_R, P(KernelTime), C.c_int])
@@ -229,7 +230,8 @@ EXPORTED = [
     "gps_renderer_composite", "gps_renderer_read_accum", "gps_renderer_read_stats", "gps_renderer_read_sorted",
     "gps_renderer_request_stats", "gps_renderer_latest_stats", "gps_renderer_stats_issued", "gps_renderer_wait_idle", "gps_renderer_submitted", "gps_renderer_accumulated", "gps_renderer_info",
     "gps_estimate_memory", "gps_radius_lut", "gps_renderer_set_debug", "gps_renderer_read_counts",
-    "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_renderer_read_particle_counters", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32", "gps_math_eval",
+    "gps_renderer_read_depth_id", "gps_renderer_read_particle_keys", "gps_renderer_read_particle_counters", "gps_sort_pairs_u64", "gps_phase0_atomic", "gps_scan_exclusive_u32",
+    "gps_scan_exclusive_u32_noalloc", "gps_math_eval",
     "gps_renderer_profile", "gps_renderer_profile_read", "gps_tuning_get", "gps_tuning_set",
     "gps_renderer_autotune", "gps_batch_create", "gps_batch_destroy", "gps_batch_render", "gps_batch_read",
     "gps_batch_timing", "gps_renderer_set_features", "gps_renderer_read_features", "gps_renderer_set_cull",
@@ -745,6 +747,11 @@ def phase0_atomic(seed, pixels, samples, device=0):
     if lib.gps_phase0_atomic(device, seed, pixels, samples, _ptr(d, u32), _ptr(c, u32)) != 0:
         raise GpsError("phase-0 atomic kernel failed")
     return d, c
+
+
+# gps_scan_exclusive_u32_noalloc's limit and its error code (include/gps_hip.h)
+SCAN_NOALLOC_MAX_N = 1 << 28
+SCAN_NOALLOC_TOO_LARGE = -2
 
 
 def scan_exclusive_u32(x, device=0):

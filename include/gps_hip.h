@@ -194,6 +194,17 @@ int gps_sort_pairs_u64(int device, uint64_t *keys, uint32_t *vals, uint64_t n);
 int gps_phase0_atomic(int device, uint32_t seed, uint32_t pixels, uint32_t samples, uint32_t *depth,
                       uint32_t *color);
 int gps_scan_exclusive_u32(int device, const uint32_t *in, uint32_t *out, uint64_t n);
+/* Allocation-free variant of gps_scan_exclusive_u32 (no counterpart in the reference): for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  `in`
+ * and `out` are device buffers of n u32 on `device` (they may alias); the scan is enqueued on `stream` (a hipStream_t
+ * of that device, NULL for the null stream) and not waited for.  It calls no hipMalloc, hipFree or allocating
+ * routine: the look-back scratch (tile ticket and status words, 1 MiB + 256 B) is a file-scope device array that
+ * each GPU holds until the process ends, cleared by a fill kernel at every call.  Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (on different streams or host threads) race.  Makes `device`
+ * current.  n above GPS_SCAN_NOALLOC_MAX_N returns GPS_SCAN_NOALLOC_TOO_LARGE and enqueues nothing. */
+#define GPS_SCAN_NOALLOC_MAX_N (1ull << 28)
+enum { GPS_SCAN_NOALLOC_TOO_LARGE = -2 };
+int gps_scan_exclusive_u32_noalloc(int device, const uint32_t *in, uint32_t *out, uint64_t n, void *stream);
 /* Device evaluation of the kernels' f32 functions (0 exp, 1 log, 2 sin, 3 cos, 4 sqrt, 5 div x/y). */
 int gps_math_eval(int device, int fn, const float *x, const float *y, float *out, uint64_t n);
 
