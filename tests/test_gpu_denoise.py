@@ -176,3 +176,51 @@ def test_fallback_returns_plain_mean():
         d.apply(r2)
     d.close()
     r.close()
+
+
+def test_forward_at_720p_on_a_heldout_view_matches_reference():
+    """The config-5 shape: a clustered held-out scene (seed 200, 560k SH3) at 1280 x 720 from a random view, the
+    4-ensemble PBVR Extinction C3+R input, the shipped weights: features bit-exact and the forward within the
+    bf16 rounding bound of the PyTorch float32 reference, as at the small sizes."""
+    from gps_hip import denoise as D
+    g, sh = S.clustered(560_000, seed=200, sh_degree=3)
+    cam = D.random_view(np.random.default_rng(12345))
+    say("[denoise forward] 1280x720 held-out view")
+    r = gh.GpsRenderer(1280, 720, 1)
+    r.set_gaussians_sh(g, sh, 3)
+    r.set_features(True)
+    r.set_cull(True)
+    r.render_ensembles(cam, D.params_ext_c3r(), 1, 4)
+    d = gh.Denoiser()
+    d.load(D.WEIGHTS)
+    params = d.get_params()
+    d.apply(r, fallback_ratio=0.0)
+    x8, mean = d.read_features()
+    ex8, emean = R.features(r.read_accum(), *r.read_features(), 1)
+    assert np.array_equal(x8.view(np.uint32), ex8.view(np.uint32))
+    assert np.array_equal(mean.view(np.uint32), emean.view(np.uint32))
+    out = d.read()
+    ref = R.forward(ex8[None], emean[None], params)[0].detach().numpy()[0]
+    resid = ref - emean[..., :3]
+    err = np.abs(out[..., :3] - ref)
+    scale = float(np.abs(resid).max())
+    say(f"  residual scale {scale:.4g}, max err {err.max():.3g}, mean err {err.mean():.3g}")
+    assert scale > 1e-2
+    assert err.max() <= 2e-2 * scale and err.mean() <= 1e-3 * scale
+    d.close()
+    r.close()
+
+
+def test_heldout_error_is_bounded():
+    """Config 5's evaluation on two held-out scenes (seeds 200, 201): against the plain 1024-ensemble render the
+    denoised 4-ensemble image gains at least 8 dB over the plain 4-ensemble mean, its max abs error stays below
+    0.6 (0.32-0.43 measured on these views), and the fallback rate below 50%."""
+    from gps_hip import denoise as D
+    d = gh.Denoiser()
+    d.load(D.WEIGHTS)
+    rows = D.evaluate_heldout(d, seeds=range(200, 202), log=say)
+    d.close()
+    for row in rows:
+        assert row["psnr_denoised"] >= row["psnr_plain"] + 8.0, row
+        assert row["max_abs_denoised"] < 0.6 and row["max_abs_denoised"] < row["max_abs_plain"], row
+        assert row["fallback_rate"] < 0.5, row

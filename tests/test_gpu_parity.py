@@ -513,6 +513,54 @@ def test_sorted_particle_path_3m_splats_1080p():
     assert b["total"] > 10_000_000
 
 
+@pytest.mark.timeout(900)
+def test_config3_near_sorted_path_equals_packed_minimum_and_oracle():
+    """Config 3 at its own camera: 3,000,000 clustered Gaussians (seed 2, SH3), 1920x1080, SPP side 2, GPS, cap
+    65,536, near orbit camera; about 827M particles, so 827M 64-bit keys per ensemble.  The sorted keys are
+    unique and ascending, their particle indices are a permutation of 0..n-1, and the depth-sorted resolve equals
+    the packed-minimum path and the oracle bit for bit (depth|id buffer, image, counters)."""
+    g, sh = S.clustered(3_000_000, seed=2, sh_degree=3)
+    cam = S.clustered_camera("near", 1920, 1080)
+    p = gh.RenderParams(spp_side=2, method=0, max_points_per_splat=65536)
+    res = {}
+    for path in (2, 1):
+        say(f"[config3 near] path {path}")
+        r = gh.GpsRenderer(1920, 1080, 2)
+        try:
+            r.set_gaussians_sh(g, sh, 3)
+            r.set_debug(gh.DEBUG_KEEP_BUFFERS)
+            r.render_ensembles(cam, p.copy(path=path), 1, 1)
+            res[path] = dict(accum=r.read_accum(), depth=r.read_depth_id(), stats=r.read_stats(),
+                             total=r.read_counts()["total"])
+            if path == 2:
+                keys, _ = r.read_particle_keys()
+        finally:
+            r.close()
+    n = res[2]["total"]
+    say(f"[config3 near] {n} sorted keys")
+    assert n > 800_000_000 and keys.size == n
+    step = 1 << 26
+    for i in range(0, n - 1, step):                       # strictly ascending: unique keys
+        j = min(i + step, n - 1)
+        assert (keys[i + 1:j + 1] > keys[i:j]).all(), i
+    seen = np.zeros(n, np.bool_)                          # n indices below n, all present: a permutation
+    for i in range(0, n, step):
+        idx = (keys[i:i + step] & np.uint64(0xFFFFFFFF)).astype(np.int64)
+        assert idx.max() < n
+        seen[idx] = True
+    assert seen.all()
+    del keys, seen
+    a, b = res[1], res[2]
+    assert np.array_equal(a["depth"], b["depth"])
+    assert np.array_equal(a["accum"].view(np.uint32), b["accum"].view(np.uint32))
+    assert a["stats"] == b["stats"]
+    m = O.mirror_render(g, sh, 3, ocam(cam), mparams(p.copy(path=1)), 1920, 1080, 1, 1, want_buffers=True)
+    assert m["total"] == n
+    assert stats_tuple(a["stats"]) == mirror_stats_tuple(m["stats"])
+    assert np.array_equal(a["accum"].view(np.uint32), m["accum"].view(np.uint32))
+    assert np.array_equal(a["depth"], m["depth_id"])
+
+
 # ------------------------------------------------------------------ launch tuning never changes results
 def test_tunings_are_bit_identical():
     scene = S.small_uniform(4096, seed=0)
@@ -653,6 +701,64 @@ def test_batch_graph_equals_separate_renders():
         r.close()
 
 
+C4_SIZES = (20_000, 50_000, 100_000, 250_000, 560_000, 1_000_000, 1_500_000, 2_000_000)
+C4_RES = ((640, 360), (1280, 720), (1920, 1080))
+
+
+def c4_camera(v, w, h, views=16):
+    return gh.orbit_camera((0.0, 0.0, 0.0), 6.0, 2.0 * math.pi * v / views, math.radians(15.0), (0.0, 0.0, 1.0),
+                           S.FOV_60, w, h)
+
+
+@pytest.mark.timeout(1200)
+def test_config4_batch_graph_equals_separate_renders_and_oracle():
+    """Config 4: eight clustered scenes (seeds 10-17; 20k to 2M Gaussians, SH3), each with a 16-view orbit (radius 6,
+    elevation 15 degrees) at 640x360 / 1280x720 / 1920x1080 cycling per scene, PBVR Extinction C3+R, 32 ensembles
+    per view.  All 128 views in one hipGraph equal the same views rendered one call each (images and counters).
+    For every scene one view rendered at 640x360 (2 ensembles) equals the oracle."""
+    p = gh.RenderParams(spp_side=1, method=gh.Method.Extinction, max_points_per_splat=16384,
+                        pbvr=dict(calibration=3, radial_correction=1))
+    scenes, rends, items = [], [], []
+    for i, n in enumerate(C4_SIZES):
+        w, h = C4_RES[i % 3]
+        g, sh = S.clustered(n, seed=10 + i, sh_degree=3)
+        scenes.append((g, sh))
+        r = gh.GpsRenderer(w, h, 1)
+        r.set_gaussians_sh(g, sh, 3)
+        rends.append(r)
+        items += [(r, c4_camera(v, w, h), p, 1 + 1000 * v, 32) for v in range(16)]
+    try:
+        say(f"[config4] {len(items)} views in one graph")
+        b = gh.Batch()
+        b.render(items, use_graph=True)
+        t = b.timing()
+        say(f"[config4] graph: {t['graph_nodes']} nodes, {t['gpu_ms']:.1f} ms")
+        for k, (r, cam, q, seed, count) in enumerate(items):
+            r.reset_accum()
+            r.reset_stats()
+            r.render_ensembles(cam, q, seed, count)
+            assert np.array_equal(b.read(k).view(np.uint32), r.read_accum().view(np.uint32)), k
+            assert b.read_stats(k) == r.read_stats(), k
+        b.close()
+    finally:
+        for r in rends:
+            r.close()
+    for i, (g, sh) in enumerate(scenes):
+        v = (3 * i) % 16
+        cam = c4_camera(v, 640, 360)
+        say(f"[config4] scene {i} ({C4_SIZES[i]} Gaussians) view {v} at 640x360 against the oracle")
+        r = gh.GpsRenderer(640, 360, 1)
+        try:
+            r.set_gaussians_sh(g, sh, 3)
+            r.render_ensembles(cam, p, 1 + 1000 * v, 2)
+            acc, st = r.read_accum(), r.read_stats()
+        finally:
+            r.close()
+        m = O.mirror_render(g, sh, 3, ocam(cam), mparams(p), 640, 360, 1 + 1000 * v, 2)
+        assert stats_tuple(st) == mirror_stats_tuple(m["stats"]), i
+        assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32)), i
+
+
 # ------------------------------------------------------------------ denoiser features from the resolve
 @pytest.mark.parametrize("method,spp,path", [(0, 2, 1), (2, 1, 1), (0, 1, 0)])
 def test_features_bit_exact(method, spp, path):
@@ -757,13 +863,20 @@ def test_trajectory_export_matches_single_renders():
         k = T.cam_at(keys, i / 4)
         r.reset_accum()
         r.reset_stats()
-        r.render_ensembles(T.key_camera(k, 80, 60), p, 3, 4)
+        cam = T.key_camera(k, 80, 60)
+        r.render_ensembles(cam, p, 3, 4)
         want = T.png_rgba8(r.composite(p))
         assert z.read(f"frame_{i:04d}.png") == want, i
         st = r.read_stats()
         cols = dict(zip(["frame"] + T.CSV_COLS, csv[1 + i].split(",")))
         assert int(cols["points"]) == st.points and int(cols["candidates"]) == st.candidates
         assert int(cols["accumulated"]) == 4 and int(cols["visible"]) == st.visible_splats
+        # the frame against the oracle: its accumulation, counters and composited PNG
+        m = O.mirror_render(gg, g.sh_rest, g.sh_degree, ocam(cam), mparams(p), 80, 60, 3, 4)
+        assert np.array_equal(r.read_accum().view(np.uint32), m["accum"].view(np.uint32)), i
+        assert stats_tuple(st) == mirror_stats_tuple(m["stats"]), i
+        png = T.png_rgba8(O.mirror_composite(m["accum"], p.exposure, p.srgb_output, 80, 60))
+        assert z.read(f"frame_{i:04d}.png") == png, i
     assert t["graph_nodes"] > 5
     r.close()
 

@@ -27,6 +27,17 @@ def test_frames_accumulate_the_same_history_as_one_call():
         assert np.array_equal(acc.view(np.uint32), r.read_accum().view(np.uint32))
         assert np.array_equal(v.image, r.composite(v.params, 160, 120))
         r.close()
+        # the viewer's history and presented frame against the oracle
+        import ctypes
+        from oracle import oracle as O
+        m = O.MParams()
+        ctypes.memmove(ctypes.byref(m), ctypes.byref(v.params), ctypes.sizeof(O.MParams))
+        oc = O.OrcCamera()
+        ctypes.memmove(ctypes.byref(oc), ctypes.byref(v.cam.to_camera(160, 120)), ctypes.sizeof(O.OrcCamera))
+        ref = O.mirror_render(V.demo_gaussians(), None, 0, oc, m, 160, 120, 1, 12)
+        assert np.array_equal(acc.view(np.uint32), ref["accum"].view(np.uint32))
+        assert np.array_equal(v.image, O.mirror_composite(ref["accum"], v.params.exposure, v.params.srgb_output,
+                                                          160, 120))
     finally:
         v.renderer.close()
 
