@@ -149,7 +149,8 @@ def bench_atomics(a):
 
 def bench_valu(a):
     """VALU issue throughput (gps_valu_bench.hip): 8 independent chains per thread of f32 FMAs, of the particle
-    streams' PCG hash, and of its add/xor/shift part alone, at full occupancy (65,536 workgroups of 256 threads).
+    streams' PCG hash, of its add/xor/shift part alone, and of single instruction classes, at full occupancy
+    (65,536 workgroups of 256 threads).
     Run under rocprofv3 --pmc SQ_INSTS_VALU to count the lane-instructions of each launch; the kernel's rate against
     the fastest of these is the particle kernel's VALU roofline."""
     lib = ctypes.CDLL(os.path.join(ROOT, "gps-pbvr-web_amd", "libgps_atomicbench.so"))
@@ -157,7 +158,8 @@ def bench_valu(a):
                                    ctypes.POINTER(ctypes.c_float)]
     blocks, iters, reps = 65536, 256, 5
     rows = []
-    for mode, name in ((0, "fma_f32"), (1, "pcg_hash"), (2, "add_xor_shift")):
+    for mode, name in ((0, "fma_f32"), (1, "pcg_hash"), (2, "add_xor_shift"), (3, "add_u32"), (4, "xor_b32"),
+                       (5, "mul_lo_u32"), (6, "add_f32"), (7, "fma_f32+add_u32")):
         ms = ctypes.c_float()
         rc = lib.gps_valu_bench(mode, blocks, iters, reps, ctypes.byref(ms))
         if rc != 0:
@@ -426,7 +428,6 @@ def main():
     ap.add_argument("--view", default="near", choices=["near", "mid", "far"])
     ap.add_argument("--views", default="near,mid,far")
     ap.add_argument("--ensembles", type=int, default=64)
-    ap.add_argument("--cpu-fraction", type=int, default=32, help="CPU baseline uses 1/F of the splats")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-psnr", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
@@ -485,6 +486,8 @@ def main():
     n = len(g)
     p = gh.RenderParams(spp_side=1, method=gh.Method.Extinction, max_points_per_splat=16384,
                         pbvr=dict(calibration=3, radial_correction=1))
+    p_gps = gh.RenderParams(spp_side=1, method=gh.Method.Gps, max_points_per_splat=16384)
+    p_occ = gh.RenderParams(spp_side=1, method=gh.Method.ScreenOccupancy, max_points_per_splat=16384)
     r = gh.GpsRenderer(W, H, 1)
     r.set_gaussians_sh(g, sh, 3)
     views = [v for v in a.views.split(",") if v]
@@ -556,6 +559,23 @@ def main():
         r.profile(False)
         r.set_cull(False)
         per_view[v]["ms_per_ensemble_cull"] = sum(t for _, t in cprof.values()) / ens
+        # the reference's bench times GPS and screen occupancy on the same view (crates/gps-verify/src/main.rs:124-142)
+        for mname, q in (("gps", p_gps), ("occupancy", p_occ)):
+            r.reset_accum()
+            r.render_ensembles(cam, q, seed, a.ensembles)
+            r.wait_idle()
+            r.reset_stats()
+            r.profile(True)
+            for _ in range(a.steps):
+                r.reset_accum()
+                r.render_ensembles(cam, q, seed, a.ensembles)
+                seed += a.ensembles
+            r.wait_idle()
+            mprof = r.profile_read()
+            r.profile(False)
+            mst = r.read_stats()
+            per_view[v][f"{mname}_ms_per_ensemble"] = sum(t for _, t in mprof.values()) / ens
+            per_view[v][f"{mname}_points_per_ensemble"] = mst.points / ens
         log(f"[{v}] {per_view[v]}")
         log(f"[{v}] kernels: " + ", ".join(f"{k}={t / c:.3f}ms x{c}" for k, (c, t) in prof.items()))
 
@@ -629,57 +649,61 @@ def main():
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
-        frac = max(1, a.cpu_fraction)
-        m = max(1, n // frac)
-        cam = S.clustered_camera(head, W, H)
         mp = O.MParams()
         ctypes.memmove(ctypes.byref(mp), ctypes.byref(p), ctypes.sizeof(mp))
-        oc = O.OrcCamera()
-        ctypes.memmove(ctypes.byref(oc), ctypes.byref(cam), ctypes.sizeof(oc))
-        # one thread on 1/frac of the splats, then every host thread (OMP_NUM_THREADS) on 4x as many
-        O.lib.mir_set_threads(1)
-        t0 = time.perf_counter()
-        O.mirror_render(g[:m], sh[:m * 45], 3, oc, mp, W, H, 1, 1)
-        dt1 = time.perf_counter() - t0
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-        frac_mt = max(1, frac // 4)
-        m_mt = max(1, n // frac_mt)
-        O.lib.mir_set_threads(threads)
-        t0 = time.perf_counter()
-        O.mirror_render(g[:m_mt], sh[:m_mt * 45], 3, oc, mp, W, H, 1, 1)
-        dtn = time.perf_counter() - t0
+
+        def ocam(view):
+            oc = O.OrcCamera()
+            ctypes.memmove(ctypes.byref(oc), ctypes.byref(S.clustered_camera(view, W, H)), ctypes.sizeof(oc))
+            return oc
+
+        def mirror_ms(view, nthreads):
+            """the f32 oracle of the same pipeline on the whole scene, one ensemble (seed 1)"""
+            O.lib.mir_set_threads(nthreads)
+            t0 = time.perf_counter()
+            m = O.mirror_render(g, sh, 3, ocam(view), mp, W, H, 1, 1)
+            return 1000.0 * (time.perf_counter() - t0), m["stats"]
+
+        head_ms, head_st = mirror_ms(head, threads)
+        views_ms = {head: round(head_ms, 1)}
+        for v in views:
+            if v != head:
+                views_ms[v] = round(mirror_ms(v, threads)[0], 1)
+        far1_ms, _ = mirror_ms("far", 1)
         # the f64 port of the reference's own CPU particle renderer (oracle.rs:426-494), single-threaded like it:
-        # PBVR Extinction C3 with the radial correction and per-particle seeds, the same splats without their SH
-        # bands (it shades with the base colour) and without the per-splat cap, on 1/(16 frac) of the splats
-        frac64 = 16 * frac
-        m64 = max(1, n // frac64)
-        sub = g[:m64]
-        scene = [dict(pos=tuple(float(t) for t in x["pos_opacity"][:3]), log_scale=tuple(float(t) for t in x["log_scale"][:3]),
-                      rot_xyzw=(float(x["rot_wxyz"][1]), float(x["rot_wxyz"][2]), float(x["rot_wxyz"][3]),
-                                float(x["rot_wxyz"][0])),
-                      opacity=float(x["pos_opacity"][3]), color=tuple(float(t) for t in x["color"][:3])) for x in sub]
+        # PBVR Extinction C3 with the radial correction and per-particle seeds, base colour (it has no SH) and no
+        # per-splat cap (it has none), on the whole scene at the far view, where no splat reaches the cap
+        o64 = np.zeros((n, 14), np.float64)
+        o64[:, 0:3] = g["pos_opacity"][:, :3]
+        o64[:, 3:6] = g["log_scale"][:, :3]
+        o64[:, 6:9] = g["rot_wxyz"][:, 1:4]
+        o64[:, 9] = g["rot_wxyz"][:, 0]
+        o64[:, 10] = g["pos_opacity"][:, 3]
+        o64[:, 11:14] = g["color"][:, :3]
         popt = O.default_particle_options(level=3, rule=1, radial_correction=1, per_particle_seed=1)
         t0 = time.perf_counter()
-        O.render_particles_3d(scene, oc, W, H, 1, 1, tuple(float(t) for t in p.background), 1, popt)
-        dt64 = time.perf_counter() - t0
+        _, st64 = O.render_particles_3d(o64, ocam("far"), W, H, 1, 1, tuple(float(t) for t in p.background), 1, popt)
+        f64_ms = 1000.0 * (time.perf_counter() - t0)
         model = ""
         try:
             model = subprocess.run(["bash", "-c", "lscpu | grep 'Model name' | head -1"], capture_output=True,
                                    text=True).stdout.split(":", 1)[-1].strip()
         except Exception:
             pass
-        cpu = {"value": round(dtn * 1000.0 * frac_mt, 1), "unit": "ms/ensemble", "cores": threads, "kind": "port",
-               "sample": f"f32 oracle of the same pipeline (particle loop over splats on {threads} OpenMP threads), "
-                         f"first {m_mt} of {n} splats (1/{frac_mt}), 1 ensemble, {head} view, scaled x{frac_mt}; "
-                         f"host {model}, {os.cpu_count()} cpus visible",
-               "sample_seconds": round(dtn, 2),
-               "single_thread": {"value": round(dt1 * 1000.0 * frac, 1), "cores": 1,
-                                 "sample": f"first {m} splats (1/{frac}), scaled x{frac}", "sample_seconds": round(dt1, 2)},
-               "f64_reference_port": {"value": round(dt64 * 1000.0 * frac64, 1), "cores": 1,
-                                      "sample": f"f64 port of the reference's CPU particle renderer (render_particles_3d, "
-                                                f"per-particle seeds, base colour, no per-splat cap), first {m64} "
-                                                f"splats (1/{frac64}), scaled x{frac64}",
-                                      "sample_seconds": round(dt64, 2)}}
+        cpu = {"value": round(head_ms, 1), "unit": "ms/ensemble", "cores": threads, "kind": "port",
+               "sample": f"f32 oracle of the same pipeline (particle loop over splats on {threads} OpenMP threads) on "
+                         f"the whole {n}-splat scene, 1 ensemble (seed 1), {head} view, measured, not extrapolated: "
+                         f"{int(head_st[5])} candidates, {int(head_st[0])} points; host {model}, {os.cpu_count()} cpus "
+                         f"visible",
+               "views_all_threads_ms": views_ms,
+               "single_thread": {"value": round(far1_ms, 1), "cores": 1, "view": "far",
+                                 "sample": "the same oracle on one thread, whole scene, 1 ensemble, far view"},
+               "f64_reference_port": {"value": round(f64_ms, 1), "cores": 1, "view": "far",
+                                      "sample": "f64 port of the reference's CPU particle renderer (render_particles_3d, "
+                                                "per-particle seeds, base colour, no per-splat cap), whole scene, 1 "
+                                                "ensemble, far view, where no splat reaches the GPU path's cap",
+                                      "candidates": int(st64.candidates), "points": int(st64.points)}}
         log(f"[cpu] {cpu}")
 
     v = per_view[head]

@@ -5,7 +5,10 @@
 //   mode 0: f32 fused multiply-add (v_fma_f32), one per chain and iteration
 //   mode 1: the 32-bit PCG hash of the particle streams (v_mul_lo_u32, v_mad_u32, shifts, xors)
 //   mode 2: 32-bit integer add and xor with a variable shift (the hash without the multiplies)
-// The lane-instruction count of a launch is read from the SQ_INSTS_VALU counter (x 64).
+//   modes 3-7: one instruction class per chain step, pinned with register-only inline asm (no memory operands):
+//   3 v_add_u32, 4 v_xor_b32, 5 v_mul_lo_u32, 6 v_add_f32, 7 v_fma_f32 and v_add_u32 on alternate chains
+// The lane-instruction count of a launch is read from the SQ_INSTS_VALU counter (x 64); SQ_ACTIVE_INST_VALU2
+// counts the quad-cycles in which a SIMD issued two VALU instructions.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -33,6 +36,26 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t *__restrict__ out, uint32
 #pragma unroll
         for (int j = 0; j < 8; j++) s += x[j];
         out[tid] = __float_as_uint(s);
+    } else if constexpr (MODE >= 3) {
+        uint32_t x[8];
+        const uint32_t c = 0x9e3779b9u ^ seed;
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = tid * 8u + (uint32_t)j + seed;
+        for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if constexpr (MODE == 3) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 4) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 5) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 6) asm volatile("v_add_f32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else if (j & 1) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else asm volatile("v_fma_f32 %0, %0, %1, %1" : "+v"(x[j]) : "v"(c));
+            }
+        }
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) s ^= x[j];
+        out[tid] = s;
     } else {
         uint32_t x[8];
 #pragma unroll
@@ -54,10 +77,10 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t *__restrict__ out, uint32
 }  // namespace
 
 // Times `reps` launches of blocks x 256 threads, each running 8 chains of `iters` iterations of mode's operation;
-// out_ms receives the mean launch time.  Returns 0, -1 on invalid arguments (mode 0..2, blocks 1..65536, iters
+// out_ms receives the mean launch time.  Returns 0, -1 on invalid arguments (mode 0..7, blocks 1..65536, iters
 // 1..65536, reps 1..100), or -2 on a HIP error.
 extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int reps, float *out_ms) {
-    if (mode < 0 || mode > 2 || blocks < 1 || blocks > 65536 || iters < 1 || iters > 65536 || reps < 1 || reps > 100 ||
+    if (mode < 0 || mode > 7 || blocks < 1 || blocks > 65536 || iters < 1 || iters > 65536 || reps < 1 || reps > 100 ||
         !out_ms)
         return -1;
     uint32_t *buf = nullptr;
@@ -71,7 +94,12 @@ extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int rep
         hipEventRecord(e0, nullptr);
         if (mode == 0) hipLaunchKernelGGL(k_valu<0>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 1) hipLaunchKernelGGL(k_valu<1>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
-        else hipLaunchKernelGGL(k_valu<2>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 2) hipLaunchKernelGGL(k_valu<2>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 3) hipLaunchKernelGGL(k_valu<3>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 4) hipLaunchKernelGGL(k_valu<4>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 5) hipLaunchKernelGGL(k_valu<5>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 6) hipLaunchKernelGGL(k_valu<6>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else hipLaunchKernelGGL(k_valu<7>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         hipEventRecord(e1, nullptr);
         if (hipEventSynchronize(e1) != hipSuccess) rc = -2;
         float ms = 0.0f;

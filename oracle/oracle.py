@@ -234,9 +234,16 @@ def render_monte_carlo(scene, cam, w, h, spp_side, sets, bg, seed, opt=None):
 
 
 def render_particles_3d(scene, cam, w, h, spp_side, sets, bg, seed, opt=None):
+    """scene: a list of dicts, or an (n, 14) f64 array of OrcGaussian rows (pos, log_scale, rot xyzw, opacity,
+    colour)."""
     out = np.zeros((h, w, 3), np.float64)
     st = OrcStats()
-    arr = gaussians_f64(scene)
+    if isinstance(scene, np.ndarray):
+        keep = np.ascontiguousarray(scene, np.float64).reshape(-1, 14)
+        assert C.sizeof(OrcGaussian) == 14 * 8
+        arr = keep.ctypes.data_as(P(OrcGaussian))
+    else:
+        arr = gaussians_f64(scene)
     opt = opt or default_particle_options()
     lib.orc_render_particles_3d(arr, len(scene), C.byref(cam), w, h, spp_side, sets, (f64 * 3)(*bg), seed,
                                 C.byref(opt), ptr(out, f64), C.byref(st))
