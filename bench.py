@@ -72,6 +72,7 @@ def particle_roofline(dname, dcount, dms, n, ens, particles, hits, Sx, view):
     out.update(bound="valu", achieved=round(achieved, 3), peak=peak, unit="T VALU lane-instructions/s",
                frac=round(achieved / peak, 4), traffic=None,
                lane_insts_per_particle=lane_per_particle, particles_per_launch=int(ppl),
+               valu_port_busy=vp["views"][view].get("issue", {}).get("valu_port_busy"),
                floor_lane_insts_per_particle=vp["floor_lane_insts_per_particle"],
                source="profiles/particles_valu.json (SQ_INSTS_VALU per particle; ceiling from bench.py --mode valu)")
     hbm = {"algorithmic_bytes_per_launch": int(alg), "algorithmic_GBps": round(alg / avg_s / 1e9, 2),
