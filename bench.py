@@ -259,9 +259,10 @@ BATCH_RES = ((640, 360), (1280, 720), (1920, 1080))
 def bench_batch8(a):
     """Eight clustered scenes (seeds 10-17, SH3) of uneven size; scene i renders at BATCH_RES[i % 3]; each
     scene gets a 16-view 360-degree orbit (radius 6, elevation 15 degrees) with 32 ensembles per view of PBVR
-    Extinction C3+R (SPP 1, cap 16384).  All 128 views are recorded into one hipGraph and launched once.
-    Compared with the same items launched straight onto the stream and with one submission per ensemble
-    (one render_ensembles call per ensemble, the reference's trajectory-export pattern)."""
+    Extinction C3+R (SPP 1, cap 16384).  Legs: the fused batch (every view in the same few launches per round of
+    ensembles) recorded into one hipGraph and launched straight onto the stream; the per-item batch (each view's
+    own launches, on up to four streams) the same two ways; one submission per ensemble (one render_ensembles
+    call per ensemble, the reference's trajectory-export pattern).  Every leg's images are compared bit for bit."""
     import gps_hip as gh
     from gps_hip import scenes as S
     views, ens = 16, 32
@@ -286,19 +287,26 @@ def bench_batch8(a):
         log(f"[batch8 autotune] {tuning} {tms:.3f} ms/ens")
     b = gh.Batch()
     out = {}
-    for leg, use_graph in (("graph", True), ("stream", False)):
+    last = [len(items) - 1 - k * views for k in range(len(BATCH_SIZES))][::-1]
+    snap = None
+    for leg, use_graph, per_item in (("graph", True, False), ("stream", False, False),
+                                     ("per_item_graph", True, True), ("per_item_stream", False, True)):
         for _ in range(max(a.warmup, 1)):
-            b.render(items, use_graph=use_graph)
+            b.render(items, use_graph=use_graph, per_item=per_item)
         gms, rec = [], []
         for _ in range(max(a.steps, 1)):
-            b.render(items, use_graph=use_graph)
+            b.render(items, use_graph=use_graph, per_item=per_item)
             t = b.timing()
             gms.append(t["gpu_ms"])
             rec.append(t["record_ms"])
         out[leg] = {"gpu_ms": round(min(gms), 2), "record_ms": round(min(rec), 2), "graph_nodes": t["graph_nodes"]}
+        imgs = {i: b.read(i) for i in last}
+        if snap is None:
+            snap = imgs
+        else:
+            out[leg]["bit_identical_to_graph"] = all(np.array_equal(snap[i].view(np.uint32), imgs[i].view(np.uint32))
+                                                     for i in last)
         log(f"[batch8 {leg}] {out[leg]}")
-    last = [len(items) - 1 - k * views for k in range(len(BATCH_SIZES))][::-1]
-    snap = {i: b.read(i) for i in last}
     for r in rends:
         r.wait_idle()
     t0 = time.perf_counter()
@@ -331,7 +339,9 @@ def bench_batch8(a):
         "views": len(items), "ensembles": len(items) * ens, "legs": out,
         "ms_per_view": round(gm / len(items), 3), "ensembles_per_s": round(len(items) * ens / (gm * 1e-3), 1),
         "gaussian_ensembles_per_s": round(total_g / (gm * 1e-3), 1),
-        "graph_speedup_vs_per_ensemble_submits": round(per_ens_ms / gm, 3), "tuning": tuning}), flush=True)
+        "graph_speedup_vs_per_ensemble_submits": round(per_ens_ms / gm, 3),
+        "fused_speedup_vs_per_item_graph": round(out["per_item_graph"]["gpu_ms"] / gm, 3), "tuning": tuning}),
+        flush=True)
     b.close()
     for r in rends:
         r.close()

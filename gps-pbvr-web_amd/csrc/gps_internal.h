@@ -66,6 +66,32 @@ struct BatchState {
     uint32_t *skip;         // [MAX_BATCH] 1 if the ensemble overflowed u32
 };
 
+// One item of a fused batch (gps_batch_render): a view of a renderer's scene, rendered with buffers the batch owns
+// for it.  Every launch of a fused batch covers all items; a round renders up to `round_ens` ensembles of each item.
+struct FusedItem {
+    GpsFrame f;
+    const float4 *splats;
+    const float *sh;
+    const float *lut;
+    float4 *view;           // [N] x 6 float4
+    uint2 *cinfo;           // [N]
+    float4 *colour;         // [N]
+    uint32_t *perm;         // [N + 3] count-only splat order, or null
+    uint32_t *offsets;      // [round_ens][N]
+    uint64_t *depth;        // [round_ens][sub_count], all ones between rounds
+    BatchState bs;          // [MAX_BATCH] tickets, totals, skip; [round_ens][tiles] status
+    float4 *accum;          // [X] the item's image
+    float4 *mom;            // [X] denoiser moments, or null
+    float *zmin;            // [X]
+    uint64_t *stats;        // [ST_WORDS]
+    uint64_t sub_count;
+    uint32_t tiles;         // scan tiles of N
+    uint32_t first_seed;
+    uint32_t mk;            // particle emitter: 0 GPS, 1 PBVR, 2 occupancy
+};
+constexpr uint32_t FUSED_NONE = 0xFFFFFFFFu;
+constexpr int FUSED_MAX_ITEMS = 1024;
+
 // kernel launchers (gps_kernels.hip)
 // perm (optional, [N + 3]): the view kernel writes the splat order with the count-only splats first into
 // perm[0, N) and counts them in the 8-byte word at perm + perm_counter(N) (zeroed before); the count scan and
@@ -98,6 +124,19 @@ void launch_phase0(hipStream_t s, uint32_t seed, uint32_t pixels, uint32_t sampl
 void launch_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ticket, uint64_t *status,
                      uint32_t tiles, uint32_t epoch);
 void launch_math(hipStream_t s, int fn, const float *x, const float *y, float *out, uint64_t n);
+// fused batch launches (gps_kernels.hip); tables are device arrays built by the host at record time
+void launch_fused_clear(hipStream_t s, const FusedItem *items, uint32_t nitems, uint32_t max_x);
+void launch_fused_view(hipStream_t s, const FusedItem *items, const uint32_t *blk_item, const uint32_t *item_blk0,
+                       uint32_t blocks, size_t lds);
+void launch_fused_count_scan(hipStream_t s, const FusedItem *items, const uint32_t *blk_pair, uint32_t blocks,
+                             uint32_t ens0, uint32_t epoch);
+void launch_fused_assign(hipStream_t s, const FusedItem *items, uint32_t nitems, const uint32_t *nens, uint32_t mk,
+                         uint32_t chunk, uint32_t grid, uint32_t *blk_item, uint32_t *blk_rank, uint32_t *item_nblk);
+void launch_fused_particles(hipStream_t s, const FusedItem *items, uint32_t mk, const uint32_t *nens, uint32_t ens0,
+                            const uint32_t *blk_item, const uint32_t *blk_rank, const uint32_t *item_nblk, int grid,
+                            int ppt, int block);
+void launch_fused_resolve(hipStream_t s, const FusedItem *items, const uint32_t *nens, const uint32_t *blk_item,
+                          const uint32_t *item_blk0, uint32_t blocks);
 
 // device buffers of a renderer for the denoiser: accumulation, moments and nearest depth (null when the
 // renderer's features are off)

@@ -400,12 +400,13 @@ GPS_DEV bool box_offscreen(const GpsFrame &f, const CamBox &b) {
 // The block's splats and SH rest coefficients are contiguous; they are staged in LDS with 16-byte loads
 // across the block (a per-thread gather of 180-byte records would re-fetch cache lines), and each thread
 // then reads its own record from LDS.
-__global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restrict__ splats,
-                                              const float *__restrict__ sh, float4 *__restrict__ view,
-                                              uint2 *__restrict__ cinfo, float4 *__restrict__ colour,
-                                              uint32_t *__restrict__ perm) {
+// The view kernel's body for block `blk` of one view (k_view: one launch per view; k_view_fused: every item of a
+// batch in one launch).
+GPS_DEV void view_body(const GpsFrame &f, const float4 *__restrict__ splats, const float *__restrict__ sh,
+                       float4 *__restrict__ view, uint2 *__restrict__ cinfo, float4 *__restrict__ colour,
+                       uint32_t *__restrict__ perm, uint32_t blk) {
     extern __shared__ float4 s_stage[];  // 4 float4 per splat, then 3 * sh_stride floats per splat
-    const uint32_t i0 = blockIdx.x * blockDim.x;
+    const uint32_t i0 = blk * blockDim.x;
     const uint32_t nb = min((uint32_t)blockDim.x, f.N - i0);
     for (uint32_t t = threadIdx.x; t < 4u * nb; t += blockDim.x) s_stage[t] = splats[4u * i0 + t];
     float *s_sh = reinterpret_cast<float *>(s_stage + 4u * blockDim.x);
@@ -549,6 +550,13 @@ __global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restri
     }
 }
 
+__global__ __launch_bounds__(256) void k_view(GpsFrame f, const float4 *__restrict__ splats,
+                                              const float *__restrict__ sh, float4 *__restrict__ view,
+                                              uint2 *__restrict__ cinfo, float4 *__restrict__ colour,
+                                              uint32_t *__restrict__ perm) {
+    view_body(f, splats, sh, view, cinfo, colour, perm, blockIdx.x);
+}
+
 // ================================================================== decoupled look-back
 // status word: [63:38] epoch, [37:36] flag (1 aggregate, 2 inclusive prefix), [35:0] value: saturated at 2^36 - 1
 // for the particle counts (a total above 2^32 - 1 marks the ensemble as skipped), or modulo 2^32 (WRAP32) for the
@@ -619,13 +627,11 @@ GPS_DEV uint64_t block_exclusive_scan(uint64_t v, uint64_t &excl, uint64_t *s_wa
 }
 
 // ================================================================== k_count_scan
-// grid = (tiles, ensembles of the batch)
-__global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan(GpsFrame f, const uint2 *__restrict__ cinfo,
-                                                           uint32_t first_seed, uint32_t *__restrict__ offsets,
-                                                           BatchState bs, uint32_t tiles, uint32_t epoch,
-                                                           uint64_t *__restrict__ stats,
-                                                           const uint32_t *__restrict__ perm) {
-    const uint32_t e = blockIdx.y;
+// The body for one tile of ensemble e (k_count_scan: grid = (tiles, ensembles of the launch); k_count_scan_fused:
+// every (item, ensemble) of a batch round in one launch).  Tiles are drawn by ticket per ensemble.
+GPS_DEV void count_scan_body(const GpsFrame &f, const uint2 *__restrict__ cinfo, uint32_t first_seed,
+                             uint32_t *__restrict__ offsets, const BatchState &bs, uint32_t tiles, uint32_t epoch,
+                             uint64_t *__restrict__ stats, const uint32_t *__restrict__ perm, uint32_t e) {
     const uint32_t seed = first_seed + e;
     __shared__ uint32_t s_tile;
     __shared__ uint64_t s_wave[SCAN_BLOCK / 64];
@@ -706,6 +712,14 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan(GpsFrame f, const uin
         if (over) atomicAdd((unsigned long long *)&stats[ST_SKIPPED], 1ull);
         bs.tickets[e] = 0u;   // every tile of this ensemble has drawn its ticket
     }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan(GpsFrame f, const uint2 *__restrict__ cinfo,
+                                                           uint32_t first_seed, uint32_t *__restrict__ offsets,
+                                                           BatchState bs, uint32_t tiles, uint32_t epoch,
+                                                           uint64_t *__restrict__ stats,
+                                                           const uint32_t *__restrict__ perm) {
+    count_scan_body(f, cinfo, first_seed, offsets, bs, tiles, epoch, stats, perm, blockIdx.y);
 }
 
 // ================================================================== particles
@@ -972,13 +986,14 @@ GPS_DEV int keep_draw(const GpsFrame &f, uint32_t seed, const float4 *__restrict
 // 4 particles per thread, lets a CU hold seven blocks; the register limit costs a few bytes of spills, which the
 // count-only views repay), six at 128 and 512 threads, where the exact path's spills would cost more (measured).
 // The other paths keep the compiler's choice.
+// The body run by block `rank` of the `nblocks` blocks that share one view's nens ensembles (k_particles: the whole
+// grid; k_particles_fused: the blocks a batch round assigns to one item).
 template <int MK, int PPT, int OUT, int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MK == 1 && OUT == 0 ? (BS == 256 ? 7 : 6) : 1)))
-void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__restrict__ offsets,
-                 const uint32_t *__restrict__ totals, uint32_t nens, uint32_t first_seed,
-                 uint64_t *__restrict__ depth, uint64_t sub_count, const float *__restrict__ lut,
-                 uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys, uint32_t *__restrict__ pidx,
-                 const uint32_t *__restrict__ perm) {
+GPS_DEV void particles_body(const GpsFrame &f, const float4 *__restrict__ view, const uint32_t *__restrict__ offsets,
+                            const uint32_t *__restrict__ totals, uint32_t nens, uint32_t first_seed,
+                            uint64_t *__restrict__ depth, uint64_t sub_count, const float *__restrict__ lut,
+                            uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys, uint32_t *__restrict__ pidx,
+                            const uint32_t *__restrict__ perm, uint32_t rank, uint32_t nblocks) {
     constexpr uint32_t CH = BS * PPT;
     // PBVR packed-minimum path: a particle of a splat marked off-screen is settled by keep_draw when its draws
     // alone decide; otherwise (splat, particle seed) goes to its wave's buffer, which after each chunk runs
@@ -1032,8 +1047,8 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
     uint32_t rn = 0;   // this wave's pending count_only work: entries [0, rn) of its buffer (the same in every lane)
     uint32_t prev = 0xFFFFFFFFu, prev_hi = 0;   // the block's previous chunk and the owner of its last particle
     // runs of CRUN consecutive chunks per block: a chunk's owners are searched from the previous chunk's last
-    for (uint32_t chunk = blockIdx.x * CRUN; chunk < nchunks;
-         chunk = (chunk % CRUN == CRUN - 1u) ? chunk + (gridDim.x - 1u) * CRUN + 1u : chunk + 1u) {
+    for (uint32_t chunk = rank * CRUN; chunk < nchunks;
+         chunk = (chunk % CRUN == CRUN - 1u) ? chunk + (nblocks - 1u) * CRUN + 1u : chunk + 1u) {
         uint32_t b = 0;
         while (b + 1u < nens && s_cstart[b + 1u] <= chunk) b++;
         const uint32_t total = totals[b];
@@ -1251,6 +1266,17 @@ void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__
     }
 }
 
+template <int MK, int PPT, int OUT, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MK == 1 && OUT == 0 ? (BS == 256 ? 7 : 6) : 1)))
+void k_particles(GpsFrame f, const float4 *__restrict__ view, const uint32_t *__restrict__ offsets,
+                 const uint32_t *__restrict__ totals, uint32_t nens, uint32_t first_seed,
+                 uint64_t *__restrict__ depth, uint64_t sub_count, const float *__restrict__ lut,
+                 uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys, uint32_t *__restrict__ pidx,
+                 const uint32_t *__restrict__ perm) {
+    particles_body<MK, PPT, OUT, BS>(f, view, offsets, totals, nens, first_seed, depth, sub_count, lut, stats, pkeys,
+                                     pidx, perm, blockIdx.x, gridDim.x);
+}
+
 // ================================================================== splat-parallel GPS path
 // gps_splat.wgsl:304-333: one thread per splat, one sequential stream for its count and particles
 __global__ __launch_bounds__(64) void k_splat_pass(GpsFrame f, const float4 *__restrict__ view,
@@ -1283,14 +1309,14 @@ __global__ __launch_bounds__(64) void k_splat_pass(GpsFrame f, const float4 *__r
 // With `mom` set it also accumulates, per pixel, the sum of squares of each ensemble's pixel value (x, y, z)
 // and the number of covered subpixels (w), and keeps in `zmin` the nearest depth seen over the ensembles:
 // the denoiser's variance, particle-count and depth features.
-__global__ __launch_bounds__(256) void k_resolve(GpsFrame f, uint64_t *__restrict__ depth, uint64_t sub_count,
-                                                 const uint32_t *__restrict__ skip, uint32_t nens,
-                                                 const float4 *__restrict__ colour, float4 *__restrict__ accum,
-                                                 int clear, uint64_t *__restrict__ stats, float4 *__restrict__ mom,
-                                                 float *__restrict__ zmin) {
+// The body for pixel block `blk` of one view (k_resolve; k_resolve_fused: every item of a batch round).
+GPS_DEV void resolve_body(const GpsFrame &f, uint64_t *__restrict__ depth, uint64_t sub_count,
+                          const uint32_t *__restrict__ skip, uint32_t nens, const float4 *__restrict__ colour,
+                          float4 *__restrict__ accum, int clear, uint64_t *__restrict__ stats, float4 *__restrict__ mom,
+                          float *__restrict__ zmin, uint32_t blk) {
     const uint32_t X = f.W * f.H;
     const uint32_t spp = f.spp_side * f.spp_side;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = blk * blockDim.x + threadIdx.x;
     if (i == 0) atomicAdd((unsigned long long *)&stats[ST_ENSEMBLES], (unsigned long long)nens);
     if (i >= X) return;
     float4 acc = accum[i];
@@ -1339,6 +1365,14 @@ __global__ __launch_bounds__(256) void k_resolve(GpsFrame f, uint64_t *__restric
         mom[i] = m2;
         zmin[i] = __uint_as_float(zbits);
     }
+}
+
+__global__ __launch_bounds__(256) void k_resolve(GpsFrame f, uint64_t *__restrict__ depth, uint64_t sub_count,
+                                                 const uint32_t *__restrict__ skip, uint32_t nens,
+                                                 const float4 *__restrict__ colour, float4 *__restrict__ accum,
+                                                 int clear, uint64_t *__restrict__ stats, float4 *__restrict__ mom,
+                                                 float *__restrict__ zmin) {
+    resolve_body(f, depth, sub_count, skip, nens, colour, accum, clear, stats, mom, zmin, blockIdx.x);
 }
 
 // moments to zero and nearest depth to +inf
@@ -1484,6 +1518,136 @@ __global__ void k_math(int fn, const float *__restrict__ x, const float *__restr
     out[i] = r;
 }
 
+
+// ================================================================== fused batch (gps_batch_render)
+// Every launch covers all items of a batch; each block finds its item (and its block index within the item) in a
+// table the host builds at record time, or, for the particle kernel, one k_fused_assign writes on the device once
+// the round's particle totals are known.  The bodies are the single-view kernels' own.  A round's ensembles of
+// item i are [ens0, ens0 + nens[i]) of the item's own sequence: nens comes from the host's table (0 once the
+// item's ensembles are done), ens0 is the round's first ensemble (added to the item's first seed modulo 2^32, as
+// the single-view path adds the ensembles it has done).
+
+// grid = (blocks over the largest image, items): each item's image, denoiser moments (nearest depth to +inf),
+// counters, and the count-only splat order's counter word, before the batch
+__global__ __launch_bounds__(256) void k_fused_clear(const FusedItem *__restrict__ items) {
+    const FusedItem &it = items[blockIdx.y];
+    const uint64_t X = (uint64_t)it.f.W * it.f.H;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < X; i += (uint64_t)gridDim.x * blockDim.x) {
+        it.accum[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (it.mom) {
+            it.mom[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            it.zmin[i] = __uint_as_float(0x7F800000u);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (uint32_t)ST_WORDS) it.stats[threadIdx.x] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < 2u && it.perm) it.perm[perm_counter(it.f.N) + threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_view_fused(const FusedItem *__restrict__ items,
+                                                    const uint32_t *__restrict__ blk_item,
+                                                    const uint32_t *__restrict__ item_blk0) {
+    const uint32_t item = blk_item[blockIdx.x];
+    const FusedItem &it = items[item];
+    view_body(it.f, it.splats, it.sh, it.view, it.cinfo, it.colour, it.perm, blockIdx.x - item_blk0[item]);
+}
+
+// blk_pair: item << 8 | ensemble of the round, one entry per (item, ensemble) tile
+__global__ __launch_bounds__(SCAN_BLOCK) void k_count_scan_fused(const FusedItem *__restrict__ items,
+                                                                const uint32_t *__restrict__ blk_pair, uint32_t ens0,
+                                                                uint32_t epoch) {
+    const uint32_t pr = blk_pair[blockIdx.x];
+    const FusedItem &it = items[pr >> 8];
+    count_scan_body(it.f, it.cinfo, it.first_seed + ens0, it.offsets, it.bs, it.tiles, epoch, it.stats, it.perm,
+                    pr & 255u);
+}
+
+// One block: gives each item of emitter class mk that has particles this round a contiguous range of the particle
+// grid, at least one block and otherwise in proportion to its chunks of `chunk` particles; blocks beyond the
+// ranges get FUSED_NONE.  nitems <= FUSED_MAX_ITEMS = blockDim.x, grid >= nitems.
+__global__ __launch_bounds__(FUSED_MAX_ITEMS) void k_fused_assign(const FusedItem *__restrict__ items,
+                                                                  uint32_t nitems, const uint32_t *__restrict__ nens,
+                                                                  uint32_t mk, uint32_t chunk, uint32_t grid,
+                                                                  uint32_t *__restrict__ blk_item,
+                                                                  uint32_t *__restrict__ blk_rank,
+                                                                  uint32_t *__restrict__ item_nblk) {
+    __shared__ uint32_t s_off[FUSED_MAX_ITEMS + 1], s_nb[FUSED_MAX_ITEMS];
+    __shared__ unsigned long long s_total;
+    __shared__ uint32_t s_active;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        s_total = 0ull;
+        s_active = 0u;
+    }
+    __syncthreads();
+    uint64_t ch = 0;
+    if (t < nitems && items[t].mk == mk) {
+        const FusedItem &it = items[t];
+        const uint32_t ne = nens[t];
+        for (uint32_t e = 0; e < ne; e++) ch += ((uint64_t)it.bs.totals[e] + (chunk - 1u)) / chunk;
+    }
+    if (ch) {
+        atomicAdd(&s_total, (unsigned long long)ch);
+        atomicAdd(&s_active, 1u);
+    }
+    __syncthreads();
+    const uint64_t T = s_total;
+    const uint32_t A = s_active;
+    const uint32_t nb = ch ? 1u + (uint32_t)((uint64_t)(grid - A) * ch / T) : 0u;   // sum over items <= grid
+    if (t < nitems) {
+        s_nb[t] = nb;
+        item_nblk[t] = nb;
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (uint32_t i = 0; i < nitems; i++) {
+            s_off[i] = acc;
+            acc += s_nb[i];
+        }
+        s_off[nitems] = acc;
+    }
+    __syncthreads();
+    const uint32_t used = s_off[nitems];
+    for (uint32_t g = t; g < grid; g += blockDim.x) {
+        uint32_t item = FUSED_NONE, rank = 0;
+        if (g < used) {   // the last item whose range starts at or before g (items without blocks have empty ranges)
+            uint32_t lo = 0, hi = nitems - 1u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (s_off[mid] <= g) lo = mid; else hi = mid - 1u;
+            }
+            item = lo;
+            rank = g - s_off[lo];
+        }
+        blk_item[g] = item;
+        blk_rank[g] = rank;
+    }
+}
+
+template <int MK, int PPT, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MK == 1 ? (BS == 256 ? 7 : 6) : 1)))
+void k_particles_fused(const FusedItem *__restrict__ items, const uint32_t *__restrict__ nens, uint32_t ens0,
+                       const uint32_t *__restrict__ blk_item, const uint32_t *__restrict__ blk_rank,
+                       const uint32_t *__restrict__ item_nblk) {
+    const uint32_t item = blk_item[blockIdx.x];
+    if (item == FUSED_NONE) return;   // the whole block
+    const FusedItem &it = items[item];
+    particles_body<MK, PPT, 0, BS>(it.f, it.view, it.offsets, it.bs.totals, nens[item], it.first_seed + ens0,
+                                   it.depth, it.sub_count, it.lut, it.stats, nullptr, nullptr, it.perm,
+                                   blk_rank[blockIdx.x], item_nblk[item]);
+}
+
+__global__ __launch_bounds__(256) void k_resolve_fused(const FusedItem *__restrict__ items,
+                                                       const uint32_t *__restrict__ nens,
+                                                       const uint32_t *__restrict__ blk_item,
+                                                       const uint32_t *__restrict__ item_blk0) {
+    const uint32_t item = blk_item[blockIdx.x];
+    const FusedItem &it = items[item];
+    const uint32_t ne = nens[item];
+    if (ne == 0) return;
+    resolve_body(it.f, it.depth, it.sub_count, it.bs.skip, ne, it.colour, it.accum, 1, it.stats, it.mom, it.zmin,
+                 blockIdx.x - item_blk0[item]);
+}
 }  // namespace
 
 // ================================================================== launchers
@@ -1606,4 +1770,58 @@ void launch_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, uint64_t 
 
 void launch_math(hipStream_t s, int fn, const float *x, const float *y, float *out, uint64_t n) {
     hipLaunchKernelGGL(k_math, dim3(blocks_for(n, 256)), dim3(256), 0, s, fn, x, y, out, n);
+}
+
+void launch_fused_clear(hipStream_t s, const FusedItem *items, uint32_t nitems, uint32_t max_x) {
+    hipLaunchKernelGGL(k_fused_clear, dim3(max(1u, min(4096u, (max_x + 255u) / 256u)), nitems), dim3(256), 0, s,
+                       items);
+}
+
+void launch_fused_view(hipStream_t s, const FusedItem *items, const uint32_t *blk_item, const uint32_t *item_blk0,
+                       uint32_t blocks, size_t lds) {
+    if (blocks) hipLaunchKernelGGL(k_view_fused, dim3(blocks), dim3(256), lds, s, items, blk_item, item_blk0);
+}
+
+void launch_fused_count_scan(hipStream_t s, const FusedItem *items, const uint32_t *blk_pair, uint32_t blocks,
+                             uint32_t ens0, uint32_t epoch) {
+    if (blocks)
+        hipLaunchKernelGGL(k_count_scan_fused, dim3(blocks), dim3(SCAN_BLOCK), 0, s, items, blk_pair, ens0, epoch);
+}
+
+void launch_fused_assign(hipStream_t s, const FusedItem *items, uint32_t nitems, const uint32_t *nens, uint32_t mk,
+                         uint32_t chunk, uint32_t grid, uint32_t *blk_item, uint32_t *blk_rank, uint32_t *item_nblk) {
+    hipLaunchKernelGGL(k_fused_assign, dim3(1), dim3(FUSED_MAX_ITEMS), 0, s, items, nitems, nens, mk, chunk, grid,
+                       blk_item, blk_rank, item_nblk);
+}
+
+void launch_fused_particles(hipStream_t s, const FusedItem *items, uint32_t mk, const uint32_t *nens, uint32_t ens0,
+                            const uint32_t *blk_item, const uint32_t *blk_rank, const uint32_t *item_nblk, int grid,
+                            int ppt, int block) {
+#define GPS_FUSED_BS(MK, PPT, BS)                                                                             \
+    hipLaunchKernelGGL((k_particles_fused<MK, PPT, BS>), dim3(grid), dim3(BS), 0, s, items, nens, ens0,      \
+                       blk_item, blk_rank, item_nblk)
+#define GPS_FUSED(MK, PPT)                                       \
+    do {                                                         \
+        if (block == 128) GPS_FUSED_BS(MK, PPT, 128);            \
+        else if (block == 512) GPS_FUSED_BS(MK, PPT, 512);       \
+        else GPS_FUSED_BS(MK, PPT, 256);                         \
+    } while (0)
+#define GPS_FUSED_PPT(MK)                         \
+    do {                                          \
+        if (ppt == 1) GPS_FUSED(MK, 1);           \
+        else if (ppt == 2) GPS_FUSED(MK, 2);      \
+        else if (ppt == 8) GPS_FUSED(MK, 8);      \
+        else GPS_FUSED(MK, 4);                    \
+    } while (0)
+    if (mk == 0) GPS_FUSED_PPT(0);
+    else if (mk == 2) GPS_FUSED_PPT(2);
+    else GPS_FUSED_PPT(1);
+#undef GPS_FUSED_PPT
+#undef GPS_FUSED
+#undef GPS_FUSED_BS
+}
+
+void launch_fused_resolve(hipStream_t s, const FusedItem *items, const uint32_t *nens, const uint32_t *blk_item,
+                          const uint32_t *item_blk0, uint32_t blocks) {
+    if (blocks) hipLaunchKernelGGL(k_resolve_fused, dim3(blocks), dim3(256), 0, s, items, nens, blk_item, item_blk0);
 }

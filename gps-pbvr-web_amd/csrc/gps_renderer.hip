@@ -969,6 +969,18 @@ int gps_renderer_profile_read(GpsRenderer *r, GpsKernelTime *out, int max) {
 // branches under capture), one per renderer in order of first appearance (round robin), and joins them.
 constexpr int BATCH_LANES = 4;
 
+// A fused item's buffers, owned by the batch and kept for its next render when large enough.
+struct FusedBuf {
+    float4 *view = nullptr, *colour = nullptr, *mom = nullptr;
+    uint2 *cinfo = nullptr;
+    uint32_t *perm = nullptr, *offsets = nullptr;
+    uint32_t *small = nullptr;   // tickets, totals, skip: MAX_BATCH words each, whatever a batch's round size
+    uint64_t *depth = nullptr, *status = nullptr;
+    float *zmin = nullptr;
+    uint64_t n_cap = 0, off_cap = 0, depth_cap = 0, status_cap = 0, ens_cap = 0, x_cap = 0;   // in elements
+    bool depth_clean = false;   // every depth|id word is all ones (a finished round leaves it so)
+};
+
 struct GpsBatch {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -983,6 +995,13 @@ struct GpsBatch {
     uint64_t composite_cap = 0;
     double gpu_ms = 0.0, record_ms = 0.0;
     uint64_t nodes = 0;
+    // fused path: per-item buffers, the device item table and the block tables
+    std::vector<FusedBuf> fbuf;
+    FusedItem *fitems = nullptr;
+    uint32_t fitems_cap = 0;
+    uint32_t *tables = nullptr;
+    uint64_t tables_cap = 0;
+    uint32_t epoch = 1;
 };
 
 int gps_batch_create(int device, GpsBatch **out) {
@@ -1017,10 +1036,284 @@ void gps_batch_destroy(GpsBatch *b) {
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (auto *o : b->outs)
         if (o) (void)hipFree(o);
+    for (FusedBuf &f : b->fbuf) {
+        for (void *q : {(void *)f.view, (void *)f.colour, (void *)f.mom, (void *)f.cinfo, (void *)f.perm,
+                        (void *)f.offsets, (void *)f.small, (void *)f.depth, (void *)f.status, (void *)f.zmin})
+            if (q) (void)hipFree(q);
+    }
+    if (b->fitems) (void)hipFree(b->fitems);
+    if (b->tables) (void)hipFree(b->tables);
     if (b->item_stats) (void)hipFree(b->item_stats);
     if (b->composite_buf) (void)hipFree(b->composite_buf);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+
+extern "C++" {
+namespace {
+
+// (re)allocates p for `need` elements when its capacity is smaller; false when out of memory
+template <typename T>
+bool grow(T *&p, uint64_t &cap, uint64_t need) {
+    if (need <= cap && p) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, (need ? need : 1) * sizeof(T)) != hipSuccess) return false;
+    cap = need;
+    return true;
+}
+
+// Emitter class of an item on the particle path, or -1 when it cannot be fused (the splat-parallel GPS path).
+int fused_mk(const GpsRenderParams *p) {
+    if (p->method == GPS_METHOD_GPS) return p->path == GPS_PATH_PARTICLE ? 0 : -1;
+    return p->method == GPS_METHOD_SCREEN_OCCUPANCY ? 2 : 1;
+}
+
+}  // namespace
+}  // extern "C++"
+
+// Fused batch: every item renders through the same few launches.  Items get buffers of their own (view cache,
+// offsets, depth|id buffers for `round_ens` ensembles, image, counters), so items that share a renderer are
+// independent.  Launches: clear, view (all items), then per round of up to round_ens ensembles of every item: count
+// and scan (every (item, ensemble) tile), per emitter class an assignment of the particle grid to items and the
+// particle kernel, and resolve (every item's pixels).  The particle grid is split among the items in proportion
+// to their chunks, each item's blocks running the single-view kernel's body over its own chunks.
+static int fused_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, bool use_graph) {
+    const int ppt = (int)g_tuning.particles_per_thread, block = (int)g_tuning.particle_block;
+    const uint32_t chunk = (uint32_t)(ppt * block);
+    // rounds: ensembles per round from the tuning, else as many as keep the largest item's depth|id buffers
+    // within the depth window
+    uint64_t max_s = 1, max_x = 1;
+    uint32_t max_count = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const GpsRenderer *r = items[i].renderer;
+        max_s = std::max<uint64_t>(max_s, sub_count(r));
+        max_x = std::max<uint64_t>(max_x, (uint64_t)r->W * r->H);
+        max_count = std::max(max_count, items[i].count);
+    }
+    uint32_t round_ens = g_tuning.batch_ensembles;
+    if (!round_ens) {
+        const uint64_t w = ((uint64_t)g_tuning.depth_window_mib << 20) / (8 * max_s);
+        round_ens = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(w, 1), MAX_BATCH);
+    }
+    const uint32_t rounds = (max_count + round_ens - 1) / round_ens;
+    // buffers and descriptors
+    if (b->fbuf.size() < n) b->fbuf.resize(n);
+    std::vector<FusedItem> fi(n);
+    size_t lds = 0;
+    bool mk_used[3] = {false, false, false};
+    for (uint32_t i = 0; i < n; i++) {
+        const GpsBatchItem &it = items[i];
+        GpsRenderer *r = it.renderer;
+        FusedBuf &q = b->fbuf[i];
+        FusedItem &d = fi[i];
+        pack_frame(r, it.camera, it.params, d.f);
+        const uint64_t N = r->n, S = sub_count(r), X = (uint64_t)r->W * r->H;
+        const uint32_t tiles = (uint32_t)((N + SCAN_TILE - 1) / SCAN_TILE);
+        uint64_t cap_n = q.n_cap;   // the view cache, colours, count info and splat order share one capacity
+        bool ok = true;
+        if (N > q.n_cap) {
+            uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+            ok = grow(q.view, c0, (uint64_t)VIEW_VEC4 * N) && grow(q.colour, c1, N) && grow(q.cinfo, c2, N) && grow(q.perm, c3, N + 3);
+            cap_n = ok ? N : 0;
+        }
+        q.n_cap = cap_n;
+        const uint64_t ecap = round_ens;
+        ok = ok && grow(q.offsets, q.off_cap, ecap * N);
+        const bool depth_new = q.depth_cap < ecap * S || !q.depth;
+        ok = ok && grow(q.depth, q.depth_cap, ecap * S);
+        const bool status_new = q.status_cap < ecap * tiles || !q.status;
+        ok = ok && grow(q.status, q.status_cap, ecap * std::max<uint32_t>(tiles, 1));
+        const bool small_new = !q.small;
+        ok = ok && grow(q.small, q.ens_cap, 3 * (uint64_t)MAX_BATCH);
+        if (ok && r->features && (X > q.x_cap || !q.mom || !q.zmin)) {
+            uint64_t c0 = 0, c1 = 0;
+            ok = grow(q.mom, c0, X) && grow(q.zmin, c1, X);
+            q.x_cap = ok ? X : 0;
+        }
+        if (!ok) {
+            g_create_error = "out of device memory for a fused batch";
+            return -1;
+        }
+        if (depth_new) q.depth_clean = false;
+        if ((status_new && hipMemsetAsync(q.status, 0, q.status_cap * sizeof(uint64_t), b->stream) != hipSuccess) ||
+            (small_new && hipMemsetAsync(q.small, 0, q.ens_cap * sizeof(uint32_t), b->stream) != hipSuccess)) {
+            g_create_error = "fused batch buffer clear failed";
+            return -1;
+        }
+        if (!q.depth_clean) {
+            launch_fill_u64(b->stream, q.depth, q.depth_cap, EMPTY64);
+            q.depth_clean = true;
+        }
+        d.splats = r->splats;
+        d.sh = r->sh;
+        d.lut = r->lut;
+        d.view = q.view;
+        d.cinfo = q.cinfo;
+        d.colour = q.colour;
+        d.perm = (d.f.flags & FLAG_FAST) ? q.perm : nullptr;
+        d.offsets = q.offsets;
+        d.depth = q.depth;
+        d.bs.tickets = q.small;   // zero between scans: the last tile of an ensemble resets its word
+        d.bs.totals = q.small + MAX_BATCH;
+        d.bs.skip = q.small + 2 * MAX_BATCH;
+        d.bs.status = q.status;
+        d.accum = b->outs[i];
+        d.mom = r->features ? q.mom : nullptr;
+        d.zmin = r->features ? q.zmin : nullptr;
+        d.stats = b->item_stats + (size_t)i * ST_WORDS;
+        d.sub_count = S;
+        d.tiles = tiles;
+        d.first_seed = it.first_seed;
+        d.mk = (uint32_t)fused_mk(it.params);
+        mk_used[d.mk] = true;
+        const size_t l = 256 * (4 * sizeof(float4) + (d.f.sh_degree > 0 ? 3 * d.f.sh_stride * sizeof(float) : 0));
+        lds = std::max(lds, l);
+    }
+    // block tables: view (per item ceil(N / 256)), resolve (ceil(X / 256)), count-scan tiles per round, and the
+    // particle grid's assignment (written on the device)
+    GpsRenderer *r0 = items[0].renderer;
+    const uint32_t grid = std::max<uint32_t>((uint32_t)grid_for_particles(r0), n);
+    std::vector<uint32_t> tab;
+    auto blocks_of = [&](auto per_item, std::vector<uint32_t> &blk0) {
+        std::vector<uint32_t> bi;
+        for (uint32_t i = 0; i < n; i++) {
+            blk0.push_back((uint32_t)bi.size());
+            const uint64_t nb = per_item(i);
+            for (uint64_t k = 0; k < nb; k++) bi.push_back(i);
+        }
+        return bi;
+    };
+    std::vector<uint32_t> v0, x0;
+    std::vector<uint32_t> vb = blocks_of([&](uint32_t i) { return (items[i].renderer->n + 255) / 256; }, v0);
+    std::vector<uint32_t> xb = blocks_of([&](uint32_t i) {
+        return ((uint64_t)items[i].renderer->W * items[i].renderer->H + 255) / 256; }, x0);
+    // per round: each item's ensembles (0 once it has done its count) and the scan's (item, ensemble) tiles
+    std::vector<uint32_t> ne_all((size_t)rounds * n);
+    std::vector<std::vector<uint32_t>> cs(rounds);
+    for (uint32_t rd = 0; rd < rounds; rd++)
+        for (uint32_t i = 0; i < n; i++) {
+            const uint64_t e0 = (uint64_t)rd * round_ens;
+            const uint32_t ne = items[i].count > e0 ? (uint32_t)std::min<uint64_t>(round_ens, items[i].count - e0) : 0;
+            ne_all[(size_t)rd * n + i] = ne;
+            for (uint32_t e = 0; e < ne; e++)
+                for (uint32_t t = 0; t < fi[i].tiles; t++) cs[rd].push_back((i << 8) | e);
+        }
+    // layout: vb | v0 | xb | x0 | ne_all | cs[0] | cs[1] ... | blk_item (grid) | blk_rank (grid) | item_nblk (n)
+    auto put = [&](const std::vector<uint32_t> &v) {
+        const uint64_t at = tab.size();
+        tab.insert(tab.end(), v.begin(), v.end());
+        return at;
+    };
+    const uint64_t o_vb = put(vb), o_v0 = put(v0), o_xb = put(xb), o_x0 = put(x0), o_ne = put(ne_all);
+    std::vector<uint64_t> o_cs(rounds);
+    for (uint32_t rd = 0; rd < rounds; rd++) o_cs[rd] = put(cs[rd]);
+    const uint64_t o_bi = tab.size(), o_br = o_bi + grid, o_nb = o_br + grid;
+    tab.resize(o_nb + n, 0u);
+    if (!grow(b->tables, b->tables_cap, tab.size())) {
+        g_create_error = "out of device memory for a fused batch";
+        return -1;
+    }
+    if (b->fitems_cap < n) {
+        if (b->fitems) (void)hipFree(b->fitems);
+        b->fitems = nullptr;
+        b->fitems_cap = 0;
+        if (hipMalloc(&b->fitems, (size_t)n * sizeof(FusedItem)) != hipSuccess) {
+            g_create_error = "out of device memory for a fused batch";
+            return -1;
+        }
+        b->fitems_cap = n;
+    }
+    if (hipMemcpyAsync(b->tables, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->fitems, fi.data(), (size_t)n * sizeof(FusedItem), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
+        g_create_error = "fused batch upload failed";
+        return -1;
+    }
+    uint32_t *T = b->tables;
+    auto t0 = std::chrono::steady_clock::now();
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph && hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        g_create_error = "hipStreamBeginCapture failed";
+        return -1;
+    }
+    for (uint32_t i = 0; i < n; i++) b->fbuf[i].depth_clean = false;   // until the batch has finished
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    if (!use_graph) (void)hipEventRecord(e0, b->stream);
+    hipStream_t st = b->stream;
+    launch_fused_clear(st, b->fitems, n, (uint32_t)max_x);
+    launch_fused_view(st, b->fitems, T + o_vb, T + o_v0, (uint32_t)vb.size(), lds);
+    for (uint32_t rd = 0; rd < rounds; rd++) {
+        const uint32_t *ne = T + o_ne + (size_t)rd * n;
+        const uint32_t ens0 = rd * round_ens;   // seeds are first_seed + ensemble modulo 2^32
+        launch_fused_count_scan(st, b->fitems, T + o_cs[rd], (uint32_t)cs[rd].size(), ens0, b->epoch);
+        b->epoch = (b->epoch + 1u) & ((1u << 26) - 1u);
+        if (b->epoch == 0) b->epoch = 1;
+        for (uint32_t mk = 0; mk < 3; mk++) {
+            if (!mk_used[mk]) continue;
+            launch_fused_assign(st, b->fitems, n, ne, mk, chunk, grid, T + o_bi, T + o_br, T + o_nb);
+            launch_fused_particles(st, b->fitems, mk, ne, ens0, T + o_bi, T + o_br, T + o_nb, (int)grid, ppt, block);
+        }
+        launch_fused_resolve(st, b->fitems, ne, T + o_xb, T + o_x0, (uint32_t)xb.size());
+    }
+    int rc = hipGetLastError() == hipSuccess ? 0 : -1;
+    if (!use_graph) (void)hipEventRecord(e1, b->stream);
+    b->nodes = 0;
+    if (use_graph) {
+        if (hipStreamEndCapture(b->stream, &graph) != hipSuccess) rc = -1;
+        if (rc == 0 && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = -1;
+        size_t nn = 0;
+        if (rc == 0 && hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess) b->nodes = nn;
+    }
+    b->record_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == 0 && use_graph) {
+        (void)hipEventRecord(e0, b->stream);
+        if (hipGraphLaunch(exec, b->stream) != hipSuccess) rc = -1;
+        (void)hipEventRecord(e1, b->stream);
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) rc = -1;
+    if (e0) {
+        float ms = 0.0f;
+        if (rc == 0) (void)hipEventElapsedTime(&ms, e0, e1);
+        b->gpu_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != 0) {
+        if (g_create_error.empty()) g_create_error = "fused batch failed";
+        return -1;
+    }
+    for (uint32_t i = 0; i < n; i++) b->fbuf[i].depth_clean = true;   // every round's resolve reset them
+    // each renderer continues the history of its last item: image, counters, denoiser moments
+    for (uint32_t i = 0; i < n; i++) {
+        GpsRenderer *r = items[i].renderer;
+        bool last = true;
+        for (uint32_t k = i + 1; k < n && last; k++) last = items[k].renderer != r;
+        if (!last) continue;
+        const uint64_t X = (uint64_t)r->W * r->H;
+        if (hipMemcpyAsync(r->accum, b->outs[i], X * sizeof(float4), hipMemcpyDeviceToDevice, b->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(r->stats, b->item_stats + (size_t)i * ST_WORDS, ST_WORDS * 8, hipMemcpyDeviceToDevice,
+                           b->stream) != hipSuccess)
+            rc = -1;
+        if (r->features && (hipMemcpyAsync(r->mom, b->fbuf[i].mom, X * sizeof(float4), hipMemcpyDeviceToDevice,
+                                           b->stream) != hipSuccess ||
+                            hipMemcpyAsync(r->zmin, b->fbuf[i].zmin, X * sizeof(float), hipMemcpyDeviceToDevice,
+                                           b->stream) != hipSuccess))
+            rc = -1;
+        r->accumulated = items[i].count;
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) rc = -1;
+    if (rc != 0) g_create_error = "fused batch copy-back failed";
+    return rc;
 }
 
 int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use_graph) {
@@ -1088,10 +1381,21 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
             b->out_px[i] = X;
         }
     }
+    // every item on the particle path of a non-empty scene, without the debug hook: the fused kernels
+    bool fusable = (use_graph & GPS_BATCH_PER_ITEM) == 0 && n <= (uint32_t)FUSED_MAX_ITEMS;
+    for (uint32_t i = 0; i < n && fusable; i++) {
+        const GpsRenderer *r = items[i].renderer;
+        if (fused_mk(items[i].params) < 0 || r->n == 0 || (r->debug & GPS_DEBUG_KEEP_BUFFERS)) fusable = false;
+    }
+    if (fusable) {
+        g_create_error.clear();
+        return fused_batch_render(b, items, n, (use_graph & GPS_BATCH_GRAPH) != 0);
+    }
+    const bool graph_on = (use_graph & GPS_BATCH_GRAPH) != 0;
     auto t0 = std::chrono::steady_clock::now();
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    if (use_graph && hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    if (graph_on && hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
         g_create_error = "hipStreamBeginCapture failed";
         return -1;
     }
@@ -1140,7 +1444,7 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
             rc = -1;
     if (rc != 0 && g_create_error.empty()) g_create_error = "batch recording failed";
     b->nodes = 0;
-    if (use_graph) {
+    if (graph_on) {
         if (hipStreamEndCapture(b->stream, &graph) != hipSuccess) rc = -1;
         if (rc == 0 && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = -1;
         size_t nn = 0;
@@ -1150,7 +1454,7 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
     auto t1 = std::chrono::steady_clock::now();
     b->record_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (rc == 0 && use_graph) {
+    if (rc == 0 && graph_on) {
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
         (void)hipEventRecord(e0, b->stream);
@@ -1173,8 +1477,10 @@ int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use
 }
 
 int gps_batch_read(GpsBatch *b, uint32_t item, float *out) {
-    if (item >= b->outs.size() || !b->outs[item]) return -1;
-    if (hipMemcpy(out, b->outs[item], b->out_px[item] * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    // the last render's item size: an output buffer keeps the largest size it has held
+    if (item >= b->item_w.size() || item >= b->outs.size() || !b->outs[item]) return -1;
+    const uint64_t X = (uint64_t)b->item_w[item] * b->item_h[item];
+    if (hipMemcpy(out, b->outs[item], X * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess) return -1;
     return 0;
 }
 

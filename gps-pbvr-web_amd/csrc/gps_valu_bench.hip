@@ -86,12 +86,12 @@ extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int rep
     uint32_t *buf = nullptr;
     if (hipMalloc(&buf, (size_t)blocks * 256 * 4) != hipSuccess) return -2;
     hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
     int rc = 0;
     float total = 0.0f;
     for (int r = 0; r <= reps && rc == 0; r++) {   // launch 0 warms up
-        hipEventRecord(e0, nullptr);
+        (void)hipEventRecord(e0, nullptr);
         if (mode == 0) hipLaunchKernelGGL(k_valu<0>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 1) hipLaunchKernelGGL(k_valu<1>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 2) hipLaunchKernelGGL(k_valu<2>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
@@ -100,16 +100,16 @@ extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int rep
         else if (mode == 5) hipLaunchKernelGGL(k_valu<5>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 6) hipLaunchKernelGGL(k_valu<6>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else hipLaunchKernelGGL(k_valu<7>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
-        hipEventRecord(e1, nullptr);
+        (void)hipEventRecord(e1, nullptr);
         if (hipEventSynchronize(e1) != hipSuccess) rc = -2;
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         if (r > 0) total += ms;
     }
     if (hipGetLastError() != hipSuccess) rc = -2;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(buf);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
     *out_ms = total / (float)reps;
     return rc;
 }

@@ -476,8 +476,12 @@ class GpsRenderer:
         return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(min(n, 64))}
 
 
+BATCH_GRAPH, BATCH_PER_ITEM = 1, 2
+
+
 class Batch:
-    """Many (renderer, view, ensembles) items recorded on one stream and launched as one hipGraph."""
+    """Many (renderer, view, ensembles) items through one set of fused kernels (or each item's own launches with
+    per_item), captured into one hipGraph with use_graph."""
 
     def __init__(self, device=0):
         h = C.c_void_p()
@@ -496,7 +500,7 @@ class Batch:
         except Exception:
             pass
 
-    def render(self, items, use_graph=True):
+    def render(self, items, use_graph=True, per_item=False):
         """items: list of (GpsRenderer, GpsCamera, RenderParams, first_seed, count)."""
         self._keep = [(cam, p) for _, cam, p, _, _ in items]
         arr = (BatchItem * len(items))()
@@ -507,7 +511,8 @@ class Batch:
             arr[i].first_seed = seed & 0xFFFFFFFF
             arr[i].count = count
         self._sizes = [r.size() for r, _, _, _, _ in items]
-        if lib.gps_batch_render(self._h, arr, len(items), 1 if use_graph else 0) != 0:
+        flags = (BATCH_GRAPH if use_graph else 0) | (BATCH_PER_ITEM if per_item else 0)
+        if lib.gps_batch_render(self._h, arr, len(items), flags) != 0:
             raise GpsError(lib.gps_renderer_last_error(None).decode())
 
     def read(self, i):

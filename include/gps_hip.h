@@ -261,11 +261,22 @@ int gps_denoiser_loss_grad(GpsDenoiser *d, uint32_t batch, float *loss, float *g
 int gps_denoiser_read_output(GpsDenoiser *d, uint32_t images, float *rgba); /* network output of the last step */
 
 /* ---- batches of views (no counterpart in the reference: its trajectory export renders one view per frame loop,
- * web/index.html:445-469).  Each item resets its renderer's accumulation buffer and counters, renders
- * `count` ensembles of one view and snapshots the image and the counters into the batch's slot `i`.  Items
- * of different renderers are recorded on up to four forked streams (parallel branches of the graph); with
- * use_graph the batch is captured into one hipGraph and launched once.  Items may share a renderer: a later
- * item overwrites the earlier one's history but not its snapshot.  The sorted particle path is not batchable. */
+ * web/index.html:445-469).  Each item renders `count` ensembles of one view (a renderer's scene, camera and
+ * parameters) from an empty image and zero counters into the batch's slot `i`; afterwards each renderer holds
+ * the image, counters and ensemble count of its last item, and continues that history exactly.
+ *
+ * Fused (the default for items on the particle path of non-empty scenes without the debug hook, up to 1024
+ * items): every item gets buffers of its own in the batch, and the whole batch runs through one set of
+ * kernels: a clear, one view kernel for all items, then per round of ensembles (the tuning's batch_ensembles,
+ * or as many as keep the largest item's depth|id buffers within depth_window_mib) one count-and-scan launch over
+ * every (item, ensemble), per emitter class one assignment of the particle grid to the items in proportion to
+ * their particles and one particle launch, and one resolve launch over every item's pixels.  With
+ * GPS_BATCH_GRAPH these launches are captured into one hipGraph (a few nodes per round) and launched once.
+ *
+ * GPS_BATCH_PER_ITEM records each item's own launches instead (the single-view kernels), items of different
+ * renderers on up to four forked streams (parallel branches of the graph); it is also taken when an item
+ * cannot be fused.  Results are bit-identical either way.  The sorted particle path is not batchable. */
+enum { GPS_BATCH_GRAPH = 1, GPS_BATCH_PER_ITEM = 2 };
 typedef struct GpsBatch GpsBatch;
 typedef struct GpsBatchItem {
     GpsRenderer *renderer;
@@ -276,7 +287,8 @@ typedef struct GpsBatchItem {
 } GpsBatchItem;
 int gps_batch_create(int device, GpsBatch **out);
 void gps_batch_destroy(GpsBatch *b);
-int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int use_graph);
+/* flags: GPS_BATCH_GRAPH and / or GPS_BATCH_PER_ITEM (0: the fused launches straight onto the batch's stream) */
+int gps_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n, int flags);
 /* w*h*4 floats of item `item`'s accumulation snapshot */
 int gps_batch_read(GpsBatch *b, uint32_t item, float *out);
 /* item `item`'s counters: every item also resets its renderer's counters before rendering */
@@ -284,7 +296,8 @@ int gps_batch_read_stats(GpsBatch *b, uint32_t item, GpsStats *out);
 /* composite (gps.rs:967-986, as gps_renderer_composite) of item `item`'s snapshot into out_w*out_h RGBA8 */
 int gps_batch_composite(GpsBatch *b, uint32_t item, const GpsRenderParams *p, uint32_t out_w, uint32_t out_h,
                         uint8_t *rgba8);
-/* GPU time of the last gps_batch_render (graph launch to completion), host time to record it, graph nodes */
+/* GPU time of the last gps_batch_render (graph launch, or the fused launches, to completion; the host's time for
+ * per-item launches without a graph), host time to record it, graph nodes */
 void gps_batch_timing(const GpsBatch *b, double *gpu_ms, double *record_ms, uint64_t *graph_nodes);
 
 /* ---- 3DGS PLY ingest (crates/gps-io/src/lib.rs:1-373; the activation of crates/gps-render/src/gps.rs:41-49).

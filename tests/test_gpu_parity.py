@@ -675,17 +675,37 @@ def test_batch_graph_equals_separate_renders():
         r.render_ensembles(cam, p, seed, count)
         expect.append(r.read_accum())
         expect_st.append(r.read_stats())
+    rb.set_features(True)
+    rb.reset_accum()
+    rb.render_ensembles(*items[3][1:])
+    expect_mom, expect_z = rb.read_features()
     b = gh.Batch()
-    for use_graph in (True, False, True):
-        say(f"[batch] graph={use_graph}")
-        b.render(items, use_graph=use_graph)
-        for i in range(len(items)):
-            assert np.array_equal(b.read(i).view(np.uint32), expect[i].view(np.uint32)), i
-            assert b.read_stats(i) == expect_st[i], i
-        t = b.timing()
-        assert t["gpu_ms"] > 0
-        if use_graph:
-            assert t["graph_nodes"] > len(items)
+    saved = gh.get_tuning()
+    try:
+        # fused (all items in a few launches per round of ensembles) and per item, with and without a graph, with
+        # the tuning's round size (one round) and with 16 ensembles per round (5 rounds; items of 3-5 ensembles
+        # have none after the first)
+        for round_ens, use_graph, per_item in ((0, True, False), (0, False, False), (16, True, False),
+                                               (16, False, False), (0, True, True), (0, False, True),
+                                               (0, True, False)):
+            gh.set_tuning(batch_ensembles=round_ens)
+            say(f"[batch] round_ens={round_ens} graph={use_graph} per_item={per_item}")
+            b.render(items, use_graph=use_graph, per_item=per_item)
+            for i in range(len(items)):
+                assert np.array_equal(b.read(i).view(np.uint32), expect[i].view(np.uint32)), i
+                assert b.read_stats(i) == expect_st[i], i
+            mom, z = rb.read_features()
+            assert np.array_equal(mom.view(np.uint32), expect_mom.view(np.uint32))
+            assert np.array_equal(z.view(np.uint32), expect_z.view(np.uint32))
+            t = b.timing()
+            assert t["gpu_ms"] > 0
+            if use_graph and per_item:
+                assert t["graph_nodes"] > len(items)
+            elif use_graph and round_ens == 16:
+                # clear, view, then per round: count-scan, (assign, particles) for emitter classes 0 and 1, resolve
+                assert t["graph_nodes"] == 2 + 5 * (2 + 2 * 2), t
+    finally:
+        gh.set_tuning(**saved)
     # after a batch the renderer carries the last item's history and continues it exactly
     rb.render_ensembles(S.clustered_camera("far", 160, 90), pb, 77, 2)
     rc = gh.GpsRenderer(160, 90, 1)
