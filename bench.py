@@ -160,7 +160,11 @@ def bench_valu(a):
     blocks, iters, reps = 65536, 256, 5
     rows = []
     for mode, name in ((0, "fma_f32"), (1, "pcg_hash"), (2, "add_xor_shift"), (3, "add_u32"), (4, "xor_b32"),
-                       (5, "mul_lo_u32"), (6, "add_f32"), (7, "fma_f32+add_u32")):
+                       (5, "mul_lo_u32"), (6, "add_f32"), (7, "fma_f32+add_u32"), (8, "lshrrev_const"),
+                       (9, "lshrrev_vgpr"), (10, "bfe_u32"), (11, "alignbit_b32"), (12, "lshl_add_u32"),
+                       (13, "xad_u32"), (14, "mad_u32_u24"), (15, "mul_hi_u32"), (16, "lshl_or_b32"),
+                       (17, "xor_b32+lshrrev_const"), (18, "pcg_hash_x2_asm"), (19, "pcg_hash_x4_asm"),
+                       (20, "pcg_hash_x2_asm_literals")):
         ms = ctypes.c_float()
         rc = lib.gps_valu_bench(mode, blocks, iters, reps, ctypes.byref(ms))
         if rc != 0:

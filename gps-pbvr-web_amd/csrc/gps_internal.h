@@ -31,9 +31,8 @@ enum : uint32_t { KIND_NONE = 0, KIND_POISSON = 1, KIND_FIXED = 2, KIND_UNDERCOV
 enum : int {
     ST_POINTS = 0, ST_TRUNC = 1, ST_DROPPED = 2, ST_VISIBLE = 3, ST_ORPHAN = 4, ST_CAND = 5, ST_UNDER = 6,
     ST_SKIPPED = 7, ST_ENSEMBLES = 8, ST_PROBE = 9,
-    // high-water marks of the particle kernel's count-only LDS buffers since the last stats reset: the emit_pbvr
-    // queue's fill (capacity particle_block x (particles_per_thread + 1)) and a wave's count_only work buffer
-    // (capacity 64 x (particles_per_thread + 1))
+    // high-water marks of the particle kernel's count-only LDS buffers since the last stats reset: a wave's
+    // emit_pbvr queue and a wave's count_only work buffer (capacity PARTICLE_WAVE_CAP each)
     ST_QPEAK = 10, ST_WPEAK = 11,
     // particles of the particle kernel that reached a subpixel (the packed minimum or a sorted key on screen)
     ST_HITS = 12,
@@ -41,6 +40,7 @@ enum : int {
 };
 
 constexpr int VIEW_VEC4 = 6;          // float4 per splat in the view cache
+constexpr uint32_t PARTICLE_WAVE_CAP = 128;   // count-only work records / emit_pbvr entries per wave (LDS)
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;

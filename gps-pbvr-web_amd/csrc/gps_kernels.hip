@@ -48,6 +48,21 @@ GPS_DEV uint32_t particle_seed(uint32_t splat, uint32_t k, uint32_t frame) {
     return s ? s : 1u;
 }
 
+// The input of pcg for its output: each step is a bijection of u32 (the multiplications by their inverses modulo
+// 2^32; the xorshift by r = (state >> 28) + 4 >= 4 leaves the top four bits, which give r back, and is undone by
+// xoring the shifts by r, 2r, ...).
+GPS_DEV uint32_t pcg_inv(uint32_t out) {
+    const uint32_t w = out ^ (out >> 22u);
+    const uint32_t x = w * 0xACB86D69u;   // 277803737^-1 mod 2^32
+    const uint32_t r = (x >> 28u) + 4u;
+    uint32_t s = x;
+    for (uint32_t sh = r; sh < 32u; sh += r) s ^= x >> sh;
+    return (s - 2891336453u) * 0xCC2BFE9Du;   // 747796405^-1 mod 2^32
+}
+
+// Rng::next's uniform for the PCG output c
+GPS_DEV float unif24(uint32_t c) { return (float)(c >> 8u) * (1.0f / 16777216.0f); }
+
 // particle_seed with the ensemble term frame * 2654435761 + 1 computed once per ensemble
 GPS_DEV uint32_t particle_seed_mixed(uint32_t splat, uint32_t k, uint32_t frame_mix) {
     uint32_t s = pcg(splat * 747796405u + k + 1u);
@@ -62,7 +77,7 @@ struct Rng {
     GPS_DEV explicit Rng(uint32_t seed) : s(seed), spare(0.0f), has_spare(false) {}
     GPS_DEV float next() {
         s = pcg(s);
-        return (float)(s >> 8u) * (1.0f / 16777216.0f);
+        return unif24(s);
     }
     // gps_splat.wgsl:402-414 (Box-Muller, sine half kept)
     GPS_DEV float normal() {
@@ -875,9 +890,13 @@ GPS_DEV Hit emit_occupancy(const GpsFrame &f, Rng &r, uint32_t k, const float4 *
 
 // Count-only evaluation (gps_fastcount.h) of a particle of a splat that the view kernel marked off-screen
 // (v[3].w != 0; no particle of it can write a subpixel): 1 or 0 when emit_pbvr's generated-points decision is
-// certain, -1 when it is not and the particle must go through emit_pbvr.  It draws the uniforms in emit_pbvr's
-// order (two Box-Muller pairs for the three normals, the view-conditioned draw, the jitter normals: the second
-// pair's sine and a third pair, the radial keep draw).  Each decision compares the count-only value with a bound on its
+// certain, -1 when it is not and the particle must go through emit_pbvr.  Its uniforms are emit_pbvr's, taken
+// from the particle's PCG chain as the caller drew it: c1..c4 the two Box-Muller pairs of the three normals;
+// then, after the view-conditioned draw (which the caller took: only particles it keeps come here), the jitter
+// normals (the second pair's sine and a third pair, c5 c6) and the radial keep draw ck.  For a splat whose
+// particles all pass the near plane (v[3].w < 0) the caller drew the whole chain and t0, t1, t2 are c5, c6, ck;
+// otherwise t0 is the state they follow from, hashed here only when the particle passes the near plane.
+// Each decision compares the count-only value with a bound on its
 // difference from emit_pbvr's: camz against near_z with margin bnd_z (v[1].z, cam_box); with the radial
 // correction, the keep ratio K = clamp(h(min(o e^(-r2/2), 1 - 1e-6)) / d.z, 0, 1) against the keep draw with
 // margin dk.  dk collects the pixel difference (bx, by: camera bounds bnd_x, bnd_y, bnd_z through
@@ -885,15 +904,15 @@ GPS_DEV Hit emit_occupancy(const GpsFrame &f, Rng &r, uint32_t k, const float4 *
 // this path), the jitter and dv), the r2 difference through the conic (10 EPS of its absolute terms for the
 // roundings of both evaluations), its effect on e^(-r2/2) and on h (h' <= 1/(1-x)), and the relative errors of
 // both evaluations of e^(-r2/2), h and the divisions.
-GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) {
+GPS_DEV int count_only(const GpsFrame &f, const float4 *__restrict__ v, uint32_t c1, uint32_t c2, uint32_t c3,
+                       uint32_t c4, uint32_t t0, uint32_t t1, uint32_t t2) {
     // Fused multiply-adds only remove roundings from this path's arithmetic, which every bound below counts
     // as separate operations; the measured gpsf functions are defined outside this scope and stay unfused.
 #pragma clang fp contract(fast)
     using namespace gpsf;
-    const float u1 = fmax(r.next(), 5.96e-8f), u2 = r.next();
-    const float u3 = fmax(r.next(), 5.96e-8f), u4 = r.next();
+    const float u1 = fmax(unif24(c1), 5.96e-8f), u2 = unif24(c2);
+    const float u3 = fmax(unif24(c3), 5.96e-8f), u4 = unif24(c4);
     const float4 b = v[1];
-    if (f.method == M_VIEW_CONDITIONED && r.next() >= b.w) return 0;
     const float r1 = bm_radius(u1), r3 = bm_radius(u3);
     const float uz = r1 * cos_turn(u2), uy = r1 * sin_turn(u2), ux = r3 * cos_turn(u4);
     const float4 c = v[2], ca = v[4], cb = v[5];
@@ -905,6 +924,16 @@ GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) 
     const float dz = camz - f.near_z, mz = 1.001f * b.z;
     if (!(dz > mz)) return dz <= -mz ? 0 : -1;
     if (!(f.flags & FLAG_RADIAL)) return 1;
+    uint32_t c5 = t0, c6 = t1, ck = t2;   // the rest of the chain
+    if (!(v[3].w < 0.0f)) {
+        uint32_t c = t0;
+        if (f.flags & FLAG_JITTER) {
+            c5 = pcg(c);
+            c6 = pcg(c5);
+            c = c6;
+        }
+        ck = pcg(c);
+    }
     const float zl = camz - b.z;   // <= camz on both paths
     if (!(zl > 0.0f)) return -1;
     const float camx = dot3(f.rot[0], px, py, pz), camy = dot3(f.rot[1], px, py, pz);
@@ -916,7 +945,7 @@ GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) 
     float jb = 0.0f;
     if (f.flags & FLAG_JITTER) {
         const float jy = r3 * sin_turn(u4);
-        const float u5 = fmax(r.next(), 5.96e-8f), u6 = r.next();
+        const float u5 = fmax(unif24(c5), 5.96e-8f), u6 = unif24(c6);
         const float jx = bm_radius(u5) * cos_turn(u6);
         pixx = pixx + f.jitter * jx;
         pixy = pixy + f.jitter * jy;
@@ -956,27 +985,10 @@ GPS_DEV int count_only(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) 
     const float eta = fmin(ETA_HE, ETA_HE_X * rx) + fmin(ETA_HF, ETA_HF_X * rx);
     const float hu = h * (1.0f + 2.0f * ETA_HF) + dh;        // >= h at both points
     const float dk = (dh + (eta + 2.0f * EPS + E_RCP) * hu) * rdz * 1.01f + 4.0f * EPS;
-    const float uk = r.next();
+    const float uk = unif24(ck);
     if (uk < kf - dk) return 1;
     if (uk >= kf + dk) return 0;
     return -1;
-}
-
-// The count of a particle of a splat whose particles all pass the near plane (v[3].w < 0) from its draws alone,
-// without the arithmetic: 0 when the view-conditioned draw rejects; 1 when there is no radial correction, or
-// the keep draw is below -v[3].w, a lower bound of every particle's keep ratio; -1 when count_only must decide.
-// The draws follow emit_particle_pbvr's order (gps_splat.wgsl:490-532): two normal pairs, the view-conditioned
-// draw, a third pair for the jitter (jy is the second pair's spare, jx starts the third), the radial keep draw.
-GPS_DEV int keep_draw(const GpsFrame &f, uint32_t seed, const float4 *__restrict__ v) {
-    uint32_t c = pcg(pcg(pcg(pcg(seed))));   // the two normal pairs
-    if (f.method == M_VIEW_CONDITIONED) {
-        c = pcg(c);
-        if ((float)(c >> 8u) * (1.0f / 16777216.0f) >= v[1].w) return 0;
-    }
-    if (!(f.flags & FLAG_RADIAL)) return 1;
-    if (f.flags & FLAG_JITTER) c = pcg(pcg(c));
-    c = pcg(c);
-    return (float)(c >> 8u) * (1.0f / 16777216.0f) < -v[3].w ? 1 : -1;
 }
 
 // OUT = 0: packed 64-bit minimum into the depth|id buffer of the particle's ensemble.
@@ -995,29 +1007,30 @@ GPS_DEV void particles_body(const GpsFrame &f, const float4 *__restrict__ view, 
                             uint64_t *__restrict__ stats, uint64_t *__restrict__ pkeys, uint32_t *__restrict__ pidx,
                             const uint32_t *__restrict__ perm, uint32_t rank, uint32_t nblocks) {
     constexpr uint32_t CH = BS * PPT;
-    // PBVR packed-minimum path: a particle of a splat marked off-screen is settled by keep_draw when its draws
-    // alone decide; otherwise (splat, particle seed) goes to its wave's buffer, which after each chunk runs
-    // count_only on whole waves of 64 entries.  Entries count_only cannot decide, and the buffer's last
-    // remainder, are queued for emit_pbvr, which the whole block runs once the queue may not hold another
-    // chunk's worth, and at the end.
+    // PBVR packed-minimum path: a particle of a splat marked off-screen has its PCG chain drawn here; it is
+    // settled when the draws alone decide (the view-conditioned draw rejects, or, for a splat whose particles all
+    // pass the near plane, there is no radial correction or the keep draw is below the splat's lower bound of the
+    // keep ratio, -v[3].w).  Otherwise its record (splat and chain) goes to its wave's work buffer, which runs
+    // count_only on whole waves of 64 records as soon as it holds 64; records count_only cannot decide go to
+    // the wave's emit_pbvr queue, run on whole waves of 64 the same way.  The remainders run at the end.
     constexpr bool COUNT_ONLY = MK == 1 && OUT == 0;
-    constexpr uint32_t QCAP = COUNT_ONLY ? CH + BS : 1u;   // a chunk appends at most CH; flushed above BS
+    constexpr uint32_t NWAVE = BS / 64;
+    constexpr uint32_t WCAP = COUNT_ONLY ? PARTICLE_WAVE_CAP : 1u;   // per wave: fewer than 64 waiting + 64 appended
     constexpr int NV = MK == 0 ? 2 : (MK == 2 ? 4 : VIEW_VEC4);   // view record float4s the emitter reads
     __shared__ uint32_t s_cstart[MAX_BATCH + 1];
     constexpr uint32_t OCAP = 256u;   // chunk splat ranges up to this many positions are staged in LDS
     constexpr uint32_t CRUN = 8u;     // consecutive chunks per block turn
-    __shared__ uint32_t s_lo, s_hi, s_qn;
-    __shared__ uint32_t s_qpeak, s_wpeak;   // high-water marks of s_qn and of a wave's rn (ST_QPEAK, ST_WPEAK)
+    __shared__ uint32_t s_lo, s_hi;
+    __shared__ uint32_t s_qpeak, s_wpeak;   // high-water marks of a wave's queue and work buffer (ST_QPEAK, ST_WPEAK)
     __shared__ uint32_t s_off[OCAP + 1], s_prm[OCAP];
-    __shared__ uint64_t s_pts[BS / 64], s_hits[BS / 64];
-    __shared__ uint2 s_q[QCAP];
-    constexpr uint32_t WCAP = 64u * (PPT + 1u);   // per-wave count_only work: fewer than 64 left + PPT x 64 appended
-    __shared__ uint2 s_work[COUNT_ONLY ? BS / 64 : 1][COUNT_ONLY ? WCAP : 1];
+    __shared__ uint64_t s_pts[NWAVE], s_hits[NWAVE];
+    // work records: (splat, c1, c2, c3) and (c4, t0, t1, t2) of count_only; queue entries: (splat, seed)
+    __shared__ uint4 s_wa[COUNT_ONLY ? NWAVE : 1][WCAP], s_wb[COUNT_ONLY ? NWAVE : 1][WCAP];
+    __shared__ uint2 s_eq[COUNT_ONLY ? NWAVE : 1][WCAP];
     static_assert(PPT == 1 || PPT == 2 || PPT == 4 || PPT == 8, "particles per thread");
     static_assert(BS == 128 || BS == 256 || BS == 512, "workgroup size");
     const bool count_only_on = COUNT_ONLY && (f.flags & FLAG_FAST) != 0u;
     if (threadIdx.x == 0) {
-        s_qn = 0u;
         s_qpeak = 0u;
         s_wpeak = 0u;
         uint32_t acc = 0;
@@ -1034,17 +1047,65 @@ GPS_DEV void particles_body(const GpsFrame &f, const float4 *__restrict__ view, 
     // write a subpixel in the high 32 bits; a thread's share of a launch stays below 2^32 particles (at most
     // 64 ensembles of fewer than 2^32 particles over at least 32,768 threads)
     uint64_t pts = 0;
-    // queued (splat, particle seed) entries, all of splats marked off-screen: emit_pbvr decides their count, and
-    // none of their particles can reach a subpixel (the box test that marked the splat), so nothing is written
-    auto flush = [&](uint32_t nq) {
-        for (uint32_t t = threadIdx.x; t < nq; t += BS) {
-            const uint2 e = s_q[t];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    // this wave's pending count_only records [0, rn) and emit_pbvr entries [0, qn) (the same in every lane)
+    uint32_t rn = 0, qn = 0;
+    // LDS written by other lanes of the wave is read after this
+    auto wave_sync = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // the first n entries of the wave's queue through emit_pbvr, one per lane; the rest (n = 64) to the front.
+    // All are of splats marked off-screen: emit_pbvr decides their count, and none of their particles can reach
+    // a subpixel (the box test that marked the splat), so nothing is written.
+    auto run_queue = [&](uint32_t n) {
+        wave_sync();
+        if (lane < n) {
+            const uint2 e = s_eq[wave][lane];
             Rng r(e.y);
             pts += emit_pbvr(f, r, view + (size_t)VIEW_VEC4 * e.x).counted;
         }
+        qn -= n;
+        if (lane < qn) s_eq[wave][lane] = s_eq[wave][n + lane];   // n = 64 > qn: no overlap
+        wave_sync();
     };
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t rn = 0;   // this wave's pending count_only work: entries [0, rn) of its buffer (the same in every lane)
+    // the first n records of the work buffer through count_only, one per lane; the rest (n = 64) to the front
+    auto run_work = [&](uint32_t n, float4 *v, uint32_t &vsp) {
+        wave_sync();
+        bool undecided = false;
+        uint32_t qs = 0, qseed = 0;
+        if (lane < n) {
+            const uint4 a = s_wa[wave][lane], b = s_wb[wave][lane];
+            if (a.x != vsp) {
+#pragma unroll
+                for (int q = 0; q < NV; q++) v[q] = view[(size_t)VIEW_VEC4 * a.x + q];
+                vsp = a.x;
+            }
+            const int c = count_only(f, v, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
+            if (c >= 0) {
+                pts += (uint32_t)c;
+            } else {
+                undecided = true;
+                qs = a.x;
+                qseed = pcg_inv(a.y);   // the particle's seed: c1 = pcg(seed)
+            }
+        }
+        rn -= n;
+        if (lane < rn) {   // n = 64 > rn: no overlap
+            s_wa[wave][lane] = s_wa[wave][n + lane];
+            s_wb[wave][lane] = s_wb[wave][n + lane];
+        }
+        const uint64_t m = __ballot(undecided);
+        if (undecided) {
+            const uint32_t k = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            s_eq[wave][qn + k] = make_uint2(qs, qseed);
+        }
+        qn += (uint32_t)__popcll(m);
+        if (lane == 0u && qn > 0u) atomicMax(&s_qpeak, qn);
+        wave_sync();
+        if (qn >= 64u) run_queue(64u);
+    };
     uint32_t prev = 0xFFFFFFFFu, prev_hi = 0;   // the block's previous chunk and the owner of its last particle
     // runs of CRUN consecutive chunks per block: a chunk's owners are searched from the previous chunk's last
     for (uint32_t chunk = rank * CRUN; chunk < nchunks;
@@ -1159,7 +1220,7 @@ GPS_DEV void particles_body(const GpsFrame &f, const float4 *__restrict__ view, 
             for (uint32_t j = 0; j < PPT; j++) {   // every lane runs PPT iterations (the work buffer needs whole waves)
                 const uint32_t p = pfirst + j;
                 bool push = false;
-                uint32_t pseed = 0;
+                uint32_t c1, c2, c3, c4, t0, t1, t2;   // the record; set on every path that pushes it
                 if (q0 + j < cnt) {
                     if (p >= nxt) {   // every particle of the chunk belongs to a position <= s_hi
                         do {
@@ -1173,13 +1234,37 @@ GPS_DEV void particles_body(const GpsFrame &f, const float4 *__restrict__ view, 
                     const uint32_t k = p - cur;
                     const uint32_t seed = particle_seed_mixed(sp, k + 1u, seed_mix);
                     if (count_only_on && v[3].w != 0.0f) {
-                        const int c = v[3].w < 0.0f ? keep_draw(f, seed, v) : -1;
-                        if (c >= 0) {
-                            pts += (uint32_t)c;
-                        } else {
-                            push = true;
-                            pseed = seed;
+                        // the chain of emit_pbvr's draws: c1..c4 the normals, the view-conditioned draw, then (for
+                        // a splat whose particles all pass the near plane, with the radial correction) the jitter
+                        // pair and the keep draw
+                        c1 = pcg(seed);
+                        c2 = pcg(c1);
+                        c3 = pcg(c2);
+                        c4 = pcg(c3);
+                        uint32_t c = c4;
+                        int dec = -1;
+                        if (f.method == M_VIEW_CONDITIONED) {
+                            c = pcg(c);
+                            if (unif24(c) >= v[1].w) dec = 0;
                         }
+                        if (!(v[3].w < 0.0f)) {   // count_only hashes the rest when it needs it
+                            t0 = t1 = t2 = c;
+                        } else if (dec < 0) {
+                            if (!(f.flags & FLAG_RADIAL)) {
+                                dec = 1;
+                            } else {
+                                t0 = t1 = c;
+                                if (f.flags & FLAG_JITTER) {
+                                    t0 = pcg(c);
+                                    t1 = pcg(t0);
+                                    c = t1;
+                                }
+                                t2 = pcg(c);
+                                if (unif24(t2) < -v[3].w) dec = 1;   // below every particle's keep ratio
+                            }
+                        }
+                        if (dec >= 0) pts += (uint32_t)dec;
+                        else push = true;
                     } else {
                         Rng r(seed);
                         emit_out(r, p, k, sp, v);
@@ -1188,58 +1273,24 @@ GPS_DEV void particles_body(const GpsFrame &f, const float4 *__restrict__ view, 
                 if (count_only_on) {   // every lane of the wave is here
                     const uint64_t m = __ballot(push);
                     if (push) {
-                        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        s_work[wave][rn + rank] = make_uint2(sp, pseed);
+                        const uint32_t at = rn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        s_wa[wave][at] = make_uint4(sp, c1, c2, c3);
+                        s_wb[wave][at] = make_uint4(c4, t0, t1, t2);
                     }
-                    rn += (uint32_t)__popcll(m);
+                    rn += (uint32_t)__popcll(m);   // at most 63 + 64 < WCAP
+                    if (lane == 0u && rn > 0u) atomicMax(&s_wpeak, rn);
+                    if (rn >= 64u) run_work(64u, v, vsp);
                 }
             }
-            if (count_only_on && lane == 0u) atomicMax(&s_wpeak, rn);
-            if (count_only_on && rn >= 64u) {   // whole-wave rounds of count_only, then the rest to the front
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                uint32_t base = 0;
-                for (; base + 64u <= rn; base += 64u) {
-                    const uint2 e = s_work[wave][base + lane];
-                    load_view(e.x);
-                    Rng r(e.y);
-                    const int c = count_only(f, r, v);
-                    if (c >= 0) pts += (uint32_t)c;
-                    else s_q[atomicAdd(&s_qn, 1u)] = e;
-                }
-                rn -= base;   // fewer than 64 left, at [base, base + rn); base >= 64 > rn, so the copy does not overlap
-                if (lane < rn) s_work[wave][lane] = s_work[wave][base + lane];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
         }
-        __syncthreads();
-        if (COUNT_ONLY) {
-            const uint32_t nq = s_qn;   // the same for every thread: read after the barrier
-            if (threadIdx.x == 0 && nq > s_qpeak) s_qpeak = nq;
-            if (nq > QCAP - CH) {       // the next chunk could overflow the queue
-                flush(nq);
-                __syncthreads();        // every thread has read s_qn; the next chunk appends after its first barrier
-                if (threadIdx.x == 0) s_qn = 0u;
-            }
-        }
+        __syncthreads();   // the chunk's owners and staged offsets are read by every wave before the next chunk
     }
-    if (COUNT_ONLY) {
-        // the last chunk's flush may have ended with thread 0 resetting s_qn after its barrier: the remainder's
-        // appends below must follow that reset
-        __syncthreads();
-        if (count_only_on && rn) {   // the work buffer's remainder (fewer than 64), one entry per lane
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane < rn) s_q[atomicAdd(&s_qn, 1u)] = s_work[wave][lane];   // to emit_pbvr
-        }
-        __syncthreads();
-        if (threadIdx.x == 0 && s_qn > s_qpeak) s_qpeak = s_qn;
-        flush(s_qn);
+    if (count_only_on) {   // the remainders (fewer than 64 each), one per lane
+        float4 v[NV];
+        uint32_t vsp = 0xFFFFFFFFu;
+        if (rn) run_work(rn, v, vsp);
+        if (qn) run_queue(qn);
     }
     {
         const uint64_t sp = wave_sum(pts & 0xFFFFFFFFull), sh = wave_sum(pts >> 32);

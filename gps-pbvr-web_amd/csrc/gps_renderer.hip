@@ -920,12 +920,11 @@ int gps_renderer_read_particle_counters(GpsRenderer *r, uint64_t *out) {
     uint64_t w[ST_WORDS];
     HIP_TRY(r, hipMemcpyAsync(w, r->stats, sizeof w, hipMemcpyDeviceToHost, r->stream));
     HIP_TRY(r, hipStreamSynchronize(r->stream));
-    const uint64_t ppt = g_tuning.particles_per_thread;
     out[0] = w[ST_HITS];
     out[1] = w[ST_QPEAK];
-    out[2] = g_tuning.particle_block * (ppt + 1u);
+    out[2] = PARTICLE_WAVE_CAP;
     out[3] = w[ST_WPEAK];
-    out[4] = 64u * (ppt + 1u);
+    out[4] = PARTICLE_WAVE_CAP;
     return 0;
 }
 

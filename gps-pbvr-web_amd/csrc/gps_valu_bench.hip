@@ -5,8 +5,14 @@
 //   mode 0: f32 fused multiply-add (v_fma_f32), one per chain and iteration
 //   mode 1: the 32-bit PCG hash of the particle streams (v_mul_lo_u32, v_mad_u32, shifts, xors)
 //   mode 2: 32-bit integer add and xor with a variable shift (the hash without the multiplies)
-//   modes 3-7: one instruction class per chain step, pinned with register-only inline asm (no memory operands):
-//   3 v_add_u32, 4 v_xor_b32, 5 v_mul_lo_u32, 6 v_add_f32, 7 v_fma_f32 and v_add_u32 on alternate chains
+//   modes 3-17: one instruction class per chain step, pinned with register-only inline asm (no memory operands):
+//   3 v_add_u32, 4 v_xor_b32, 5 v_mul_lo_u32, 6 v_add_f32, 7 v_fma_f32 and v_add_u32 on alternate chains,
+//   8 v_lshrrev_b32 by a constant, 9 v_lshrrev_b32 by a register, 10 v_bfe_u32, 11 v_alignbit_b32,
+//   12 v_lshl_add_u32, 13 v_xad_u32, 14 v_mad_u32_u24, 15 v_mul_hi_u32, 16 v_lshl_or_b32,
+//   17 v_xor_b32 and v_lshrrev_b32 by a constant on alternate chains
+//   modes 18-20: the PCG hash of mode 1 written out instruction by instruction (register-only inline asm), two
+//   chains in lockstep with the constants in scalar registers (18), four chains in lockstep (19), two chains
+//   with the constants as literals (20)
 // The lane-instruction count of a launch is read from the SQ_INSTS_VALU counter (x 64); SQ_ACTIVE_INST_VALU2
 // counts the quad-cycles in which a SIMD issued two VALU instructions.
 #include <hip/hip_runtime.h>
@@ -20,10 +26,128 @@ __device__ inline uint32_t pcg_h(uint32_t v) {
     return (w >> 22u) ^ w;
 }
 
+// one PCG step of two chains in lockstep; c1, c2, k in scalar registers
+__device__ inline void pcg_x2(uint32_t &a, uint32_t &b, uint32_t c1, uint32_t c2, uint32_t k) {
+    uint32_t ta, tb;
+    asm volatile(
+        "v_mul_lo_u32 %0, %0, %4\n\t"
+        "v_mul_lo_u32 %1, %1, %4\n\t"
+        "v_add_u32_e32 %0, %6, %0\n\t"
+        "v_add_u32_e32 %1, %6, %1\n\t"
+        "v_lshrrev_b32_e32 %2, 28, %0\n\t"
+        "v_lshrrev_b32_e32 %3, 28, %1\n\t"
+        "v_add_u32_e32 %2, 4, %2\n\t"
+        "v_add_u32_e32 %3, 4, %3\n\t"
+        "v_lshrrev_b32_e32 %2, %2, %0\n\t"
+        "v_lshrrev_b32_e32 %3, %3, %1\n\t"
+        "v_xor_b32_e32 %0, %2, %0\n\t"
+        "v_xor_b32_e32 %1, %3, %1\n\t"
+        "v_mul_lo_u32 %0, %0, %5\n\t"
+        "v_mul_lo_u32 %1, %1, %5\n\t"
+        "v_lshrrev_b32_e32 %2, 22, %0\n\t"
+        "v_lshrrev_b32_e32 %3, 22, %1\n\t"
+        "v_xor_b32_e32 %0, %2, %0\n\t"
+        "v_xor_b32_e32 %1, %3, %1"
+        : "+v"(a), "+v"(b), "=&v"(ta), "=&v"(tb)
+        : "s"(c1), "s"(c2), "s"(k));
+}
+
+// the same with the added constant as a literal (v_mul_lo_u32, VOP3, takes no literal)
+__device__ inline void pcg_x2_lit(uint32_t &a, uint32_t &b, uint32_t c1, uint32_t c2) {
+    uint32_t ta, tb;
+    asm volatile(
+        "v_mul_lo_u32 %0, %0, %4\n\t"
+        "v_mul_lo_u32 %1, %1, %4\n\t"
+        "v_add_u32_e32 %0, 0xac564b05, %0\n\t"
+        "v_add_u32_e32 %1, 0xac564b05, %1\n\t"
+        "v_lshrrev_b32_e32 %2, 28, %0\n\t"
+        "v_lshrrev_b32_e32 %3, 28, %1\n\t"
+        "v_add_u32_e32 %2, 4, %2\n\t"
+        "v_add_u32_e32 %3, 4, %3\n\t"
+        "v_lshrrev_b32_e32 %2, %2, %0\n\t"
+        "v_lshrrev_b32_e32 %3, %3, %1\n\t"
+        "v_xor_b32_e32 %0, %2, %0\n\t"
+        "v_xor_b32_e32 %1, %3, %1\n\t"
+        "v_mul_lo_u32 %0, %0, %5\n\t"
+        "v_mul_lo_u32 %1, %1, %5\n\t"
+        "v_lshrrev_b32_e32 %2, 22, %0\n\t"
+        "v_lshrrev_b32_e32 %3, 22, %1\n\t"
+        "v_xor_b32_e32 %0, %2, %0\n\t"
+        "v_xor_b32_e32 %1, %3, %1"
+        : "+v"(a), "+v"(b), "=&v"(ta), "=&v"(tb)
+        : "s"(c1), "s"(c2));
+}
+
+// four chains in lockstep
+__device__ inline void pcg_x4(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, uint32_t c1, uint32_t c2,
+                              uint32_t k) {
+    uint32_t ta, tb, tc, td;
+    asm volatile(
+        "v_mul_lo_u32 %0, %0, %8\n\t"
+        "v_mul_lo_u32 %1, %1, %8\n\t"
+        "v_mul_lo_u32 %2, %2, %8\n\t"
+        "v_mul_lo_u32 %3, %3, %8\n\t"
+        "v_add_u32_e32 %0, %10, %0\n\t"
+        "v_add_u32_e32 %1, %10, %1\n\t"
+        "v_add_u32_e32 %2, %10, %2\n\t"
+        "v_add_u32_e32 %3, %10, %3\n\t"
+        "v_lshrrev_b32_e32 %4, 28, %0\n\t"
+        "v_lshrrev_b32_e32 %5, 28, %1\n\t"
+        "v_lshrrev_b32_e32 %6, 28, %2\n\t"
+        "v_lshrrev_b32_e32 %7, 28, %3\n\t"
+        "v_add_u32_e32 %4, 4, %4\n\t"
+        "v_add_u32_e32 %5, 4, %5\n\t"
+        "v_add_u32_e32 %6, 4, %6\n\t"
+        "v_add_u32_e32 %7, 4, %7\n\t"
+        "v_lshrrev_b32_e32 %4, %4, %0\n\t"
+        "v_lshrrev_b32_e32 %5, %5, %1\n\t"
+        "v_lshrrev_b32_e32 %6, %6, %2\n\t"
+        "v_lshrrev_b32_e32 %7, %7, %3\n\t"
+        "v_xor_b32_e32 %0, %4, %0\n\t"
+        "v_xor_b32_e32 %1, %5, %1\n\t"
+        "v_xor_b32_e32 %2, %6, %2\n\t"
+        "v_xor_b32_e32 %3, %7, %3\n\t"
+        "v_mul_lo_u32 %0, %0, %9\n\t"
+        "v_mul_lo_u32 %1, %1, %9\n\t"
+        "v_mul_lo_u32 %2, %2, %9\n\t"
+        "v_mul_lo_u32 %3, %3, %9\n\t"
+        "v_lshrrev_b32_e32 %4, 22, %0\n\t"
+        "v_lshrrev_b32_e32 %5, 22, %1\n\t"
+        "v_lshrrev_b32_e32 %6, 22, %2\n\t"
+        "v_lshrrev_b32_e32 %7, 22, %3\n\t"
+        "v_xor_b32_e32 %0, %4, %0\n\t"
+        "v_xor_b32_e32 %1, %5, %1\n\t"
+        "v_xor_b32_e32 %2, %6, %2\n\t"
+        "v_xor_b32_e32 %3, %7, %3"
+        : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "=&v"(ta), "=&v"(tb), "=&v"(tc), "=&v"(td)
+        : "s"(c1), "s"(c2), "s"(k));
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_valu(uint32_t *__restrict__ out, uint32_t iters, uint32_t seed) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if constexpr (MODE == 0) {
+    if constexpr (MODE >= 18) {
+        uint32_t x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = tid * 8u + (uint32_t)j + seed;
+        const uint32_t c1 = 747796405u ^ (seed & 0u), c2 = 277803737u ^ (seed & 0u), k = 2891336453u ^ (seed & 0u);
+        for (uint32_t it = 0; it < iters; it++) {
+            if constexpr (MODE == 18) {
+#pragma unroll
+                for (int j = 0; j < 8; j += 2) pcg_x2(x[j], x[j + 1], c1, c2, k);
+            } else if constexpr (MODE == 19) {
+#pragma unroll
+                for (int j = 0; j < 8; j += 4) pcg_x4(x[j], x[j + 1], x[j + 2], x[j + 3], c1, c2, k);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j += 2) pcg_x2_lit(x[j], x[j + 1], c1, c2);
+            }
+        }
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) s ^= x[j];
+        out[tid] = s;
+    } else if constexpr (MODE == 0) {
         float x[8];
         const float m = 0.999f + (float)(seed & 1u) * 1e-7f, c = 1e-3f;
 #pragma unroll
@@ -48,8 +172,22 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t *__restrict__ out, uint32
                 else if constexpr (MODE == 4) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
                 else if constexpr (MODE == 5) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
                 else if constexpr (MODE == 6) asm volatile("v_add_f32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
-                else if (j & 1) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
-                else asm volatile("v_fma_f32 %0, %0, %1, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 7) {
+                    if (j & 1) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                    else asm volatile("v_fma_f32 %0, %0, %1, %1" : "+v"(x[j]) : "v"(c));
+                } else if constexpr (MODE == 8) asm volatile("v_lshrrev_b32 %0, 13, %0" : "+v"(x[j]));
+                else if constexpr (MODE == 9) asm volatile("v_lshrrev_b32 %0, %1, %0" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 10) asm volatile("v_bfe_u32 %0, %0, 3, 20" : "+v"(x[j]));
+                else if constexpr (MODE == 11) asm volatile("v_alignbit_b32 %0, %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 12) asm volatile("v_lshl_add_u32 %0, %0, 3, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 13) asm volatile("v_xad_u32 %0, %0, %1, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 14) asm volatile("v_mad_u32_u24 %0, %0, %1, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 15) asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else if constexpr (MODE == 16) asm volatile("v_lshl_or_b32 %0, %0, 3, %1" : "+v"(x[j]) : "v"(c));
+                else {
+                    if (j & 1) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                    else asm volatile("v_lshrrev_b32 %0, 13, %0" : "+v"(x[j]));
+                }
             }
         }
         uint32_t s = 0;
@@ -77,10 +215,10 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t *__restrict__ out, uint32
 }  // namespace
 
 // Times `reps` launches of blocks x 256 threads, each running 8 chains of `iters` iterations of mode's operation;
-// out_ms receives the mean launch time.  Returns 0, -1 on invalid arguments (mode 0..7, blocks 1..65536, iters
+// out_ms receives the mean launch time.  Returns 0, -1 on invalid arguments (mode 0..20, blocks 1..65536, iters
 // 1..65536, reps 1..100), or -2 on a HIP error.
 extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int reps, float *out_ms) {
-    if (mode < 0 || mode > 7 || blocks < 1 || blocks > 65536 || iters < 1 || iters > 65536 || reps < 1 || reps > 100 ||
+    if (mode < 0 || mode > 20 || blocks < 1 || blocks > 65536 || iters < 1 || iters > 65536 || reps < 1 || reps > 100 ||
         !out_ms)
         return -1;
     uint32_t *buf = nullptr;
@@ -99,7 +237,20 @@ extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int rep
         else if (mode == 4) hipLaunchKernelGGL(k_valu<4>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 5) hipLaunchKernelGGL(k_valu<5>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 6) hipLaunchKernelGGL(k_valu<6>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
-        else hipLaunchKernelGGL(k_valu<7>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 7) hipLaunchKernelGGL(k_valu<7>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 8) hipLaunchKernelGGL(k_valu<8>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 9) hipLaunchKernelGGL(k_valu<9>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 10) hipLaunchKernelGGL(k_valu<10>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 11) hipLaunchKernelGGL(k_valu<11>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 12) hipLaunchKernelGGL(k_valu<12>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 13) hipLaunchKernelGGL(k_valu<13>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 14) hipLaunchKernelGGL(k_valu<14>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 15) hipLaunchKernelGGL(k_valu<15>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 16) hipLaunchKernelGGL(k_valu<16>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 17) hipLaunchKernelGGL(k_valu<17>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 18) hipLaunchKernelGGL(k_valu<18>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 19) hipLaunchKernelGGL(k_valu<19>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else hipLaunchKernelGGL(k_valu<20>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         (void)hipEventRecord(e1, nullptr);
         if (hipEventSynchronize(e1) != hipSuccess) rc = -2;
         float ms = 0.0f;

@@ -174,9 +174,8 @@ int gps_renderer_read_depth_id(GpsRenderer *r, uint64_t *out);
 /* Particle kernel counters since the last reset_stats (no counterpart in the reference; measurement and test
  * hooks): out[0] the particles that reached a subpixel (on screen: a packed-minimum update or an on-screen sorted
  * key), the HBM-writing part of the particle work; the high-water marks of the count-only LDS buffers against
- * their capacities under the current tuning: out[1] the emit_pbvr queue's peak fill, out[2] its capacity
- * (particle_block x (particles_per_thread + 1)), out[3] a wave's count_only work buffer's peak fill, out[4] its
- * capacity (64 x (particles_per_thread + 1)). */
+ * their capacities: out[1] a wave's emit_pbvr queue's peak fill, out[2] its capacity (128), out[3] a wave's
+ * count_only work buffer's peak fill, out[4] its capacity (128). */
 int gps_renderer_read_particle_counters(GpsRenderer *r, uint64_t *out);
 
 /* GPS_PATH_PARTICLE_SORTED: the sorted keys and subpixels (0xFFFFFFFF = off screen) of the last ensemble;

@@ -126,8 +126,7 @@ def test_config2_bit_exact_in_timed_configuration(view, method):
     assert stats_tuple(st) == tuple(int(x) for x in m["stats"][:9])
     assert st.points > 0 and st.skipped_ensembles == 0 and st.ensembles == 4
     assert np.array_equal(gpu["accum"].view(np.uint32), m["accum"].view(np.uint32))
-    # the count-only LDS buffers stayed within their capacities (queue: a chunk appends at most block x particles
-    # per thread and the queue is flushed above one block; work buffer: under 64 left plus 64 x particles per thread)
+    # the count-only LDS buffers stayed within their capacities (per wave: under 64 waiting plus one round's 64)
     q, qcap, wk, wcap = gpu["peaks"]
     say(f"    count-only queue peak {q} / {qcap}, work buffer peak {wk} / {wcap}")
     assert q <= qcap and wk < wcap
