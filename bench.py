@@ -213,7 +213,8 @@ def bench_alpha(a):
 
 def bench_sort3m(a):
     """3,000,000 clustered Gaussians (seed 2, SH3), 1920x1080, SPP side 2, GPS, cap 65,536: the packed-minimum
-    resolve and the depth-sorted resolve (64-bit onesweep sort of every particle) per view."""
+    resolve and the depth-sorted resolve (64-bit onesweep sort of every particle) per view, the sorted one at
+    each radix digit width (4..8 bits; 8 is the default)."""
     import gps_hip as gh
     from gps_hip import scenes as S
     W, H = 1920, 1080
@@ -224,7 +225,9 @@ def bench_sort3m(a):
     for v in [x for x in a.views.split(",") if x]:
         cam = S.clustered_camera(v, W, H)
         res = {}
-        for path in (1, 2):
+        saved = gh.get_tuning()
+        for path, bits in ((1, 8), (2, 8), (2, 7), (2, 6), (2, 5), (2, 4)):
+            gh.set_tuning(sort_digit_bits=bits)
             p = gh.RenderParams(spp_side=2, method=gh.Method.Gps, max_points_per_splat=65536, path=path)
             ens = max(1, min(a.ensembles, 8))
             r.reset_accum()
@@ -248,8 +251,9 @@ def bench_sort3m(a):
                 d["sort_passes_per_ensemble"] = c / ens
                 d["sort_pass_GBps"] = round(24.0 * keys * (c / ens) / (t * 1e-3) / 1e9, 1)
                 d["sort_keys_per_s"] = round(keys / ((t + prof.get("sort_hist", (0, 0))[1]) * 1e-3), 1)
-            res["atomic" if path == 1 else "sorted"] = d
-            log(f"[sort3m {v} path={path}] {d}")
+            res["atomic" if path == 1 else ("sorted" if bits == 8 else f"sorted_{bits}bit")] = d
+            log(f"[sort3m {v} path={path} digit bits {bits}] {d}")
+        gh.set_tuning(**saved)
         out[v] = res
     print(json.dumps({"mode": "sort3m", "workload": "clustered 3M Gaussians SH3 seed 2, 1920x1080, SPP side 2, GPS, "
                       "cap 65536", "views": out}), flush=True)
@@ -450,7 +454,8 @@ def main():
     ap.add_argument("--no-cull", action="store_true", help="skip the timing of the culled renders")
     ap.add_argument("--heldout", type=int, default=10, help="held-out scenes (seeds 200..) for the denoiser")
     ap.add_argument("--tuning", default=None,
-                    help="waves,ppt,items,batch,window[,count_only[,block]]: fixed launch tuning (implies --no-autotune)")
+                    help="waves,ppt,items,batch,window[,count_only[,block[,digit_bits]]]: fixed launch tuning "
+                         "(implies --no-autotune)")
     ap.add_argument("--ply", default=None, help="bench-ply: the 3DGS PLY file (default: the test fixture)")
     ap.add_argument("--size", default="480x270", help="bench-ply: WxH")
     ap.add_argument("--spp", type=int, default=1, help="bench-ply: SPP side")
@@ -472,7 +477,8 @@ def main():
         v = [int(x) for x in a.tuning.split(",")]
         gh.set_tuning(particle_waves_per_cu=v[0], particles_per_thread=v[1], sort_items_per_thread=v[2],
                       batch_ensembles=v[3], depth_window_mib=v[4],
-                      count_only=v[5] if len(v) > 5 else 1, particle_block=v[6] if len(v) > 6 else 256)
+                      count_only=v[5] if len(v) > 5 else 1, particle_block=v[6] if len(v) > 6 else 256,
+                      sort_digit_bits=v[7] if len(v) > 7 else 8)
         a.no_autotune = True
     if a.mode == "calib":
         x = np.ones(64 << 20, np.uint32)

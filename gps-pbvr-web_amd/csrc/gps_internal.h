@@ -56,6 +56,7 @@ struct GpsTuningDev {
     uint32_t depth_window_mib;        // depth|id bytes one launch may span when batch_ensembles is 0
     uint32_t count_only;              // 1: count-only path for the particles of off-screen PBVR splats
     uint32_t particle_block;          // particle kernel workgroup size: 128, 256 or 512 threads
+    uint32_t sort_digit_bits;         // radix digit width of the 64-bit sort, 4..8
 };
 GpsTuningDev gps_tuning_current();
 

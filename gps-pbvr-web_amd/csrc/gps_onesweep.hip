@@ -1,13 +1,14 @@
 // Onesweep LSD radix sort of 64-bit keys with 32-bit values, and the depth-sorted particle resolve.
 //
-// Sort (8-bit digits, stable):
+// Sort (digits of 4 to 8 bits, the tuning's sort_digit_bits; stable):
 //   k_os_hist   one read of the keys builds the histograms of every digit position the sort uses at once
 //   k_os_pass   per digit pass, one kernel: ticketed tiles (16,384 keys by default) rank their keys with
 //               wave64 ballots, publish per-digit counts and resolve their per-digit global offsets by decoupled
 //               look-back
 //               (one digit per thread), stage the tile in LDS in digit order, then write it out in
 //               contiguous runs.  Passes whose digit is the same for every key are skipped.
-//   Traffic per executed pass: 12 B read + 12 B written per key.
+//   Traffic per executed pass: 12 B read + 12 B written per key.  Narrower digits make cheaper passes (fewer
+//   ballots per key, fewer look-back chains per tile) but more of them.
 //
 // Depth-sorted resolve (the cross-check of the packed-minimum path): each particle of an ensemble carries
 // the key (ordered depth key << 32 | particle index) and its subpixel.  After the sort, the first sorted
@@ -24,20 +25,20 @@ namespace {
 constexpr int OS_BLOCK = 256;
 constexpr uint64_t OS_VAL_MASK = (1ull << 38) - 1ull;
 constexpr uint32_t OS_EPOCH_MASK = (1u << 24) - 1u;
+constexpr int OS_MAX_PASSES = 16;   // 64 bits of 4-bit digits
 
-// histograms of digit positions dlo..7 (the lower ones are not sorted and stay zero)
+// histograms (256 words per pass) of the npass digits of rb bits from bit lo_bit up (the last may be narrower)
 __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict__ keys, uint64_t n,
-                                                      uint32_t *__restrict__ ghist, int dlo) {
-    __shared__ uint32_t h[8][256];
-    for (int i = threadIdx.x; i < 8 * 256; i += OS_BLOCK) (&h[0][0])[i] = 0u;
+                                                      uint32_t *__restrict__ ghist, int lo_bit, int rb, int npass) {
+    __shared__ uint32_t h[OS_MAX_PASSES][256];
+    for (int i = threadIdx.x; i < npass * 256; i += OS_BLOCK) (&h[0][0])[i] = 0u;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * OS_BLOCK;
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t mask = (1u << rb) - 1u;
     auto add = [&](uint64_t k, bool full) {   // full: every lane of this wave holds a key
-#pragma unroll
-        for (int d = 0; d < 8; d++) {
-            if (d < dlo) continue;
-            uint32_t dg = (uint32_t)((k >> (8 * d)) & 255u);
+        for (int d = 0; d < npass; d++) {
+            uint32_t dg = (uint32_t)((k >> (lo_bit + rb * d)) & mask);
             // a wave whose 64 keys share the digit adds 64 once instead of 64 serialised LDS atomics
             uint32_t first = __builtin_amdgcn_readfirstlane(dg);
             if (full && __ballot(dg != first) == 0ull) {
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(OS_BLOCK) void k_os_hist(const uint64_t *__restrict
     }
     for (; i < n; i += stride) add(keys[i], i - lane + 64 <= n);
     __syncthreads();
-    for (int i = threadIdx.x; i < 8 * 256; i += OS_BLOCK) {
+    for (int i = threadIdx.x; i < npass * 256; i += OS_BLOCK) {
         uint32_t c = (&h[0][0])[i];
         if (c) atomicAdd(&ghist[i], c);
     }
@@ -82,28 +83,30 @@ __device__ __forceinline__ uint32_t scan_block_exclusive(uint32_t v, uint32_t *s
     return base + inc - v;
 }
 
-// BLK threads x OS_ITEMS keys per thread per tile.  The tile is staged in digit order through one
-// 8-byte-per-key LDS buffer, keys first and then values (a 4096-key tile takes 32 KB of LDS).  Threads 0..255
-// own one digit each for the per-digit counts, offsets and look-back.
-template <int BLK, int OS_ITEMS>
+// BLK threads x OS_ITEMS keys per thread per tile, RB-bit digits (the key's bits from `shift` up).  The tile is
+// staged in digit order through one 8-byte-per-key LDS buffer, keys first and then values (a 4096-key tile takes
+// 32 KB of LDS).  Threads 0..2^RB - 1 own one digit each for the per-digit counts, offsets and look-back.
+template <int BLK, int OS_ITEMS, int RB>
 __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                       uint64_t n, uint32_t shift, const uint32_t *__restrict__ ghist,
                                                       uint32_t *ticket, uint64_t *status, uint32_t epoch) {
     constexpr int WAVES = BLK / 64;
     constexpr int OS_TILE = BLK * OS_ITEMS;
+    constexpr uint32_t NB = 1u << RB;   // digits; NB itself marks a key beyond n
+    static_assert(RB >= 4 && RB <= 8 && (int)NB <= BLK, "digit width");
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_tmp[WAVES];
-    __shared__ uint32_t s_wh[WAVES][257];
-    __shared__ uint32_t s_toff[256];
-    __shared__ uint64_t s_gpre[256];
+    __shared__ uint32_t s_wh[WAVES][NB + 1];
+    __shared__ uint32_t s_toff[NB];
+    __shared__ uint64_t s_gpre[NB];
     __shared__ uint64_t s_keys[OS_TILE];
     uint32_t *s_vals = reinterpret_cast<uint32_t *>(s_keys);
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
-    for (uint32_t d = lane; d < 257; d += 64) s_wh[w][d] = 0u;
+    for (uint32_t d = lane; d < NB + 1u; d += 64) s_wh[w][d] = 0u;
     // global base of digit t for this pass
-    const uint32_t gbase = scan_block_exclusive(t < 256u ? ghist[t] : 0u, s_tmp);
+    const uint32_t gbase = scan_block_exclusive(t < NB ? ghist[t] : 0u, s_tmp);
     const uint64_t tile = s_tile;
     const uint64_t lt = (1ull << lane) - 1ull;
     uint64_t key[OS_ITEMS];
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ ki
     for (int j = 0; j < OS_ITEMS; j++) {
         uint64_t i = tile * OS_TILE + (uint64_t)w * (64 * OS_ITEMS) + (uint64_t)j * 64 + lane;
         bool valid = i < n;
-        uint32_t d = valid ? (uint32_t)((key[j] >> shift) & 255ull) : 256u;
+        uint32_t d = valid ? (uint32_t)((key[j] >> shift) & (NB - 1u)) : NB;
         uint64_t peers;
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
         if (__ballot(d != d0) == 0ull) {
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ ki
         } else {
             peers = ~0ull;
 #pragma unroll
-            for (int b = 0; b < 9; b++) {
+            for (int b = 0; b <= RB; b++) {
                 uint64_t bb = __ballot((d >> b) & 1u);
                 peers &= ((d >> b) & 1u) ? bb : ~bb;
             }
@@ -142,7 +145,7 @@ __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ ki
     __syncthreads();
     // per digit t: offsets of the waves inside the tile and the tile's count
     uint32_t run = 0;
-    if (t < 256u)
+    if (t < NB)
         for (int ww = 0; ww < WAVES; ww++) {
             uint32_t c = s_wh[ww][t];
             s_wh[ww][t] = run;
@@ -150,9 +153,9 @@ __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ ki
         }
     const uint32_t tcount = run;
     const uint32_t toff = scan_block_exclusive(tcount, s_tmp);
-    if (t < 256u) s_toff[t] = toff;
-    // decoupled look-back for digit t
-    if (t < 256u) {
+    if (t < NB) s_toff[t] = toff;
+    // decoupled look-back for digit t (status words: 256 per tile, the first NB used)
+    if (t < NB) {
         const uint64_t tag = (uint64_t)(epoch & OS_EPOCH_MASK) << 40;
         uint64_t excl = 0;
         if (tile == 0) {
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ ki
 #pragma unroll
     for (int j = 0; j < OS_ITEMS; j++) {
         if (rank[j] != 0xFFFFFFFFu) {
-            const uint32_t d = (uint32_t)((key[j] >> shift) & 255ull);
+            const uint32_t d = (uint32_t)((key[j] >> shift) & (NB - 1u));
             rank[j] += s_toff[d] + s_wh[w][d];
             s_keys[rank[j]] = key[j];
         }
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(BLK) void k_os_pass(const uint64_t *__restrict__ ki
         const uint32_t q = t + (uint32_t)qi * BLK;
         if (q < cnt) {
             uint64_t k = s_keys[q];
-            uint32_t d = (uint32_t)((k >> shift) & 255ull);
+            uint32_t d = (uint32_t)((k >> shift) & (NB - 1u));
             gdst[qi] = (uint32_t)(s_gpre[d] + (q - s_toff[d]));
             kout[gdst[qi]] = k;
         }
@@ -262,7 +265,7 @@ unsigned nb(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 // ====================================================================== host
 struct OnesweepState {
-    uint32_t *ghist = nullptr;        // [8][256]
+    uint32_t *ghist = nullptr;        // [OS_MAX_PASSES][256]
     uint32_t *ticket = nullptr;
     uint64_t *status = nullptr;
     uint64_t status_tiles = 0;
@@ -312,22 +315,43 @@ struct OProf {
         }                                                                            \
     } while (0)
 
-// Stable sort of n (key, value) pairs.  k[0]/v[0] hold the input; the result ends in buffer `result`
-// (0 or 1).  Reads the eight histograms back to skip passes whose digit is constant.
-// first_digit = 4 skips the four low-word passes: valid when the input is already ordered by the low 32
-// bits (LSD passes over an already sorted field are identities).
+// Stable sort of n (key, value) pairs by their bits lo_bit..63.  k[0]/v[0] hold the input; the result ends in
+// buffer `result` (0 or 1).  Digits of the tuning's sort_digit_bits; reads the histograms back to skip passes
+// whose digit is constant.  lo_bit = 32 skips the low word: valid when the input is already ordered by it (LSD
+// passes over an already sorted field are identities).
+template <int RB>
+static void launch_pass(int items, unsigned tiles, hipStream_t st, const uint64_t *ki, const uint32_t *vi,
+                        uint64_t *ko, uint32_t *vo, uint64_t n, uint32_t shift, const uint32_t *gh, uint32_t *ticket,
+                        uint64_t *status, uint32_t epoch) {
+    if (items == 8)
+        hipLaunchKernelGGL((k_os_pass<256, 8, RB>), dim3(tiles), dim3(256), 0, st, ki, vi, ko, vo, n, shift, gh,
+                           ticket, status, epoch);
+    else if (items == 16)
+        hipLaunchKernelGGL((k_os_pass<256, 16, RB>), dim3(tiles), dim3(256), 0, st, ki, vi, ko, vo, n, shift, gh,
+                           ticket, status, epoch);
+    else if (items == 32)
+        hipLaunchKernelGGL((k_os_pass<512, 16, RB>), dim3(tiles), dim3(512), 0, st, ki, vi, ko, vo, n, shift, gh,
+                           ticket, status, epoch);
+    else
+        hipLaunchKernelGGL((k_os_pass<1024, 16, RB>), dim3(tiles), dim3(1024), 0, st, ki, vi, ko, vo, n, shift, gh,
+                           ticket, status, epoch);
+}
+
 int onesweep_sort(OnesweepState *s, hipStream_t st, uint64_t *k[2], uint32_t *v[2], uint64_t n, int &result,
-                  std::string &err, ProfList *prof, int first_digit) {
+                  std::string &err, ProfList *prof, int lo_bit) {
     result = 0;
     if (n <= 1) return 0;
     if (!s->ghist) {
-        O_TRY(hipMalloc(&s->ghist, 8 * 256 * 4));
+        O_TRY(hipMalloc(&s->ghist, OS_MAX_PASSES * 256 * 4));
         O_TRY(hipMalloc(&s->ticket, 4));
         O_TRY(hipMemset(s->ticket, 0, 4));
     }
     // tiles of 256 x items keys; 32 runs as 512 threads x 16, 64 as 1024 threads x 16
-    const uint32_t ti = gps_tuning_current().sort_items_per_thread;
+    const GpsTuningDev tun = gps_tuning_current();
+    const uint32_t ti = tun.sort_items_per_thread;
     const int items = (ti == 8 || ti == 16 || ti == 32) ? (int)ti : 64;
+    const int rb = (tun.sort_digit_bits >= 4 && tun.sort_digit_bits <= 8) ? (int)tun.sort_digit_bits : 8;
+    const int npass = (64 - lo_bit + rb - 1) / rb;
     const uint64_t tile_keys = (uint64_t)256 * items;
     uint64_t tiles = (n + tile_keys - 1) / tile_keys;
     if (tiles > s->status_tiles) {
@@ -337,39 +361,35 @@ int onesweep_sort(OnesweepState *s, hipStream_t st, uint64_t *k[2], uint32_t *v[
         O_TRY(hipMemset(s->status, 0, tiles * 256 * 8));
         s->status_tiles = tiles;
     }
-    hipLaunchKernelGGL(k_u32_set, dim3(8), dim3(256), 0, st, s->ghist, (uint64_t)2048, 0u);
+    hipLaunchKernelGGL(k_u32_set, dim3(8), dim3(256), 0, st, s->ghist, (uint64_t)npass * 256, 0u);
     {
         OProf p(prof, st, "sort_hist");
         unsigned grid = nb(n, OS_BLOCK * 16);
         if (grid > 4096) grid = 4096;
-        hipLaunchKernelGGL(k_os_hist, dim3(grid), dim3(OS_BLOCK), 0, st, k[0], n, s->ghist, first_digit);
+        hipLaunchKernelGGL(k_os_hist, dim3(grid), dim3(OS_BLOCK), 0, st, k[0], n, s->ghist, lo_bit, rb, npass);
     }
-    std::vector<uint32_t> h(2048);
-    O_TRY(hipMemcpyAsync(h.data(), s->ghist, 2048 * 4, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> h((size_t)npass * 256);
+    O_TRY(hipMemcpyAsync(h.data(), s->ghist, h.size() * 4, hipMemcpyDeviceToHost, st));
     O_TRY(hipStreamSynchronize(st));
     int cur = 0;
-    for (int d = first_digit; d < 8; d++) {
+    for (int d = 0; d < npass; d++) {
         bool constant = false;
         for (int b = 0; b < 256; b++)
-            if (h[d * 256 + b] == n) constant = true;
+            if (h[(size_t)d * 256 + b] == n) constant = true;
         if (constant) continue;
         OProf p(prof, st, "sort_pass");
-        if (items == 8)
-            hipLaunchKernelGGL((k_os_pass<256, 8>), dim3((unsigned)tiles), dim3(256), 0, st, k[cur], v[cur],
-                               k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
-                               s->epoch);
-        else if (items == 16)
-            hipLaunchKernelGGL((k_os_pass<256, 16>), dim3((unsigned)tiles), dim3(256), 0, st, k[cur], v[cur],
-                               k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
-                               s->epoch);
-        else if (items == 32)
-            hipLaunchKernelGGL((k_os_pass<512, 16>), dim3((unsigned)tiles), dim3(512), 0, st, k[cur], v[cur],
-                               k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
-                               s->epoch);
-        else
-            hipLaunchKernelGGL((k_os_pass<1024, 16>), dim3((unsigned)tiles), dim3(1024), 0, st, k[cur], v[cur],
-                               k[cur ^ 1], v[cur ^ 1], n, (uint32_t)(8 * d), s->ghist + 256 * d, s->ticket, s->status,
-                               s->epoch);
+        const uint32_t shift = (uint32_t)(lo_bit + rb * d);
+        const uint32_t *gh = s->ghist + 256 * d;
+        if (rb == 4) launch_pass<4>(items, (unsigned)tiles, st, k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], n, shift, gh,
+                                    s->ticket, s->status, s->epoch);
+        else if (rb == 5) launch_pass<5>(items, (unsigned)tiles, st, k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], n, shift,
+                                         gh, s->ticket, s->status, s->epoch);
+        else if (rb == 6) launch_pass<6>(items, (unsigned)tiles, st, k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], n, shift,
+                                         gh, s->ticket, s->status, s->epoch);
+        else if (rb == 7) launch_pass<7>(items, (unsigned)tiles, st, k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], n, shift,
+                                         gh, s->ticket, s->status, s->epoch);
+        else launch_pass<8>(items, (unsigned)tiles, st, k[cur], v[cur], k[cur ^ 1], v[cur ^ 1], n, shift, gh,
+                            s->ticket, s->status, s->epoch);
         s->epoch = (s->epoch + 1u) & OS_EPOCH_MASK;
         if (!s->epoch) s->epoch = 1;
         cur ^= 1;

@@ -32,7 +32,7 @@ OnesweepState *onesweep_create();
 void onesweep_destroy(OnesweepState *s);
 int onesweep_sort(OnesweepState *s, hipStream_t st, uint64_t *k[2], uint32_t *v[2], uint64_t n, int &result,
                   std::string &err, std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> *prof,
-                  int first_digit);
+                  int lo_bit);
 void launch_first_hit(hipStream_t s, const uint32_t *vals, uint64_t n, uint32_t *first, uint64_t S);
 void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *first, uint64_t S,
                         const uint32_t *offsets, uint32_t N, uint64_t *depth);
@@ -40,7 +40,7 @@ void launch_build_depth(hipStream_t s, const uint64_t *keys, const uint32_t *fir
 namespace {
 
 thread_local std::string g_create_error;
-GpsTuningDev g_tuning = {64u, 4u, 64u, 0u, 64u, 1u, 256u};
+GpsTuningDev g_tuning = {64u, 4u, 64u, 0u, 64u, 1u, 256u, 8u};
 
 constexpr uint64_t MAX_SPLATS = 1ull << 30;
 constexpr uint32_t LUT_O = 65, LUT_V = 129;
@@ -409,9 +409,9 @@ int render_sorted_particles(GpsRenderer *r, const GpsFrame &f, uint32_t first_se
                                  (int)g_tuning.particle_block);
             }
             int res = 0;
-            // keys are written at their particle index, so the low word is already in order: 4 passes
+            // keys are written at their particle index, so the low word is already in order
             if (onesweep_sort(r->os, r->stream, r->pkeys, r->pidx, total, res, r->err,
-                              r->profiling ? &r->pending : nullptr, 4) != 0)
+                              r->profiling ? &r->pending : nullptr, 32) != 0)
                 return -1;
             r->psort_result = res;
             {
@@ -1527,6 +1527,7 @@ void gps_tuning_get(GpsTuning *out) {
     out->depth_window_mib = g_tuning.depth_window_mib;
     out->count_only = g_tuning.count_only;
     out->particle_block = g_tuning.particle_block;
+    out->sort_digit_bits = g_tuning.sort_digit_bits;
 }
 
 int gps_tuning_set(const GpsTuning *t) {
@@ -1540,6 +1541,7 @@ int gps_tuning_set(const GpsTuning *t) {
     if (t->count_only > 1) return -1;
     const uint32_t pb = t->particle_block;
     if (!(pb == 128 || pb == 256 || pb == 512)) return -1;
+    if (t->sort_digit_bits < 4 || t->sort_digit_bits > 8) return -1;
     g_tuning.particle_waves_per_cu = t->particle_waves_per_cu;
     g_tuning.particles_per_thread = ppt;
     g_tuning.sort_items_per_thread = t->sort_items_per_thread;
@@ -1547,6 +1549,7 @@ int gps_tuning_set(const GpsTuning *t) {
     g_tuning.depth_window_mib = t->depth_window_mib;
     g_tuning.count_only = t->count_only;
     g_tuning.particle_block = pb;
+    g_tuning.sort_digit_bits = t->sort_digit_bits;
     return 0;
 }
 
@@ -1614,6 +1617,25 @@ int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderP
                     }
                 }
     double best_ms_per = best_ms / trial_ensembles;
+    // sorted path: the radix digit width of the 64-bit sort (narrower digits: cheaper passes, more of them)
+    if (rc == 0 && sorted_path) {
+        for (uint32_t bits = 4; bits <= 8 && rc == 0; bits++) {
+            if (bits == best.sort_digit_bits) continue;
+            g_tuning = best;
+            g_tuning.sort_digit_bits = bits;
+            (void)hipEventRecord(e0, r->stream);
+            rc = gps_renderer_render_ensembles(r, cam, p, first_seed, trial_ensembles);
+            (void)hipEventRecord(e1, r->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = (double)ms / trial_ensembles;
+            if (rc == 0 && per < best_ms_per) {
+                best_ms_per = per;
+                best = g_tuning;
+            }
+        }
+    }
     // second stage: ensembles per particle launch, through the depth window
     if (rc == 0 && !sorted_path && start.batch_ensembles == 0) {
         const uint32_t windows[] = {8u, 16u, 32u, 64u, 128u, 256u, 512u, 4096u};

@@ -123,7 +123,8 @@ class GpuStats(C.Structure):
 class Tuning(C.Structure):
     """Launch tuning (include/gps_hip.h GpsTuning); rendered bits never depend on it."""
     _fields_ = [("particle_waves_per_cu", u32), ("particles_per_thread", u32), ("sort_items_per_thread", u32),
-                ("batch_ensembles", u32), ("depth_window_mib", u32), ("count_only", u32), ("particle_block", u32)]
+                ("batch_ensembles", u32), ("depth_window_mib", u32), ("count_only", u32), ("particle_block", u32),
+                ("sort_digit_bits", u32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -347,6 +347,7 @@ typedef struct GpsTuning {
                                         count-only path (undecided particles on the full path); 0: every particle
                                         on the full path.  Images and counters are the same either way */
     uint32_t particle_block;         /* particle kernel workgroup size: 128, 256 (default) or 512 threads */
+    uint32_t sort_digit_bits;        /* radix digit width of the 64-bit sort: 4..8 bits (default 8) */
 } GpsTuning;
 void gps_tuning_get(GpsTuning *out);
 int gps_tuning_set(const GpsTuning *t);
@@ -355,7 +356,8 @@ int gps_tuning_set(const GpsTuning *t);
  * on the sorted particle path) on trial_ensembles ensembles of this scene and view, then the depth window
  * (8 .. 4096 MiB, with batch_ensembles 0) on max(trial_ensembles, 2 x the ensembles per launch it gives),
  * keeps the fastest as the process tuning, and restores the accumulation buffer, denoiser features, counters
- * and submitted count.  Writes the chosen tuning and its GPU ms per ensemble. */
+ * and submitted count; on the sorted particle path it then times the sort's digit widths 4..8 bits.  Writes the
+ * chosen tuning and its GPU ms per ensemble. */
 int gps_renderer_autotune(GpsRenderer *r, const GpsCamera *cam, const GpsRenderParams *p, uint32_t first_seed,
                           uint32_t trial_ensembles, GpsTuning *best, double *best_ms_per_ensemble);
 

@@ -451,6 +451,46 @@ def test_onesweep_sort_tiles(items, n):
     assert np.array_equal(k2, keys[order]) and np.array_equal(v2, vals[order])
 
 
+@pytest.mark.parametrize("bits", [4, 5, 6, 7, 8])
+@pytest.mark.parametrize("n", [4097, 1_000_003])
+def test_onesweep_sort_digit_widths(bits, n):
+    """Every radix digit width (4..8 bits; 64 is not a multiple of 5, 6 or 7, so the last digit is narrower)
+    gives numpy's stable order on full-range keys with duplicates and on keys with constant bytes."""
+    rng = np.random.default_rng(n + bits)
+    full = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    dup = full[3::7]
+    full[: 7 * dup.size : 7] = dup   # duplicate keys
+    narrow = (np.uint64(0xBF) << np.uint64(56)) | rng.integers(0, max(n // 5, 1), n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    saved = gh.get_tuning()
+    try:
+        gh.set_tuning(sort_digit_bits=bits)
+        for keys in (full, narrow):
+            k2, v2 = gh.sort_pairs_u64(keys, vals)
+            order = np.argsort(keys, kind="stable")
+            assert np.array_equal(k2, keys[order]) and np.array_equal(v2, vals[order])
+    finally:
+        gh.set_tuning(**saved)
+    with pytest.raises(gh.GpsError):
+        gh.set_tuning(sort_digit_bits=9)
+    with pytest.raises(gh.GpsError):
+        gh.set_tuning(sort_digit_bits=3)
+
+
+@pytest.mark.parametrize("bits", [4, 6, 7])
+def test_sorted_particle_path_digit_widths(bits):
+    """The depth-sorted resolve equals the packed minimum with every sort digit width."""
+    name, scene = phase_scenes()[3]
+    p = gh.RenderParams(spp_side=2, background=BG, method=2, max_points_per_splat=1 << 24,
+                        pbvr=dict(radial_correction=1))
+    saved = gh.get_tuning()
+    try:
+        gh.set_tuning(sort_digit_bits=bits)
+        sorted_vs_atomic(to_gpu(scene), None, 0, CAM, p, W, H, (3, 3))
+    finally:
+        gh.set_tuning(**saved)
+
+
 def test_onesweep_sort_full_range_keys():
     n = 3_000_000
     rng = np.random.default_rng(9)
@@ -651,6 +691,28 @@ def test_autotune_keeps_history():
         r2.render_ensembles(cam, p, 1, 8)
         assert np.array_equal(r2.read_accum(), after)
         r2.close()
+    finally:
+        gh.set_tuning(**saved)
+
+
+def test_autotune_sorted_path_searches_digit_widths():
+    """On the sorted particle path the autotuner also times the sort's digit widths; the chosen tuning renders
+    the same bits as the packed minimum and the history is kept."""
+    name, scene = phase_scenes()[3]
+    p = gh.RenderParams(spp_side=2, background=BG, method=2, max_points_per_splat=1 << 24, path=2,
+                        pbvr=dict(radial_correction=1))
+    saved = gh.get_tuning()
+    try:
+        r = gh.GpsRenderer(W, H, 2)
+        r.set_gaussians(to_gpu(scene))
+        r.render_ensembles(CAM, p, 3, 1)
+        before = r.read_accum()
+        best, ms = r.autotune(CAM, p, trial_ensembles=1)
+        say(f"[autotune sorted] {best} {ms:.4f} ms/ens")
+        assert 4 <= best["sort_digit_bits"] <= 8 and ms > 0
+        assert np.array_equal(r.read_accum().view(np.uint32), before.view(np.uint32))
+        r.close()
+        sorted_vs_atomic(to_gpu(scene), None, 0, CAM, p, W, H, (3, 3))
     finally:
         gh.set_tuning(**saved)
 
