@@ -5,9 +5,9 @@ radius 2 / 6 / 15 (near / mid / far); PBVR Extinction C3+R (the hot path, crates
 shaders/gps_splat.wgsl), GPS and screen occupancy on the same scene (the three methods the reference's bench times,
 crates/gps-verify/src/main.rs:124-142).
 
-The GPU runs with the launch tuning the autotuner picks for each view on an MI355X (the benchmark's tuning: 256 or 32
-workgroups per CU, 4 particles per thread, depth windows of 32 / 64 / 256 MiB, so 4 / 8 / 34 ensembles per particle
-launch, the count-only path and the splat order on) and without the debug hook.  Against oracle/gps_mirror.c on the
+The GPU runs with the launch tuning the autotuner picks for each view on an MI355X (the benchmark's tuning: 256 / 64
+/ 16 workgroups per CU, 8 particles per thread, depth windows of 64 / 128 / 512 MiB, so up to 9 / 18 / 64 ensembles per
+particle launch, the count-only path and the splat order on) and without the debug hook.  Against oracle/gps_mirror.c on the
 host's threads: the accumulation buffer bit for bit and all nine counters.  A second pass keeps the per-ensemble
 buffers (debug hook) for one ensemble and compares the per-splat counts, the exclusive offsets, the particle total and
 the packed depth|id buffer.
@@ -33,9 +33,9 @@ STAT_KEYS = ("points", "truncated_splats", "dropped_points", "visible_splats", "
 # the autotuner's choice per view on an MI355X for this workload (particle workgroups per CU, particles per thread,
 # sort items per thread, ensembles per launch (0 = from the depth window), depth window MiB, count-only, block)
 VIEW_TUNING = {
-    "near": (256, 4, 64, 0, 32, 1, 256),
-    "mid": (256, 4, 64, 0, 64, 1, 256),
-    "far": (32, 4, 64, 0, 256, 1, 128),
+    "near": (256, 8, 64, 0, 64, 1, 256),
+    "mid": (64, 8, 64, 0, 128, 1, 256),
+    "far": (16, 8, 64, 0, 512, 1, 256),
 }
 _SCENE = {}
 
