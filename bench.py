@@ -182,7 +182,8 @@ def bench_valu(a):
                        (9, "lshrrev_vgpr"), (10, "bfe_u32"), (11, "alignbit_b32"), (12, "lshl_add_u32"),
                        (13, "xad_u32"), (14, "mad_u32_u24"), (15, "mul_hi_u32"), (16, "lshl_or_b32"),
                        (17, "xor_b32+lshrrev_const"), (18, "pcg_hash_x2_asm"), (19, "pcg_hash_x4_asm"),
-                       (20, "pcg_hash_x2_asm_literals")):
+                       (20, "pcg_hash_x2_asm_literals"), (21, "exp_f32"), (22, "rcp_f32"), (23, "sqrt_f32"),
+                       (24, "sin_f32"), (25, "log_f32"), (26, "exp_f32+fmac_f32")):
         ms = ctypes.c_float()
         rc = lib.gps_valu_bench(mode, blocks, iters, reps, ctypes.byref(ms))
         if rc != 0:

@@ -13,6 +13,8 @@
 //   modes 18-20: the PCG hash of mode 1 written out instruction by instruction (register-only inline asm), two
 //   chains in lockstep with the constants in scalar registers (18), four chains in lockstep (19), two chains
 //   with the constants as literals (20)
+//   modes 21-26: transcendentals, one per chain step: 21 v_exp_f32, 22 v_rcp_f32, 23 v_sqrt_f32, 24 v_sin_f32,
+//   25 v_log_f32, 26 v_exp_f32 and v_fmac_f32 on alternate chains
 // The lane-instruction count of a launch is read from the SQ_INSTS_VALU counter (x 64); SQ_ACTIVE_INST_VALU2
 // counts the quad-cycles in which a SIMD issued two VALU instructions.
 #include <hip/hip_runtime.h>
@@ -126,7 +128,28 @@ __device__ inline void pcg_x4(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d
 template <int MODE>
 __global__ __launch_bounds__(256) void k_valu(uint32_t *__restrict__ out, uint32_t iters, uint32_t seed) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if constexpr (MODE >= 18) {
+    if constexpr (MODE >= 21) {
+        float x[8];
+        const float c = 0.999f + (float)(seed & 1u) * 1e-7f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = 0.5f + (float)(tid & 255u) * 1e-4f + (float)j * 1e-3f;
+        for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if constexpr (MODE == 21) asm volatile("v_exp_f32 %0, %0" : "+v"(x[j]));
+                else if constexpr (MODE == 22) asm volatile("v_rcp_f32 %0, %0" : "+v"(x[j]));
+                else if constexpr (MODE == 23) asm volatile("v_sqrt_f32 %0, %0" : "+v"(x[j]));
+                else if constexpr (MODE == 24) asm volatile("v_sin_f32 %0, %0" : "+v"(x[j]));
+                else if constexpr (MODE == 25) asm volatile("v_log_f32 %0, %0" : "+v"(x[j]));
+                else if (j & 1) asm volatile("v_fmac_f32 %0, %0, %1" : "+v"(x[j]) : "v"(c));
+                else asm volatile("v_exp_f32 %0, %0" : "+v"(x[j]));
+            }
+        }
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) s += x[j];
+        out[tid] = __float_as_uint(s);
+    } else if constexpr (MODE >= 18) {
         uint32_t x[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) x[j] = tid * 8u + (uint32_t)j + seed;
@@ -215,10 +238,10 @@ __global__ __launch_bounds__(256) void k_valu(uint32_t *__restrict__ out, uint32
 }  // namespace
 
 // Times `reps` launches of blocks x 256 threads, each running 8 chains of `iters` iterations of mode's operation;
-// out_ms receives the mean launch time.  Returns 0, -1 on invalid arguments (mode 0..20, blocks 1..65536, iters
+// out_ms receives the mean launch time.  Returns 0, -1 on invalid arguments (mode 0..26, blocks 1..65536, iters
 // 1..65536, reps 1..100), or -2 on a HIP error.
 extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int reps, float *out_ms) {
-    if (mode < 0 || mode > 20 || blocks < 1 || blocks > 65536 || iters < 1 || iters > 65536 || reps < 1 || reps > 100 ||
+    if (mode < 0 || mode > 26 || blocks < 1 || blocks > 65536 || iters < 1 || iters > 65536 || reps < 1 || reps > 100 ||
         !out_ms)
         return -1;
     uint32_t *buf = nullptr;
@@ -250,7 +273,13 @@ extern "C" int gps_valu_bench(int mode, uint32_t blocks, uint32_t iters, int rep
         else if (mode == 17) hipLaunchKernelGGL(k_valu<17>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 18) hipLaunchKernelGGL(k_valu<18>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         else if (mode == 19) hipLaunchKernelGGL(k_valu<19>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
-        else hipLaunchKernelGGL(k_valu<20>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 20) hipLaunchKernelGGL(k_valu<20>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 21) hipLaunchKernelGGL(k_valu<21>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 22) hipLaunchKernelGGL(k_valu<22>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 23) hipLaunchKernelGGL(k_valu<23>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 24) hipLaunchKernelGGL(k_valu<24>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else if (mode == 25) hipLaunchKernelGGL(k_valu<25>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
+        else hipLaunchKernelGGL(k_valu<26>, dim3(blocks), dim3(256), 0, nullptr, buf, iters, (uint32_t)r);
         (void)hipEventRecord(e1, nullptr);
         if (hipEventSynchronize(e1) != hipSuccess) rc = -2;
         float ms = 0.0f;
