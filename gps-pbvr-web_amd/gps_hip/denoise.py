@@ -62,13 +62,15 @@ def window_camera(cam, x0, y0):
 
 
 def build_pairs(d, seeds, targets, crop=128, in_ens=(4, 16, 64, 256, 768), tgt_ens=2048, n_splats=560_000, rng_seed=0,
-                log=print, dist_range=(2.0, 15.0)):
+                log=print, dist_range=(2.0, 15.0), time_budget=None):
     """Renders `targets` target windows spread over the scenes `seeds` and, per target, one input per count in
-    `in_ens`; adds every (input, target) pair to the denoiser.  Returns the GPU-side render count."""
+    `in_ens`; adds every (input, target) pair to the denoiser.  crop=0: full 1280 x 720 frames instead of windows.
+    With a time budget (seconds) it stops adding targets once the budget is spent.  Returns the targets rendered."""
     rng = np.random.default_rng(rng_seed)
     p = params_ext_c3r()
-    inp = GpsRenderer(crop, crop, 1)
-    tgt = GpsRenderer(crop, crop, 1)
+    w, h = (crop, crop) if crop else (FULL_W, FULL_H)
+    inp = GpsRenderer(w, h, 1)
+    tgt = GpsRenderer(w, h, 1)
     inp.set_features(True)
     inp.set_cull(True)  # a window sees a few percent of the splats; the cull keeps the image bit for bit
     tgt.set_cull(True)
@@ -78,12 +80,16 @@ def build_pairs(d, seeds, targets, crop=128, in_ens=(4, 16, 64, 256, 768), tgt_e
     for si, seed in enumerate(seeds):
         if per_scene[si] == 0:
             continue
+        if time_budget is not None and time.perf_counter() - t0 > time_budget:
+            log(f"[pairs] time budget of {time_budget:.0f} s spent after {k} targets")
+            break
         g, sh = S.clustered(n_splats, seed=seed, sh_degree=3)
         inp.set_gaussians_sh(g, sh, 3)
         tgt.set_gaussians_sh(g, sh, 3)
         for _ in range(per_scene[si]):
             full = random_view(rng, dist_range=dist_range)
-            cam = window_camera(full, int(rng.integers(0, FULL_W - crop + 1)), int(rng.integers(0, FULL_H - crop + 1)))
+            cam = window_camera(full, int(rng.integers(0, FULL_W - crop + 1)),
+                                int(rng.integers(0, FULL_H - crop + 1))) if crop else full
             tgt.reset_accum()
             tgt.render_ensembles(cam, p, 1_000_000 + 4096 * k, tgt_ens)
             for j, e in enumerate(in_ens):
@@ -190,7 +196,15 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from this parameter file")
     ap.add_argument("--dist", default="2,15", help="orbit distance range of the training views, lo,hi")
     ap.add_argument("--rng-seed", type=int, default=0, help="seed of the view and window draws")
+    ap.add_argument("--time-budget", type=float, default=None, help="seconds of pair rendering at most")
+    ap.add_argument("--config5", action="store_true",
+                    help="the denoiser workload as specified: 2000 full-frame 1280x720 pairs of 4-ensemble inputs "
+                         "and 1024-ensemble targets from seeds 100-139 (overrides --targets, --crop, --in-ens, "
+                         "--tgt-ens, --seeds; trains 2000 steps of 8 frames)")
     a = ap.parse_args(argv)
+    if a.config5:
+        a.targets, a.crop, a.in_ens, a.tgt_ens, a.seeds = 2000, 0, "4", 1024, "100-139"
+        a.steps, a.batch = 2000, 8
     lo, hi = (int(x) for x in a.seeds.split("-"))
     log = lambda *m: print(*m, file=sys.stderr, flush=True)  # noqa: E731
     d = Denoiser(seed=1)
@@ -199,7 +213,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     dist = tuple(float(x) for x in a.dist.split(","))
     build_pairs(d, list(range(lo, hi + 1)), a.targets, a.crop, tuple(int(x) for x in a.in_ens.split(",")), a.tgt_ens,
-                a.splats, rng_seed=a.rng_seed, log=log, dist_range=dist)
+                a.splats, rng_seed=a.rng_seed, log=log, dist_range=dist, time_budget=a.time_budget)
     t1 = time.perf_counter()
     losses = train(d, a.steps, a.batch, a.lr, log=log)
     t2 = time.perf_counter()
