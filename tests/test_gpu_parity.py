@@ -728,8 +728,12 @@ def test_batch_graph_equals_separate_renders():
     pa = gh.RenderParams(spp_side=2, method=2, max_points_per_splat=16384, pbvr=dict(radial_correction=1))
     pg = gh.RenderParams(spp_side=2, method=0, max_points_per_splat=4096, path=1)
     pb = gh.RenderParams(spp_side=1, method=1, max_points_per_splat=16384)
+    pv = gh.RenderParams(spp_side=2, method=3, max_points_per_splat=16384, pbvr=dict(radial_correction=1))
+    po = gh.RenderParams(spp_side=2, method=4, max_points_per_splat=16384)
+    # every emitter class of the fused batch: GPS particles (pg), PBVR (pa, pb, pv), screen occupancy (po)
     items = [(ra, cam_a, pa, 11, 5), (rb, S.clustered_camera("mid", 160, 90), pb, 3, 4),
-             (ra, cam_a, pg, 1, 3), (rb, S.clustered_camera("far", 160, 90), pb, 7, 70)]
+             (ra, cam_a, pg, 1, 3), (rb, S.clustered_camera("far", 160, 90), pb, 7, 70),
+             (ra, cam_a, pv, 21, 6), (ra, cam_a, po, 5, 3)]
     expect, expect_st = [], []
     for r, cam, p, seed, count in items:
         r.reset_accum()
@@ -764,8 +768,9 @@ def test_batch_graph_equals_separate_renders():
             if use_graph and per_item:
                 assert t["graph_nodes"] > len(items)
             elif use_graph and round_ens == 16:
-                # clear, view, then per round: count-scan, (assign, particles) for emitter classes 0 and 1, resolve
-                assert t["graph_nodes"] == 2 + 5 * (2 + 2 * 2), t
+                # clear, view, then per round: count-scan, (assign, particles) for each of the three emitter
+                # classes, resolve
+                assert t["graph_nodes"] == 2 + 5 * (2 + 2 * 3), t
     finally:
         gh.set_tuning(**saved)
     # after a batch the renderer carries the last item's history and continues it exactly
