@@ -128,7 +128,7 @@ void launch_math(hipStream_t s, int fn, const float *x, const float *y, float *o
 // fused batch launches (gps_kernels.hip); tables are device arrays built by the host at record time
 void launch_fused_clear(hipStream_t s, const FusedItem *items, uint32_t nitems, uint32_t max_x);
 void launch_fused_view(hipStream_t s, const FusedItem *items, const uint32_t *blk_item, const uint32_t *item_blk0,
-                       uint32_t blocks, size_t lds);
+                       uint32_t blocks);
 void launch_fused_count_scan(hipStream_t s, const FusedItem *items, const uint32_t *blk_pair, uint32_t blocks,
                              uint32_t ens0, uint32_t epoch);
 void launch_fused_assign(hipStream_t s, const FusedItem *items, uint32_t nitems, const uint32_t *nens, uint32_t mk,

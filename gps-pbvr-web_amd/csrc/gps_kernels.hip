@@ -236,8 +236,7 @@ GPS_DEV Proj project(const GpsFrame &f, float4 g0, float4 g1, float4 g2) {
 }
 
 // gps_splat.wgsl:210-251 (evaluated once per splat per view)
-// `sh` holds splat `id`'s rest coefficients at sh + id * 3 * sh_stride (global memory, or a block's LDS copy
-// with a block-local id)
+// `sh` holds splat `id`'s rest coefficients at sh + id * 3 * sh_stride
 GPS_DEV float4 splat_colour(const GpsFrame &f, uint32_t id, float4 g0, float4 g3, const float *__restrict__ sh) {
     float res[3] = {g3.x, g3.y, g3.z};
     uint32_t deg = f.sh_degree;
@@ -246,7 +245,7 @@ GPS_DEV float4 splat_colour(const GpsFrame &f, uint32_t id, float4 g0, float4 g3
     float len = gpsm::sqrt(px * px + py * py + pz * pz);
     float x = px / len, y = py / len, z = pz / len;
     uint32_t r = f.sh_stride;
-    uint32_t o = id * 3u * r;
+    const size_t o = (size_t)id * 3u * r;
     float out[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ch++) {
@@ -420,28 +419,16 @@ GPS_DEV bool box_offscreen(const GpsFrame &f, const CamBox &b) {
 GPS_DEV void view_body(const GpsFrame &f, const float4 *__restrict__ splats, const float *__restrict__ sh,
                        float4 *__restrict__ view, uint2 *__restrict__ cinfo, float4 *__restrict__ colour,
                        uint32_t *__restrict__ perm, uint32_t blk) {
-    extern __shared__ float4 s_stage[];  // 4 float4 per splat, then 3 * sh_stride floats per splat
-    const uint32_t i0 = blk * blockDim.x;
-    const uint32_t nb = min((uint32_t)blockDim.x, f.N - i0);
-    for (uint32_t t = threadIdx.x; t < 4u * nb; t += blockDim.x) s_stage[t] = splats[4u * i0 + t];
-    float *s_sh = reinterpret_cast<float *>(s_stage + 4u * blockDim.x);
-    if (f.sh_degree > 0u) {
-        const uint32_t per = 3u * f.sh_stride, cnt = nb * per;
-        const float *src = sh + (size_t)i0 * per;   // 16-byte aligned: 256 * 12 * sh_stride bytes per block
-        const uint32_t c4 = cnt / 4u;
-        for (uint32_t t = threadIdx.x; t < c4; t += blockDim.x)
-            reinterpret_cast<float4 *>(s_sh)[t] = reinterpret_cast<const float4 *>(src)[t];
-        for (uint32_t t = 4u * c4 + threadIdx.x; t < cnt; t += blockDim.x) s_sh[t] = src[t];
-    }
-    __syncthreads();
-    const uint32_t i = i0 + threadIdx.x;
+    // each thread reads its splat and its SH rest coefficients itself: no LDS staging, so the kernel's occupancy
+    // is set by its registers alone and the strided loads are served from the caches
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     const bool live = i < f.N;
     bool marked = false;
     if (live) {
-        const uint32_t li = threadIdx.x;
-        float4 g0 = s_stage[4u * li], g1 = s_stage[4u * li + 1u], g2 = s_stage[4u * li + 2u];
-        float4 g3 = s_stage[4u * li + 3u];
-        colour[i] = splat_colour(f, li, g0, g3, s_sh);
+        const float4 *gs = splats + 4u * (size_t)i;
+        float4 g0 = gs[0], g1 = gs[1], g2 = gs[2];
+        float4 g3 = gs[3];
+        colour[i] = splat_colour(f, i, g0, g3, sh);
         uint2 ci = make_uint2(0u, KIND_NONE);
         Proj pr = project(f, g0, g1, g2);
         float o = clamp(g0.w, 0.0f, 1.0f);
@@ -1707,8 +1694,7 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((
 void launch_view(hipStream_t s, const GpsFrame &f, const float4 *splats, const float *sh, float4 *view, uint2 *cinfo,
                  float4 *colour, uint32_t *perm) {
     if (f.N == 0) return;
-    const size_t lds = 256 * (4 * sizeof(float4) + (f.sh_degree > 0 ? 3 * f.sh_stride * sizeof(float) : 0));
-    hipLaunchKernelGGL(k_view, dim3(blocks_for(f.N, 256)), dim3(256), lds, s, f, splats, sh, view, cinfo, colour,
+    hipLaunchKernelGGL(k_view, dim3(blocks_for(f.N, 256)), dim3(256), 0, s, f, splats, sh, view, cinfo, colour,
                        perm);
 }
 
@@ -1829,8 +1815,8 @@ void launch_fused_clear(hipStream_t s, const FusedItem *items, uint32_t nitems, 
 }
 
 void launch_fused_view(hipStream_t s, const FusedItem *items, const uint32_t *blk_item, const uint32_t *item_blk0,
-                       uint32_t blocks, size_t lds) {
-    if (blocks) hipLaunchKernelGGL(k_view_fused, dim3(blocks), dim3(256), lds, s, items, blk_item, item_blk0);
+                       uint32_t blocks) {
+    if (blocks) hipLaunchKernelGGL(k_view_fused, dim3(blocks), dim3(256), 0, s, items, blk_item, item_blk0);
 }
 
 void launch_fused_count_scan(hipStream_t s, const FusedItem *items, const uint32_t *blk_pair, uint32_t blocks,

@@ -312,7 +312,7 @@ void pack_frame(const GpsRenderer *r, const GpsCamera *cam, const GpsRenderParam
     if (p->pbvr.low_pass_jitter) flags |= FLAG_JITTER;
     if (r->cull) flags |= FLAG_CULL;
     const bool pbvr = p->method == M_PROPORTIONAL || p->method == M_EXTINCTION || p->method == M_VIEW_CONDITIONED;
-    // the count-only work entries carry (splat, particle seed), so the path holds for every cap
+    // the count-only records carry (splat, draw chain), not a particle index, so the path holds for every cap
     if (g_tuning.count_only && pbvr) flags |= FLAG_FAST;
     f.flags = flags;
     f.density = p->density_scale;
@@ -1101,7 +1101,6 @@ static int fused_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n
     // buffers and descriptors
     if (b->fbuf.size() < n) b->fbuf.resize(n);
     std::vector<FusedItem> fi(n);
-    size_t lds = 0;
     bool mk_used[3] = {false, false, false};
     for (uint32_t i = 0; i < n; i++) {
         const GpsBatchItem &it = items[i];
@@ -1168,8 +1167,6 @@ static int fused_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n
         d.first_seed = it.first_seed;
         d.mk = (uint32_t)fused_mk(it.params);
         mk_used[d.mk] = true;
-        const size_t l = 256 * (4 * sizeof(float4) + (d.f.sh_degree > 0 ? 3 * d.f.sh_stride * sizeof(float) : 0));
-        lds = std::max(lds, l);
     }
     // block tables: view (per item ceil(N / 256)), resolve (ceil(X / 256)), count-scan tiles per round, and the
     // particle grid's assignment (written on the device)
@@ -1247,7 +1244,7 @@ static int fused_batch_render(GpsBatch *b, const GpsBatchItem *items, uint32_t n
     if (!use_graph) (void)hipEventRecord(e0, b->stream);
     hipStream_t st = b->stream;
     launch_fused_clear(st, b->fitems, n, (uint32_t)max_x);
-    launch_fused_view(st, b->fitems, T + o_vb, T + o_v0, (uint32_t)vb.size(), lds);
+    launch_fused_view(st, b->fitems, T + o_vb, T + o_v0, (uint32_t)vb.size());
     for (uint32_t rd = 0; rd < rounds; rd++) {
         const uint32_t *ne = T + o_ne + (size_t)rd * n;
         const uint32_t ens0 = rd * round_ens;   // seeds are first_seed + ensemble modulo 2^32
