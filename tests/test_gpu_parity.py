@@ -788,6 +788,44 @@ def test_batch_graph_equals_separate_renders():
         r.close()
 
 
+def test_fused_batch_many_items_equals_per_item():
+    """300 items over six renderers of different sizes and SPP, every method, 0 to 9 ensembles each: the fused
+    launches (the particle grid split among up to 300 items on the device) give the bits of each item's own launches,
+    with and without a graph, and with 4 ensembles per round."""
+    scenes = [S.small_uniform(n, seed=40 + n) for n in (64, 512, 3000)]
+    rends = []
+    for k, (w, h, spp) in enumerate(((32, 24, 1), (64, 40, 2), (48, 48, 1), (80, 45, 1), (40, 30, 2), (56, 56, 1))):
+        r = gh.GpsRenderer(w, h, spp)
+        r.set_gaussians(scenes[k % 3])
+        rends.append(r)
+    rng = np.random.default_rng(5)
+    items = []
+    for i in range(300):
+        r = rends[i % len(rends)]
+        w, h = r.size()
+        spp = 2 if i % len(rends) in (1, 4) else 1
+        m = int(rng.integers(0, 5))
+        p = gh.RenderParams(spp_side=spp, method=m, max_points_per_splat=4096, path=1,
+                            pbvr=dict(radial_correction=int(rng.integers(0, 2))))
+        items.append((r, S.small_uniform_camera(w, h), p, int(rng.integers(1, 1000)), int(rng.integers(0, 10))))
+    saved = gh.get_tuning()
+    b = gh.Batch()
+    try:
+        b.render(items, use_graph=False, per_item=True)
+        want = [(b.read(i).view(np.uint32).copy(), b.read_stats(i)) for i in range(len(items))]
+        for round_ens, use_graph in ((0, True), (4, False), (0, False)):
+            gh.set_tuning(batch_ensembles=round_ens)
+            b.render(items, use_graph=use_graph)
+            for i in range(len(items)):
+                assert np.array_equal(b.read(i).view(np.uint32), want[i][0]), (round_ens, use_graph, i)
+                assert b.read_stats(i) == want[i][1], (round_ens, use_graph, i)
+    finally:
+        gh.set_tuning(**saved)
+        b.close()
+        for r in rends:
+            r.close()
+
+
 C4_SIZES = (20_000, 50_000, 100_000, 250_000, 560_000, 1_000_000, 1_500_000, 2_000_000)
 C4_RES = ((640, 360), (1280, 720), (1920, 1080))
 
