@@ -431,7 +431,8 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 def bench_denoise(a):
     """The learned denoiser on the 560k clustered scene's near view at 1280x720 (320 ensembles, features on):
-    GPU ms per forward (features + 5 convolutions + fallback) from HIP events, and its MFMA roofline."""
+    GPU ms per forward (features + 5 convolutions + fallback) from HIP events, layer by layer and fused, and the
+    fused forward's MFMA roofline."""
     import gps_hip as gh
     from gps_hip import scenes as S
     from gps_hip import denoise as D
@@ -444,16 +445,23 @@ def bench_denoise(a):
     r.render_ensembles(S.clustered_camera("near", W, H), D.params_ext_c3r(), 1, 320)
     d = gh.Denoiser()
     d.load(D.WEIGHTS)
-    for _ in range(max(a.warmup, 1)):
-        d.apply(r, fallback_ratio=4.0)
-    ms = [d.apply(r, fallback_ratio=4.0)["gpu_ms"] for _ in range(max(a.steps, 1) * 10)]
-    avg = float(np.mean(ms))
     flop = DENOISE_FLOP_PER_PIXEL * W * H
-    tf = flop / (avg * 1e-3) / 1e12
-    print(json.dumps({"mode": "denoise", "workload": "denoiser forward, 1280x720, 7 feature channels, 5 layers",
-                      "forwards": len(ms), "avg_ms": round(avg, 4), "flop_per_forward": flop,
-                      "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
-                                   "unit": "TFLOP/s", "frac": round(tf / BF16_MFMA_PEAK_TFLOPS, 4)}}), flush=True)
+    out = {"mode": "denoise", "workload": "denoiser forward, 1280x720, 7 feature channels, 5 layers",
+           "flop_per_forward": flop}
+    # the layer-by-layer kernels first, then the fused kernel apply runs by default (the same bits)
+    for name, fused in (("layered", False), ("fused", True)):
+        d.set_fused(fused)
+        for _ in range(max(a.warmup, 1)):
+            d.apply(r, fallback_ratio=4.0)
+        ms = [d.apply(r, fallback_ratio=4.0)["gpu_ms"] for _ in range(max(a.steps, 1) * 10)]
+        avg = float(np.mean(ms))
+        tf = flop / (avg * 1e-3) / 1e12
+        out[name] = {"forwards": len(ms), "avg_ms": round(avg, 4), "TFLOPs": round(tf, 2)}
+    tf = out["fused"]["TFLOPs"]
+    out["avg_ms"] = out["fused"]["avg_ms"]
+    out["roofline"] = {"bound": "mfma", "achieved": tf, "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(tf / BF16_MFMA_PEAK_TFLOPS, 4)}
+    print(json.dumps(out), flush=True)
     d.close()
     r.close()
 

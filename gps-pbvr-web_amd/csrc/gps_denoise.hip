@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "gps_devmath.h"
 #include "gps_internal.h"
 #include "../../include/gps_hip.h"
 
@@ -263,8 +264,13 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t *__restrict__ in
                         if (!(bf2f((uint16_t)mw[k]) > 0.0f)) v[k] = 0.0f;
                 }
                 uint2 o;
-                o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-                o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                if (EPI == EPI_RELU) {   // forward activations: the pack the fused forward uses
+                    o.x = gpsm::bf16_pack2(v[0], v[1]);
+                    o.y = gpsm::bf16_pack2(v[2], v[3]);
+                } else {
+                    o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                    o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                }
                 *reinterpret_cast<uint2 *>(out + pix * HID + c) = o;
             }
         }
@@ -278,6 +284,177 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t *__restrict__ in
             const uint32_t b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
             lpart[b] = (red[0] + red[1]) + (red[2] + red[3]);
         }
+    }
+}
+
+// ------------------------------------------------------------------ fused forward (inference)
+// All NL layers of one FW x FH output tile in one workgroup.  The feature tile with its NL-pixel halo is staged
+// once; each layer's output over the tile, grown by the halo the later layers still need, stays in LDS as bf16
+// (the values the layer-by-layer kernels store, so both forwards give the same bits).  Layer l's output region is
+// (FW + 2 (NL - 1 - l)) x (FH + 2 (NL - 1 - l)) pixels; its pixels outside the image are stored as zero, the
+// padding the next layer's taps must see.  A region's pixels in row-major order form the 16-pixel columns of the
+// MFMA B operands; wave w computes columns w, w + 16, ..., each column's nine B operands loaded before its MFMAs.  Two activation buffers alternate: FA holds the outputs of the even layers, FB the
+// features and the outputs of the odd layers.  A layer's packed weights and biases are read from one of two LDS
+// buffers; the next layer's are loaded from memory while the layer computes and stored to the other buffer
+// before the layer's closing barrier.  One workgroup per CU (the LDS), sixteen waves.
+constexpr int FW = 32, FH = 16, FWAVES = 16, FTHREADS = FWAVES * 64;
+constexpr int fr_w(int l) { return FW + 2 * (NL - 1 - l); }
+constexpr int fr_h(int l) { return FH + 2 * (NL - 1 - l); }
+constexpr int FIN_W = FW + 2 * NL, FIN_H = FH + 2 * NL;
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// 32-channel regions: 64 B per pixel, its four 16 B chunks (channels 8k..8k+7) at chunk positions (k ^ (p >> 1)) & 3,
+// so the 16-pixel, one-chunk reads of a B operand meet no LDS bank conflict (an unswizzled 80 B stride conflicted on
+// half of the LDS cycles on gfx950)
+__device__ __forceinline__ int fsw(int p, int chunk) { return p * HID + (((chunk ^ (p >> 1)) & 3) << 3); }
+constexpr int F_A = fr_w(0) * fr_h(0) * HID;
+constexpr int F_B = cmax(FIN_W * FIN_H * IN_CH, fr_w(1) * fr_h(1) * HID);
+constexpr int fwt_a(int l) { return ((l == NL - 1 ? 3 : HID) + 15) / 16 * ksteps(l == 0 ? IN_CH : HID) * 64; }
+constexpr int FWT = 2 * 9 * 64 + HID / 4;   // uint4 per weight buffer: a 32 -> 32 layer's A operands, then 32 biases
+constexpr int FWT_LOADS = (FWT + FTHREADS - 1) / FTHREADS;
+constexpr bool FUSED_FITS = (F_A + F_B) * 2 + 2 * FWT * 16 <= 160 * 1024;   // deeper networks: layer by layer
+
+// per layer: the packed A operands followed by the biases (HID floats, zero past cout), as pack_all writes them
+struct FusedNet {
+    const uint4 *w[NL];
+};
+
+template <int L>
+__device__ __forceinline__ void fused_fetch(const FusedNet &net, uint4 (&r)[FWT_LOADS]) {
+    if constexpr (L < NL) {
+        constexpr int N = fwt_a(L) + HID / 4;
+#pragma unroll
+        for (int i = 0; i < FWT_LOADS; i++) {
+            const int t = threadIdx.x + i * FTHREADS;
+            if (t < N) r[i] = net.w[L][t];
+        }
+    }
+}
+template <int L>
+__device__ __forceinline__ void fused_store(uint4 *__restrict__ sw, const uint4 (&r)[FWT_LOADS]) {
+    if constexpr (L < NL) {
+        constexpr int N = fwt_a(L) + HID / 4;
+#pragma unroll
+        for (int i = 0; i < FWT_LOADS; i++) {
+            const int t = threadIdx.x + i * FTHREADS;
+            if (t < N) sw[t] = r[i];
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst, const uint4 *__restrict__ wl,
+                         int x0, int y0, uint32_t W, uint32_t H, const float4 *__restrict__ mean,
+                         float4 *__restrict__ out4) {
+    constexpr bool LAST = L == NL - 1;
+    constexpr int CIN = L == 0 ? IN_CH : HID, MT = LAST ? 1 : 2;
+    constexpr int RW = fr_w(L), R = RW * fr_h(L), SW = RW + 2;   // output region; its input is SW wide
+    constexpr int KS = ksteps(CIN);
+    constexpr int NT = (R + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, g = lane >> 4;
+    const int rx0 = x0 - (NL - 1 - L), ry0 = y0 - (NL - 1 - L);   // image position of the region's first pixel
+    bf16x8 A[MT][KS];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            uint4 v = wl[(mt * KS + s) * 64 + lane];
+            A[mt][s] = *reinterpret_cast<bf16x8 *>(&v);
+        }
+    const float *bl = reinterpret_cast<const float *>(wl + MT * KS * 64);
+    float bias[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) bias[mt][k] = bl[LAST ? k : mt * 16 + 4 * g + k];
+    for (int nt = wave; nt < NT; nt += FWAVES) {
+        const int q = nt * 16 + n;
+        const int qq = q < R ? q : R - 1;   // lanes past the region compute a copy, never stored
+        const int oy = qq / RW, ox = qq - oy * RW;
+        const int base = oy * SW + ox;      // input pixel under tap (0, 0)
+        bf16x8 B[KS];
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            int tap, c0;
+            kmap(CIN, s, g, tap, c0);
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (tap < 9) {
+                const int p = base + (tap / 3) * SW + tap % 3;
+                v = *reinterpret_cast<const uint4 *>(&src[CIN == HID ? fsw(p, g) : p * IN_CH]);
+            }
+            B[s] = *reinterpret_cast<bf16x8 *>(&v);
+        }
+        // all the loads are issued before the first MFMA (the scheduler would otherwise wait on each in turn)
+        // __builtin_amdgcn_sched_barrier(0);
+        f32x4 acc[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < KS; s++)
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt][s], B[s], acc[mt], 0, 0, 0);
+        // epilogue: lane holds output channels mt*16 + 4g + k of region pixel q
+        const int gx = rx0 + ox, gy = ry0 + oy;
+        const bool inside = (uint32_t)gx < W && (uint32_t)gy < H;
+        if (LAST) {
+            if (q < R && g == 0 && inside) {   // channels 0..2 live in lane group 0
+                const size_t pix = (size_t)gy * W + gx;
+                const float4 m = mean[pix];
+                out4[pix] = make_float4(m.x + (acc[0][0] + bias[0][0]), m.y + (acc[0][1] + bias[0][1]),
+                                        m.z + (acc[0][2] + bias[0][2]), 1.0f);
+            }
+        } else if (q < R) {
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) {
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = inside ? fmaxf(acc[mt][k] + bias[mt][k], 0.0f) : 0.0f;
+                const uint2 o = make_uint2(gpsm::bf16_pack2(v[0], v[1]), gpsm::bf16_pack2(v[2], v[3]));
+                *reinterpret_cast<uint2 *>(&dst[fsw(q, 2 * mt + (g >> 1)) + 4 * (g & 1)]) = o;
+            }
+        }
+    }
+}
+
+// layers L..NL-1: even layers read FB and write FA, odd layers the reverse; the last writes the image.  Layer L's
+// weights are in weight buffer L % 2; layer L + 1's are fetched during layer L into the other one.
+template <int L>
+__device__ __forceinline__ void fused_layers(uint16_t *sa, uint16_t *sb, uint4 *sw, const FusedNet &net, int x0, int y0, uint32_t W,
+                          uint32_t H, const float4 *__restrict__ mean, float4 *__restrict__ out4) {
+    if constexpr (L < NL) {
+        uint4 nxt[FWT_LOADS];
+        fused_fetch<L + 1>(net, nxt);
+        fused_layer<L>(L % 2 == 0 ? sb : sa, L % 2 == 0 ? sa : sb, sw + (L % 2) * FWT, x0, y0, W, H, mean, out4);
+        if constexpr (L < NL - 1) {
+            fused_store<L + 1>(sw + ((L + 1) % 2) * FWT, nxt);
+            __syncthreads();
+            fused_layers<L + 1>(sa, sb, sw, net, x0, y0, W, H, mean, out4);
+        }
+    }
+}
+
+// Grid: (ceil(W / FW), ceil(H / FH)).  in: [H][W][8] bf16 features; mean: [H][W] float4; out4: [H][W] float4.
+__global__ __launch_bounds__(FTHREADS) void k_dn_fused(const uint16_t *__restrict__ in, uint32_t W, uint32_t H,
+                                                       FusedNet net, const float4 *__restrict__ mean,
+                                                       float4 *__restrict__ out4) {
+    __shared__ __align__(16) uint16_t sa[FUSED_FITS ? F_A : 8];
+    __shared__ __align__(16) uint16_t sb[FUSED_FITS ? F_B : 8];
+    __shared__ __align__(16) uint4 sw[FUSED_FITS ? 2 * FWT : 1];
+    if constexpr (FUSED_FITS) {
+        const int x0 = (int)blockIdx.x * FW, y0 = (int)blockIdx.y * FH;
+        uint4 w0[FWT_LOADS];
+        fused_fetch<0>(net, w0);
+        for (int t = threadIdx.x; t < FIN_W * FIN_H; t += FTHREADS) {
+            const int gx = x0 - NL + t % FIN_W, gy = y0 - NL + t / FIN_W;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gx >= 0 && gy >= 0 && gx < (int)W && gy < (int)H)
+                v = reinterpret_cast<const uint4 *>(in)[(size_t)gy * W + gx];
+            *reinterpret_cast<uint4 *>(&sb[t * IN_CH]) = v;
+        }
+        fused_store<0>(sw, w0);
+        __syncthreads();
+        fused_layers<0>(sa, sb, sw, net, x0, y0, W, H, mean, out4);
     }
 }
 
@@ -393,6 +570,11 @@ __global__ void k_dn_pack(const float *__restrict__ w, int cin, int cout, int ba
     out[t] = o;
 }
 
+// the biases of a layer after its packed forward operands (HID floats, zero past cout)
+__global__ void k_dn_bias_tail(const float *__restrict__ b, int cout, float *__restrict__ out) {
+    if (threadIdx.x < HID) out[threadIdx.x] = (int)threadIdx.x < cout ? b[threadIdx.x] : 0.0f;
+}
+
 // Per 16x16 tile: ratio = mean over pixels and channels of (denoised - mean)^2 over the mean of the variance
 // of the mean.  Above `limit` the tile takes the plain mean.  counts[0] += 1 per fallback tile.
 __global__ void k_dn_fallback(float4 *__restrict__ out, const float4 *__restrict__ mean, uint32_t W, uint32_t H,
@@ -505,6 +687,7 @@ struct GpsDenoiser {
     uint64_t part_cap = 0, lpart_cap = 0, ratio_cap = 0;
     uint32_t *counts = nullptr;
     uint32_t last_w = 0, last_h = 0;
+    bool fused = true;  // inference forward: one fused kernel (FUSED_FITS) or the layer-by-layer kernels
     // training pairs
     uint32_t pair_w = 0, pair_h = 0, pairs = 0, pair_cap = 0;
     uint16_t *px = nullptr;
@@ -540,6 +723,8 @@ int pack_all(GpsDenoiser *d) {
         int total = ((rows + 15) / 16) * ksteps(kcin) * 64;
         hipLaunchKernelGGL(k_dn_pack, dim3(cdiv(total, 256)), dim3(256), 0, d->stream, d->w + d->lay.w[l], cin, cout,
                            0, rows, kcin, d->fwd[l]);
+        hipLaunchKernelGGL(k_dn_bias_tail, dim3(1), dim3(HID), 0, d->stream, d->w + d->lay.b[l], cout,
+                           reinterpret_cast<float *>(d->fwd[l] + total));
         if (l > 0) {
             rows = cin;
             kcin = cout == 3 ? IN_CH : HID;
@@ -614,6 +799,19 @@ void forward(GpsDenoiser *d, uint32_t W, uint32_t H, uint32_t imgs, const float4
                             d->mean, d->out, target, d->dy8, gscale, d->lpart, d->bweight);
 }
 
+// inference forward of one W x H image in d->act[0] / d->mean into d->out: the fused kernel when the network's
+// regions fit the LDS and the handle asks for it, else the layer-by-layer kernels (the same bits)
+void forward_infer(GpsDenoiser *d, uint32_t W, uint32_t H) {
+    if (FUSED_FITS && d->fused) {
+        FusedNet net;
+        for (int l = 0; l < NL; l++) net.w[l] = d->fwd[l];
+        hipLaunchKernelGGL(k_dn_fused, dim3(cdiv(W, FW), cdiv(H, FH)), dim3(FTHREADS), 0, d->stream, d->act[0], W, H,
+                           net, d->mean, d->out);
+    } else {
+        forward(d, W, H, 1, nullptr, 0.0f);
+    }
+}
+
 // backward from d->dy8 (loss gradient of the output) to the gradient of every parameter in d->g
 void backward(GpsDenoiser *d, uint32_t W, uint32_t H, uint32_t imgs) {
     const Layout &L = d->lay;
@@ -667,7 +865,8 @@ int gps_denoiser_create(int device, uint32_t seed, GpsDenoiser **out) {
         hipMalloc(&d->counts, 16) != hipSuccess)
         return bad("alloc");
     for (int l = 0; l < NL; l++) {
-        const size_t fw = (size_t)((L_COUT[l] + 15) / 16) * ksteps(L_CIN[l]) * 64 * 16;
+        // forward operands, then the biases (HID floats) the fused forward reads with them
+        const size_t fw = (size_t)((L_COUT[l] + 15) / 16) * ksteps(L_CIN[l]) * 64 * 16 + HID * sizeof(float);
         const size_t bw = (size_t)((L_CIN[l] + 15) / 16) * ksteps(L_COUT[l] == 3 ? IN_CH : HID) * 64 * 16;
         if (hipMalloc(&d->fwd[l], fw) != hipSuccess || hipMalloc(&d->bwd[l], bw) != hipSuccess) return bad("alloc");
     }
@@ -769,7 +968,7 @@ int gps_denoiser_apply(GpsDenoiser *d, GpsRenderer *r, float fallback_ratio, Gps
     DN_TRY(d, hipEventCreate(&e1));
     int rc = load_renderer(d, r, W, H, e0);
     if (rc == 0) {
-        forward(d, W, H, 1, nullptr, 0.0f);
+        forward_infer(d, W, H);
         const dim3 tg(cdiv(W, FB_TILE), cdiv(H, FB_TILE));
         hipLaunchKernelGGL(k_dn_zero, dim3(1), dim3(64), 0, d->stream, d->counts, 4u);
         if (fallback_ratio > 0.0f)
@@ -797,6 +996,11 @@ int gps_denoiser_apply(GpsDenoiser *d, GpsRenderer *r, float fallback_ratio, Gps
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
+}
+
+int gps_denoiser_set_fused(GpsDenoiser *d, int enable) {
+    d->fused = enable != 0;
+    return FUSED_FITS ? 0 : 1;
 }
 
 int gps_denoiser_read(GpsDenoiser *d, float *rgba) {

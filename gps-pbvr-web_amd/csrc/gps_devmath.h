@@ -204,4 +204,13 @@ GPS_DEV uint32_t f2u(float f) {
 GPS_DEV float clamp(float x, float lo, float hi) { return fmin(fmax(x, lo), hi); }
 GPS_DEV float mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
+// Two floats to a packed bf16 pair (first in the low half), round to nearest even: one v_cvt_pk_bf16_f32 on gfx950.
+// For every finite float it equals the integer rounding u + 0x7FFF + ((u >> 16) & 1) of the bit pattern (checked
+// over all 2^32 patterns but NaN by the device math tests).  Used for the denoiser's forward activations.
+GPS_DEV uint32_t bf16_pack2(float a, float b) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const bf16x2_t v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
 }  // namespace gpsm

@@ -194,6 +194,7 @@ _sig("gps_denoiser_set_params", C.c_int, [C.c_void_p, P(f32)])
 _sig("gps_denoiser_get_params", C.c_int, [C.c_void_p, P(f32)])
 _sig("gps_denoiser_apply", C.c_int, [C.c_void_p, C.c_void_p, f32, P(DenoiseInfo)])
 _sig("gps_denoiser_read", C.c_int, [C.c_void_p, P(f32)])
+_sig("gps_denoiser_set_fused", C.c_int, [C.c_void_p, C.c_int])
 _sig("gps_denoiser_read_features", C.c_int, [C.c_void_p, P(f32), P(f32)])
 _sig("gps_denoiser_add_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p])
 _sig("gps_denoiser_pairs", u32, [C.c_void_p])
@@ -240,7 +241,7 @@ EXPORTED = [
     "gps_denoiser_set_params", "gps_denoiser_get_params", "gps_denoiser_apply", "gps_denoiser_read",
     "gps_denoiser_read_features", "gps_denoiser_add_pair", "gps_denoiser_pairs", "gps_denoiser_clear_pairs",
     "gps_denoiser_train", "gps_denoiser_loss_grad", "gps_denoiser_read_output", "gps_denoiser_pair_weights",
-    "gps_denoiser_set_pair_weights", "gps_ply_load", "gps_ply_free", "gps_ply_last_error", "gps_ply_info",
+    "gps_denoiser_set_pair_weights", "gps_denoiser_set_fused", "gps_ply_load", "gps_ply_free", "gps_ply_last_error", "gps_ply_info",
     "gps_ply_read", "gps_ply_to_gpu", "gps_batch_read_stats", "gps_batch_composite",
 ]
 
@@ -696,6 +697,11 @@ class Denoiser:
         self._size = (info.width, info.height)
         return dict(gpu_ms=info.gpu_ms, tiles=info.tiles, fallback_tiles=info.fallback_tiles,
                     fallback_rate=info.fallback_tiles / max(info.tiles, 1))
+
+    def set_fused(self, enable=True):
+        """apply's forward: the fused five-layer kernel (default) or the layer-by-layer kernels (the same bits).
+        Returns False when this build has no fused kernel."""
+        return self._check(lib.gps_denoiser_set_fused(self._h, 1 if enable else 0)) == 0
 
     def read(self):
         w, h = self._size

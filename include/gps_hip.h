@@ -243,6 +243,10 @@ int gps_denoiser_set_params(GpsDenoiser *d, const float *params); /* also resets
 int gps_denoiser_get_params(GpsDenoiser *d, float *params);
 /* denoises renderer r's accumulation (its features must be enabled); waits for the renderer's work */
 int gps_denoiser_apply(GpsDenoiser *d, GpsRenderer *r, float fallback_ratio, GpsDenoiseInfo *info);
+/* apply's forward: 1 (default) one fused kernel for the five layers, whose intermediate activations stay on chip;
+ * 0 the layer-by-layer kernels the trainer runs.  Both give the same bits.  Returns 1 when this build's network
+ * has no fused kernel (it then always runs layer by layer). */
+int gps_denoiser_set_fused(GpsDenoiser *d, int enable);
 int gps_denoiser_read(GpsDenoiser *d, float *rgba);                        /* w*h*4 of the last apply */
 int gps_denoiser_read_features(GpsDenoiser *d, float *x8, float *mean4);    /* last apply's inputs (w*h*8, w*h*4) */
 /* training pairs: the input renderer's features and the target renderer's mean (same size for every pair) */

@@ -57,6 +57,22 @@ __global__ __launch_bounds__(BLOCK) void k_div_eol(unsigned long long *bad) {
     }
 }
 
+// bf16_pack2 (the hardware conversion of the denoiser's activations) against the integer round-to-nearest-even of
+// the bit pattern, for every float but NaN, in both halves of the pair
+__global__ __launch_bounds__(BLOCK) void k_bf16(unsigned long long *bad) {
+    const uint64_t n = 1ull << 32;
+    for (uint64_t i = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; i < n; i += (uint64_t)GRID * BLOCK) {
+        const uint32_t u = (uint32_t)i, v = u ^ 0x80000000u;
+        bool b = false;
+        if ((u & 0x7FFFFFFFu) <= 0x7F800000u) {
+            const uint32_t ru = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16, rv = (v + 0x7FFFu + ((v >> 16) & 1u)) >> 16;
+            const uint32_t got = gpsm::bf16_pack2(__uint_as_float(u), __uint_as_float(v));
+            b = (got & 0xFFFFu) != ru || (got >> 16) != rv;
+        }
+        count_bad(b, bad);
+    }
+}
+
 // sincos_fin's quadrant selects against a switch over the quadrant, for every x with bits in [0, hi]
 __global__ __launch_bounds__(BLOCK) void k_sincos(uint32_t hi, unsigned long long *bad) {
     const uint64_t n = (uint64_t)hi + 1u;
@@ -197,6 +213,10 @@ extern "C" int mathcheck_div_log(unsigned long long *bad) {
 
 extern "C" int mathcheck_div_eol(unsigned long long *bad) {
     return run(bad, [](unsigned long long *d) { hipLaunchKernelGGL(k_div_eol, dim3(GRID), dim3(BLOCK), 0, 0, d); });
+}
+
+extern "C" int mathcheck_bf16(unsigned long long *bad) {
+    return run(bad, [](unsigned long long *d) { hipLaunchKernelGGL(k_bf16, dim3(GRID), dim3(BLOCK), 0, 0, d); });
 }
 
 extern "C" int mathcheck_sincos(uint32_t hi_bits, unsigned long long *bad) {

@@ -73,6 +73,29 @@ def test_features_and_forward_match_reference(w, h, spp):
     r.close()
 
 
+@pytest.mark.parametrize("w,h", [(100, 70), (64, 4), (131, 33), (33, 17), (1280, 720)])
+def test_fused_forward_equals_layered(w, h):
+    """The fused forward (one workgroup per 32 x 16 output tile, the five layers' activations kept in LDS with the
+    halo each later layer needs) gives the bits of the layer-by-layer kernels, with partial tiles in x and y, images
+    smaller than one tile's halo, and at 1280 x 720; with the fallback on as well."""
+    say(f"[denoise fused] {w}x{h}")
+    r = render(w, h, 4, 3)
+    d = gh.Denoiser(seed=5)
+    d.set_params(random_params(11))
+    out = {}
+    for fused in (False, True):
+        assert d.set_fused(fused)
+        out[fused] = []
+        for ratio in (0.0, 4.0):
+            info = d.apply(r, fallback_ratio=ratio)
+            out[fused].append((d.read(), info["fallback_tiles"]))
+    for (a, fa), (b, fb) in zip(out[False], out[True]):
+        assert fa == fb
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    d.close()
+    r.close()
+
+
 def make_pairs(d, n, w=96, h=64, inp_ens=4, tgt_ens=64):
     scene = S.small_uniform(4096, seed=0)
     rs = []

@@ -1,6 +1,6 @@
 """Exhaustive checks of the specialised device math (gps_devmath.h: div_normal, sqrt_normal and the
 branch-free quadrant selects of sincos_fin) against the general correctly rounded operations, bit for bit,
-over every argument the particle kernels give them.  The kernels are in tests/gpu_math_check.hip."""
+over every argument the particle kernels give them; and of the denoiser's hardware bf16 rounding.  The kernels are in tests/gpu_math_check.hip."""
 import ctypes
 import os
 import struct
@@ -21,7 +21,7 @@ def lib():
     if not os.path.exists(LIB):
         raise RuntimeError(f"{LIB} is missing: run __graft_entry__.build() first")
     lib = ctypes.CDLL(LIB)
-    for name in ("mathcheck_sqrt", "mathcheck_div_log", "mathcheck_div_eol", "mathcheck_sincos"):
+    for name in ("mathcheck_sqrt", "mathcheck_div_log", "mathcheck_div_eol", "mathcheck_sincos", "mathcheck_bf16"):
         getattr(lib, name).restype = ctypes.c_int
     return lib
 
@@ -52,6 +52,11 @@ def test_div_normal_extinction_quotient(lib):
 def test_sincos_selects(lib):
     # the particle kernels take sin/cos of 2 pi u, u in [0, 1): every float in [0, 6.2832]
     assert _bad(lib.mathcheck_sincos, ctypes.c_uint32(fbits(6.2832))) == 0
+
+
+def test_bf16_pack_rounds_to_nearest_even(lib):
+    # the denoiser's forward activations: v_cvt_pk_bf16_f32 equals the integer rounding for every float but NaN
+    assert _bad(lib.mathcheck_bf16) == 0
 
 
 NAMES = ["E_R", "E_T", "ETA_TE", "TAU_E", "ETA_TF", "TAU_F", "ETA_HE", "ETA_HE_X", "ETA_HF", "ETA_HF_X", "E_RCP",
