@@ -107,10 +107,12 @@ struct Layout {
 // c0..2 mean rgb; c3 per-ensemble standard deviation (mean over rgb of the variance, square-rooted);
 // c4 8 x standard error of the mean; c5 1/(1 + nearest depth) (0 without a hit); c6 covered fraction of
 // subpixel samples; c7 zero.
+// counts (4 words) are cleared for the fallback that follows.
 __global__ void k_dn_features(const float4 *__restrict__ accum, const float4 *__restrict__ mom,
                               const float *__restrict__ zmin, uint32_t X, float spp, uint16_t *__restrict__ x8,
-                              float4 *__restrict__ mean) {
+                              float4 *__restrict__ mean, uint32_t *__restrict__ counts) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4u) counts[i] = 0u;
     if (i >= X) return;
     const float4 a = accum[i];
     const float4 m = mom[i];
@@ -297,7 +299,10 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t *__restrict__ in
 // features and the outputs of the odd layers.  A layer's packed weights and biases are read from one of two LDS
 // buffers; the next layer's are loaded from memory while the layer computes and stored to the other buffer
 // before the layer's closing barrier.  One workgroup per CU (the LDS), sixteen waves.
-constexpr int FW = 32, FH = 16, FWAVES = 16, FTHREADS = FWAVES * 64;
+#ifndef GPS_DN_FWAVES
+#define GPS_DN_FWAVES 16   // measured at 1280 x 720: 16 waves 0.155 ms per forward, 12 waves 0.160, 8 waves 0.182
+#endif
+constexpr int FW = 32, FH = 16, FWAVES = GPS_DN_FWAVES, FTHREADS = FWAVES * 64;
 constexpr int fr_w(int l) { return FW + 2 * (NL - 1 - l); }
 constexpr int fr_h(int l) { return FH + 2 * (NL - 1 - l); }
 constexpr int FIN_W = FW + 2 * NL, FIN_H = FH + 2 * NL;
@@ -325,7 +330,7 @@ __device__ __forceinline__ void fused_fetch(const FusedNet &net, uint4 (&r)[FWT_
 #pragma unroll
         for (int i = 0; i < FWT_LOADS; i++) {
             const int t = threadIdx.x + i * FTHREADS;
-            if (t < N) r[i] = net.w[L][t];
+            r[i] = net.w[L][t < N ? t : N - 1];   // every element written: the array stays in registers
         }
     }
 }
@@ -384,8 +389,6 @@ __device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, ui
             }
             B[s] = *reinterpret_cast<bf16x8 *>(&v);
         }
-        // all the loads are issued before the first MFMA (the scheduler would otherwise wait on each in turn)
-        // __builtin_amdgcn_sched_barrier(0);
         f32x4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -633,10 +636,6 @@ __global__ void k_dn_mean_of(const float4 *__restrict__ accum, uint32_t X, float
     const float4 a = accum[i];
     const float n = a.w > 1.0f ? a.w : 1.0f;
     out[i] = make_float4(a.x / n, a.y / n, a.z / n, 1.0f);
-}
-
-__global__ void k_dn_zero(uint32_t *__restrict__ p, uint32_t n) {
-    if (threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 __global__ void k_dn_sum(const float *__restrict__ part, uint32_t n, float scale, float *__restrict__ out) {
@@ -956,7 +955,7 @@ static int load_renderer(GpsDenoiser *d, GpsRenderer *r, uint32_t &W, uint32_t &
     if (gps_renderer_wait_idle(r) != 0) return dfail(d, "renderer wait failed");
     if (start) (void)hipEventRecord(start, d->stream);
     hipLaunchKernelGGL(k_dn_features, dim3(cdiv(X, 256)), dim3(256), 0, d->stream, accum, mom, zmin, X,
-                       (float)(spp_side * spp_side), d->act[0], d->mean);
+                       (float)(spp_side * spp_side), d->act[0], d->mean, d->counts);
     return 0;
 }
 
@@ -970,7 +969,6 @@ int gps_denoiser_apply(GpsDenoiser *d, GpsRenderer *r, float fallback_ratio, Gps
     if (rc == 0) {
         forward_infer(d, W, H);
         const dim3 tg(cdiv(W, FB_TILE), cdiv(H, FB_TILE));
-        hipLaunchKernelGGL(k_dn_zero, dim3(1), dim3(64), 0, d->stream, d->counts, 4u);
         if (fallback_ratio > 0.0f)
             hipLaunchKernelGGL(k_dn_fallback, tg, dim3(256), 0, d->stream, d->out, d->mean, W, H, fallback_ratio,
                                d->counts, d->ratios);
