@@ -16,6 +16,10 @@ struct GpsFrame {
     float density, near_z, base_k, ref_len;
     float object_depth, jitter, pad2, pad3;   // jitter = sqrt(max(pos.w, 0)), the same for every particle
     float bg[4];
+    // (float)W, (float)H, (float)spp_side, converted on the host (pack_frame): the particle kernels' emitters take
+    // them as uniform operands, where a conversion in the kernel held registers that the seven-wave occupancy
+    // target then spilled inside the particle loop
+    float wf, hf, sidef, pad4;
 };
 
 enum : uint32_t {

@@ -785,10 +785,10 @@ GPS_DEV Hit emit_gps(const GpsFrame &f, Rng &r, float4 v0, float4 v1, const floa
     float wx = rad * cs, wy = rad * sn;
     float pxf = v0.x + v1.x * wx, pyf = v0.y + (v1.y * wx + v1.z * wy);
     h.key = gpsm::bits(v0.z) | 0x80000000u;
-    if (pxf < 0.0f || pxf >= (float)f.W || pyf < 0.0f || pyf >= (float)f.H) return h;
+    if (pxf < 0.0f || pxf >= f.wf || pyf < 0.0f || pyf >= f.hf) return h;
     const uint32_t side = f.spp_side;
     uint32_t px = f2u(pxf), py = f2u(pyf);
-    uint32_t sx = f2u((pxf - (float)px) * (float)side), sy = f2u((pyf - (float)py) * (float)side);
+    uint32_t sx = f2u((pxf - (float)px) * f.sidef), sy = f2u((pyf - (float)py) * f.sidef);
     if (sx > side - 1u) sx = side - 1u;
     if (sy > side - 1u) sy = side - 1u;
     h.idx = (py * f.W + px) * (side * side) + sy * side + sx;
@@ -833,10 +833,10 @@ GPS_DEV Hit emit_pbvr(const GpsFrame &f, Rng &r, const float4 *__restrict__ v) {
     float depth_z = (f.flags & FLAG_CENTRE) ? a.z : camz;
     h.counted = 1u;
     h.key = gpsm::bits(depth_z) | 0x80000000u;
-    if (pixx < 0.0f || pixx >= (float)f.W || pixy < 0.0f || pixy >= (float)f.H) return h;
+    if (pixx < 0.0f || pixx >= f.wf || pixy < 0.0f || pixy >= f.hf) return h;
     const uint32_t side = f.spp_side;
     uint32_t ipx = f2u(pixx), ipy = f2u(pixy);
-    uint32_t sx = f2u((pixx - (float)ipx) * (float)side), sy = f2u((pixy - (float)ipy) * (float)side);
+    uint32_t sx = f2u((pixx - (float)ipx) * f.sidef), sy = f2u((pixy - (float)ipy) * f.sidef);
     if (sx > side - 1u) sx = side - 1u;
     if (sy > side - 1u) sy = side - 1u;
     h.idx = (ipy * f.W + ipx) * (side * side) + sy * side + sx;

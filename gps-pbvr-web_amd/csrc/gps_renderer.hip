@@ -323,6 +323,9 @@ void pack_frame(const GpsRenderer *r, const GpsCamera *cam, const GpsRenderParam
                    r->centroid[2] - cam->view_pos[2]};
     f.object_depth = (float)(cam->view_rot[6] * d[0] + cam->view_rot[7] * d[1] + cam->view_rot[8] * d[2]);
     for (int k = 0; k < 3; k++) f.bg[k] = p->background[k];
+    f.wf = (float)f.W;
+    f.hf = (float)f.H;
+    f.sidef = (float)f.spp_side;
 }
 
 int check_params(GpsRenderer *r, const GpsRenderParams *p) {

@@ -366,6 +366,37 @@ def test_sorted_path_vs_mirror_and_analytic(idx):
     assert np.array_equal(np.sort(vals), np.arange(len(vals)))
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_sorted_blend_reach_skip_with_needles(seed):
+    """Needles, whose minor axis sits at the 1e-12 eigenvalue clamp, and discs at random orientations and depths,
+    sub-pixel to frame-filling, opacities near 1/255 and near 1: keys, values and the accumulation buffer of the
+    sorted alpha path (tile lists of the binned quads) equal the oracle's, which blends every splat at every pixel."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(700):
+        big = rng.uniform(-4.0, 0.5)
+        thin = big - rng.uniform(0.0, 14.0) if i % 3 else big - rng.uniform(0.0, 1.0)
+        ls = [big, thin, rng.uniform(thin, big)]
+        rng.shuffle(ls)
+        q = rng.normal(size=4)
+        out.append(dict(pos=(rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5), rng.uniform(3.0, 9.0)),
+                        log_scale=tuple(ls), rot_xyzw=tuple(q / np.linalg.norm(q)),
+                        opacity=float(rng.choice([0.0045, 0.02, 0.5, 0.98])), color=tuple(rng.uniform(0, 1, 3))))
+    arr = to_gpu(out)
+    say(f"[sorted needles] seed {seed}")
+    p = gh.RenderParams(spp_side=1, background=BG)
+    r = gh.GpsRenderer(W, H, 1)
+    r.set_gaussians(arr)
+    r.render_sorted(CAM, p, 1)
+    acc = r.read_accum()
+    keys, vals = r.read_sorted()
+    r.close()
+    m = O.mirror_sorted(arr, None, 0, ocam(CAM), mparams(p), W, H)
+    assert np.array_equal(keys, m["keys"]) and np.array_equal(vals, m["vals"])
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+    assert (acc[..., :3] != np.asarray(BG[:3], np.float32)).any(axis=-1).mean() > 0.3
+
+
 def test_radix_sort_300k_multilevel():
     """verify.rs:827-853: 300k small splats."""
     say("[sorted] 300k")
