@@ -295,12 +295,14 @@ __global__ __launch_bounds__(256) void k_dn_conv(const uint16_t *__restrict__ in
 // (the values the layer-by-layer kernels store, so both forwards give the same bits).  Layer l's output region is
 // (FW + 2 (NL - 1 - l)) x (FH + 2 (NL - 1 - l)) pixels; its pixels outside the image are stored as zero, the
 // padding the next layer's taps must see.  A region's pixels in row-major order form the 16-pixel columns of the
-// MFMA B operands; wave w computes columns w, w + 16, ..., each column's nine B operands loaded before its MFMAs.  Two activation buffers alternate: FA holds the outputs of the even layers, FB the
-// features and the outputs of the odd layers.  A layer's packed weights and biases are read from one of two LDS
+// MFMA B operands; wave w computes columns w, w + FWAVES, ..., each column's nine B operands loaded before its
+// MFMAs.  Two activation buffers alternate: FA holds the outputs of the even layers, FB the features and the outputs
+// of the odd layers.  A layer's packed weights and biases are read from one of two LDS
 // buffers; the next layer's are loaded from memory while the layer computes and stored to the other buffer
 // before the layer's closing barrier.  One workgroup per CU (the LDS), sixteen waves.
 #ifndef GPS_DN_FWAVES
-#define GPS_DN_FWAVES 16   // measured at 1280 x 720: 16 waves 0.155 ms per forward, 12 waves 0.160, 8 waves 0.182
+#define GPS_DN_FWAVES 16   // at 1280 x 720: 16 waves 0.155 ms per forward, 12 waves 0.160, 8 waves 0.182 (measured
+                           // before the weight prefetch moved out of scratch; 16 waves now 0.141)
 #endif
 constexpr int FW = 32, FH = 16, FWAVES = GPS_DN_FWAVES, FTHREADS = FWAVES * 64;
 constexpr int fr_w(int l) { return FW + 2 * (NL - 1 - l); }
@@ -347,9 +349,9 @@ __device__ __forceinline__ void fused_store(uint4 *__restrict__ sw, const uint4 
 }
 
 template <int L>
-__device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst, const uint4 *__restrict__ wl,
-                         int x0, int y0, uint32_t W, uint32_t H, const float4 *__restrict__ mean,
-                         float4 *__restrict__ out4) {
+__device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, uint16_t *__restrict__ dst,
+                                            const uint4 *__restrict__ wl, int x0, int y0, uint32_t W, uint32_t H,
+                                            const float4 *__restrict__ mean, float4 *__restrict__ out4) {
     constexpr bool LAST = L == NL - 1;
     constexpr int CIN = L == 0 ? IN_CH : HID, MT = LAST ? 1 : 2;
     constexpr int RW = fr_w(L), R = RW * fr_h(L), SW = RW + 2;   // output region; its input is SW wide
@@ -423,8 +425,9 @@ __device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ src, ui
 // layers L..NL-1: even layers read FB and write FA, odd layers the reverse; the last writes the image.  Layer L's
 // weights are in weight buffer L % 2; layer L + 1's are fetched during layer L into the other one.
 template <int L>
-__device__ __forceinline__ void fused_layers(uint16_t *sa, uint16_t *sb, uint4 *sw, const FusedNet &net, int x0, int y0, uint32_t W,
-                          uint32_t H, const float4 *__restrict__ mean, float4 *__restrict__ out4) {
+__device__ __forceinline__ void fused_layers(uint16_t *sa, uint16_t *sb, uint4 *sw, const FusedNet &net, int x0,
+                                             int y0, uint32_t W, uint32_t H, const float4 *__restrict__ mean,
+                                             float4 *__restrict__ out4) {
     if constexpr (L < NL) {
         uint4 nxt[FWT_LOADS];
         fused_fetch<L + 1>(net, nxt);
