@@ -6,10 +6,11 @@
 //   k_sp_project   per splat: cull, GPS demand probe, quad axes, SH colour, 24-bit far->near key
 //   radix sort     stable LSD, 8-bit digits: per-tile digit histograms (digit-major), a decoupled
 //                  look-back scan over them, then a stable scatter ranked with wave64 ballots
-//   binning        each visible splat (in sorted order) emits one (tile, rank) pair per 16x16 tile its
+//   binning        each visible splat (in sorted order) emits one (tile, splat) pair per 16x16 tile its
 //                  quad touches; a stable sort by tile keeps the back-to-front order inside each tile
-//   k_sp_blend     one workgroup per tile: back-to-front blend of the tile's splats at pixel centres in
-//                  f32, written to the accumulation buffer with count 1 (sp_resolve)
+//   k_sp_blend     one workgroup per 8x8 quadrant of a tile: four waves evaluate the splats' alphas, one runs
+//                  the back-to-front blend of the tile's splats at pixel centres in f32, written to the
+//                  accumulation buffer with count 1 (sp_resolve)
 #include <string>
 #include <vector>
 
@@ -383,13 +384,16 @@ __global__ __launch_bounds__(256) void k_sp_emit(const float4 *__restrict__ sp, 
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n || j >= *visible) return;
     uint32_t tx0, ty0, tx1, ty1;
-    quad_tiles(sp, vals[j], W, H, tx0, ty0, tx1, ty1);
+    const uint32_t id = vals[j];
+    quad_tiles(sp, id, W, H, tx0, ty0, tx1, ty1);
     const uint32_t tw = (W + BIN - 1) / BIN;
     uint32_t o = off[j];
+    // the pairs of splat j sit at offsets above those of splats 0..j-1, so the stable sort by tile keeps each
+    // tile's splats in sorted (back-to-front) order; the value is the splat itself, read directly by the blend
     for (uint32_t ty = ty0; ty <= ty1; ty++)
         for (uint32_t tx = tx0; tx <= tx1; tx++) {
             pkeys[o] = ty * tw + tx;
-            pvals[o] = j;
+            pvals[o] = id;
             o++;
         }
 }
@@ -404,48 +408,126 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict_
 }
 
 // back-to-front blend of a tile's splats (gps_sorted_draw.wgsl:100-124 evaluated at pixel centres)
-__global__ __launch_bounds__(256) void k_sp_blend(GpsFrame f, const float4 *__restrict__ sp,
-                                                  const uint32_t *__restrict__ vals,
-                                                  const uint32_t *__restrict__ pvals,
-                                                  const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
-                                                  float4 *__restrict__ accum) {
-    __shared__ float4 s_a[256], s_b[256], s_c[256];
+// One workgroup per 8x8 quadrant of a 16x16 tile (four per tile, so a tile's list is spread over four CUs), four
+// waves per quadrant.  The list is walked in batches of BLEND_BATCH splats staged in LDS, double-buffered: while
+// a batch is evaluated, the staging wave has the next batch's records in flight.  Wave w evaluates the alpha of
+// splats w, w + 4, ... of the batch at its 64 pixels (the divisions and the exp, the costly part, independent
+// across splats) into LDS, with one bit per splat saying whether any of its alphas is non-zero; wave 0 then runs
+// the back-to-front recurrence over the flagged splats in list order, reading them four at a time.  The
+// arithmetic and its order per pixel are those of a one-thread-per-pixel walk of the whole list, so the image
+// has the same bits.
+constexpr int BLEND_QUAD = 8;                                   // quadrant side in pixels
+constexpr int BLEND_WAVES = 4;
+constexpr int BLEND_BATCH = 64;
+constexpr int BLEND_THREADS = 64 * BLEND_WAVES;
+constexpr uint32_t BLEND_STAGE = BLEND_WAVES - 1;               // the wave that loads the next batch
+static_assert(BLEND_QUAD * BLEND_QUAD == 64 && BIN == 2 * BLEND_QUAD, "one wave per quadrant, four per tile");
+static_assert(BLEND_BATCH == 64, "a batch is staged by one wave, one flag bit per splat");
+
+GPS_DEV void blend_over(float &cr, float &cg, float &cb, float alpha, float4 c) {
+    if (alpha > 0.0f) {
+        cr = c.x * alpha + cr * (1.0f - alpha);
+        cg = c.y * alpha + cg * (1.0f - alpha);
+        cb = c.z * alpha + cb * (1.0f - alpha);
+    }
+}
+
+GPS_DEV uint32_t take_bit(uint64_t &m, bool &has) {
+    has = m != 0u;
+    const uint32_t q = has ? (uint32_t)__builtin_ctzll(m) : 0u;
+    m &= m - 1u;
+    return q;
+}
+
+__global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const float4 *__restrict__ sp,
+                                                            const uint32_t *__restrict__ ids,
+                                                            const uint32_t *__restrict__ start,
+                                                            const uint32_t *__restrict__ end,
+                                                            float4 *__restrict__ accum) {
+    __shared__ float4 s_a[2][BLEND_BATCH], s_b[2][BLEND_BATCH], s_c[2][BLEND_BATCH];
+    __shared__ float s_alpha[BLEND_BATCH][64];   // 0: the splat leaves the pixel unchanged
+    __shared__ uint64_t s_any[BLEND_WAVES];      // bit q: some pixel of the quadrant takes splat q
     const uint32_t tw = (f.W + BIN - 1) / BIN;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t x = (tile % tw) * BIN + (threadIdx.x % BIN), y = (tile / tw) * BIN + (threadIdx.x / BIN);
+    const uint32_t tile = blockIdx.x >> 2, quad = blockIdx.x & 3u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t x = (tile % tw) * BIN + (quad & 1u) * BLEND_QUAD + (lane % BLEND_QUAD);
+    const uint32_t y = (tile / tw) * BIN + (quad >> 1) * BLEND_QUAD + (lane / BLEND_QUAD);
     const bool inside = x < f.W && y < f.H;
     float cr = f.bg[0], cg = f.bg[1], cb = f.bg[2];
     const float qx = (float)x + 0.5f, qy = (float)y + 0.5f;
     const uint32_t s0 = start[tile], s1 = end[tile];
-    for (uint32_t base = s0; base < s1; base += 256) {
-        uint32_t p = base + threadIdx.x;
-        if (p < s1) {
-            uint32_t id = vals[pvals[p]];
-            s_a[threadIdx.x] = sp[3u * id];
-            s_b[threadIdx.x] = sp[3u * id + 1u];
-            s_c[threadIdx.x] = sp[3u * id + 2u];
+    uint32_t nid = 0;   // staging wave: this lane's splat in the next batch
+    if (wave == BLEND_STAGE && s0 < s1) {
+        if (lane < s1 - s0) {
+            const uint32_t id = ids[s0 + lane];
+            s_a[0][lane] = sp[3u * id];
+            s_b[0][lane] = sp[3u * id + 1u];
+            s_c[0][lane] = sp[3u * id + 2u];
         }
+        if (lane + BLEND_BATCH < s1 - s0) nid = ids[s0 + BLEND_BATCH + lane];
+    }
+    __syncthreads();
+    uint32_t cur = 0;
+    for (uint32_t base = s0; base < s1; base += BLEND_BATCH, cur ^= 1u) {
+        const uint32_t left = s1 - base, cnt = min((uint32_t)BLEND_BATCH, left);
+        const bool more = lane + BLEND_BATCH < left;
+        float4 ra, rb, rc;
+        uint32_t nnid = 0;
+        if (wave == BLEND_STAGE) {
+            if (more) {
+                ra = sp[3u * nid];
+                rb = sp[3u * nid + 1u];
+                rc = sp[3u * nid + 2u];
+            }
+            if (lane + 2u * BLEND_BATCH < left) nnid = ids[base + 2u * BLEND_BATCH + lane];
+        }
+        uint64_t any = 0;
+        for (uint32_t q = wave; q < cnt; q += BLEND_WAVES) {
+            float alpha = 0.0f;
+            if (inside) {
+                const float4 a = s_a[cur][q], b = s_b[cur][q];
+                const float dx = qx - a.x, dy = qy - a.y;
+                const float u = (dx * b.x + dy * b.y) / (b.x * b.x + b.y * b.y);
+                const float v = (dx * b.z + dy * b.w) / (b.z * b.z + b.w * b.w);
+                const float r2 = u * u + v * v;
+                if (!(r2 > a.w * a.w)) {
+                    const float al = fmin(a.z * gpsm::exp(-0.5f * r2), 0.99f);
+                    if (!(al < SP_MIN_ALPHA)) alpha = al;
+                }
+            }
+            s_alpha[q][lane] = alpha;
+            if (__ballot(alpha > 0.0f)) any |= 1ull << q;
+        }
+        if (lane == 0u) s_any[wave] = any;
         __syncthreads();
-        uint32_t cnt = min(256u, s1 - base);
-        if (inside) {
-            for (uint32_t q = 0; q < cnt; q++) {
-                float4 a = s_a[q], b = s_b[q];
-                float dx = qx - a.x, dy = qy - a.y;
-                float u = (dx * b.x + dy * b.y) / (b.x * b.x + b.y * b.y);
-                float v = (dx * b.z + dy * b.w) / (b.z * b.z + b.w * b.w);
-                float r2 = u * u + v * v;
-                if (r2 > a.w * a.w) continue;
-                float alpha = fmin(a.z * gpsm::exp(-0.5f * r2), 0.99f);
-                if (alpha < SP_MIN_ALPHA) continue;
-                float4 c = s_c[q];
-                cr = c.x * alpha + cr * (1.0f - alpha);
-                cg = c.y * alpha + cg * (1.0f - alpha);
-                cb = c.z * alpha + cb * (1.0f - alpha);
+        if (wave == 0u) {
+            const uint64_t mv = s_any[0] | s_any[1] | s_any[2] | s_any[3];
+            // readfirstlane returns int: widen through uint32_t, or bit 31 would fill the upper half
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(mv >> 32)),
+                           lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)mv);
+            uint64_t m = ((uint64_t)hi << 32) | lo;
+            while (m) {
+                bool h0, h1, h2, h3;
+                const uint32_t q0 = take_bit(m, h0), q1 = take_bit(m, h1), q2 = take_bit(m, h2),
+                               q3 = take_bit(m, h3);
+                const float a0 = s_alpha[q0][lane], a1 = s_alpha[q1][lane], a2 = s_alpha[q2][lane],
+                            a3 = s_alpha[q3][lane];
+                const float4 c0 = s_c[cur][q0], c1 = s_c[cur][q1], c2 = s_c[cur][q2], c3 = s_c[cur][q3];
+                blend_over(cr, cg, cb, a0, c0);
+                if (h1) blend_over(cr, cg, cb, a1, c1);
+                if (h2) blend_over(cr, cg, cb, a2, c2);
+                if (h3) blend_over(cr, cg, cb, a3, c3);
             }
         }
+        if (wave == BLEND_STAGE && more) {
+            s_a[cur ^ 1u][lane] = ra;
+            s_b[cur ^ 1u][lane] = rb;
+            s_c[cur ^ 1u][lane] = rc;
+        }
+        nid = nnid;
         __syncthreads();
     }
-    if (inside) accum[(size_t)y * f.W + x] = make_float4(cr, cg, cb, 1.0f);
+    if (wave == 0u && inside) accum[(size_t)y * f.W + x] = make_float4(cr, cg, cb, 1.0f);
 }
 
 __global__ void k_u32_fill(uint32_t *p, uint32_t n, uint32_t v) {
@@ -666,12 +748,12 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
         hipLaunchKernelGGL(k_tile_ranges, dim3(nblk(npairs, 256)), dim3(256), 0, st, s->pk[pres], (uint32_t)npairs,
                            s->tstart, s->tend);
         SProf p(prof, st, "sp_blend");
-        hipLaunchKernelGGL(k_sp_blend, dim3(s->tiles_cap), dim3(256), 0, st, f, s->sp, svals, s->pv[pres], s->tstart,
-                           s->tend, accum);
+        hipLaunchKernelGGL(k_sp_blend, dim3(4u * s->tiles_cap), dim3(BLEND_THREADS), 0, st, f, s->sp, s->pv[pres],
+                           s->tstart, s->tend, accum);
     } else {
         SProf p(prof, st, "sp_blend");
-        hipLaunchKernelGGL(k_sp_blend, dim3(s->tiles_cap), dim3(256), 0, st, f, s->sp, svals, s->pv[0], s->tstart,
-                           s->tend, accum);
+        hipLaunchKernelGGL(k_sp_blend, dim3(4u * s->tiles_cap), dim3(BLEND_THREADS), 0, st, f, s->sp, s->pv[0],
+                           s->tstart, s->tend, accum);
     }
     S_TRY(hipGetLastError());
     return 0;
