@@ -409,13 +409,14 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict_
 
 // back-to-front blend of a tile's splats (gps_sorted_draw.wgsl:100-124 evaluated at pixel centres)
 // One workgroup per 8x8 quadrant of a 16x16 tile (four per tile, so a tile's list is spread over four CUs), four
-// waves per quadrant.  The list is walked in batches of BLEND_BATCH splats staged in LDS, double-buffered: while
-// a batch is evaluated, the staging wave has the next batch's records in flight.  Wave w evaluates the alpha of
-// splats w, w + 4, ... of the batch at its 64 pixels (the divisions and the exp, the costly part, independent
-// across splats) into LDS, with one bit per splat saying whether any of its alphas is non-zero; wave 0 then runs
-// the back-to-front recurrence over the flagged splats in list order, reading them four at a time.  The
-// arithmetic and its order per pixel are those of a one-thread-per-pixel walk of the whole list, so the image
-// has the same bits.
+// waves per quadrant.  The list is walked in batches of BLEND_BATCH entries.  The staging wave loads a batch's
+// records while the previous batch is evaluated and keeps, in list order, only the splats whose binning box
+// (quad_tiles' box, the one that decides which tiles list a splat) overlaps the quadrant: outside that box a
+// splat's alpha is zero, as the tile lists already assume.  Wave w then evaluates the alpha of kept splats w,
+// w + 4, ... at its 64 pixels (the divisions and the exp, the costly part, independent across splats) into LDS,
+// with one bit per splat saying whether any of its alphas is non-zero; wave 0 runs the back-to-front recurrence
+// over the flagged splats in list order, reading them four at a time.  The arithmetic and its order per pixel
+// are those of a one-thread-per-pixel walk of the whole list, so the image has the same bits.
 constexpr int BLEND_QUAD = 8;                                   // quadrant side in pixels
 constexpr int BLEND_WAVES = 4;
 constexpr int BLEND_BATCH = 64;
@@ -439,6 +440,28 @@ GPS_DEV uint32_t take_bit(uint64_t &m, bool &has) {
     return q;
 }
 
+// quad_tiles' box of splat (a, b) overlaps pixel columns [qx0, qx0 + 7] and rows [qy0, qy0 + 7]; written with
+// negated comparisons so that a NaN keeps the splat (its alpha is then evaluated as everywhere else)
+GPS_DEV bool box_touches(float4 a, float4 b, float qx0, float qy0) {
+    const float ex = a.w * (fabsf(b.x) + fabsf(b.z)) + 1.0f, ey = a.w * (fabsf(b.y) + fabsf(b.w)) + 1.0f;
+    return !(floorf(a.x - ex) > qx0 + (float)(BLEND_QUAD - 1)) && !(ceilf(a.x + ex) < qx0) &&
+           !(floorf(a.y - ey) > qy0 + (float)(BLEND_QUAD - 1)) && !(ceilf(a.y + ey) < qy0);
+}
+
+// staging wave: keep the lanes whose splat touches the quadrant, packed in lane (list) order; returns the count
+GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0, float qy0, float4 *s_a,
+                             float4 *s_b, float4 *s_c, uint32_t lane) {
+    const bool keep = valid && box_touches(a, b, qx0, qy0);
+    const uint64_t mask = __ballot(keep);
+    if (keep) {
+        const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        s_a[r] = a;
+        s_b[r] = b;
+        s_c[r] = c;
+    }
+    return (uint32_t)__popcll(mask);
+}
+
 __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const float4 *__restrict__ sp,
                                                             const uint32_t *__restrict__ ids,
                                                             const uint32_t *__restrict__ start,
@@ -446,32 +469,38 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
                                                             float4 *__restrict__ accum) {
     __shared__ float4 s_a[2][BLEND_BATCH], s_b[2][BLEND_BATCH], s_c[2][BLEND_BATCH];
     __shared__ float s_alpha[BLEND_BATCH][64];   // 0: the splat leaves the pixel unchanged
-    __shared__ uint64_t s_any[BLEND_WAVES];      // bit q: some pixel of the quadrant takes splat q
+    __shared__ uint64_t s_any[BLEND_WAVES];      // bit q: some pixel of the quadrant takes kept splat q
+    __shared__ uint32_t s_kept[2];               // kept splats per staged batch
     const uint32_t tw = (f.W + BIN - 1) / BIN;
     const uint32_t tile = blockIdx.x >> 2, quad = blockIdx.x & 3u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t x = (tile % tw) * BIN + (quad & 1u) * BLEND_QUAD + (lane % BLEND_QUAD);
-    const uint32_t y = (tile / tw) * BIN + (quad >> 1) * BLEND_QUAD + (lane / BLEND_QUAD);
+    const uint32_t qx0 = (tile % tw) * BIN + (quad & 1u) * BLEND_QUAD, qy0 = (tile / tw) * BIN + (quad >> 1) * BLEND_QUAD;
+    const uint32_t x = qx0 + (lane % BLEND_QUAD), y = qy0 + (lane / BLEND_QUAD);
+    const float fqx0 = (float)qx0, fqy0 = (float)qy0;
     const bool inside = x < f.W && y < f.H;
     float cr = f.bg[0], cg = f.bg[1], cb = f.bg[2];
     const float qx = (float)x + 0.5f, qy = (float)y + 0.5f;
     const uint32_t s0 = start[tile], s1 = end[tile];
     uint32_t nid = 0;   // staging wave: this lane's splat in the next batch
     if (wave == BLEND_STAGE && s0 < s1) {
-        if (lane < s1 - s0) {
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c = a;
+        const bool valid = lane < s1 - s0;
+        if (valid) {
             const uint32_t id = ids[s0 + lane];
-            s_a[0][lane] = sp[3u * id];
-            s_b[0][lane] = sp[3u * id + 1u];
-            s_c[0][lane] = sp[3u * id + 2u];
+            a = sp[3u * id];
+            b = sp[3u * id + 1u];
+            c = sp[3u * id + 2u];
         }
         if (lane + BLEND_BATCH < s1 - s0) nid = ids[s0 + BLEND_BATCH + lane];
+        const uint32_t kept = stage_batch(valid, a, b, c, fqx0, fqy0, s_a[0], s_b[0], s_c[0], lane);
+        if (lane == 0u) s_kept[0] = kept;
     }
     __syncthreads();
     uint32_t cur = 0;
     for (uint32_t base = s0; base < s1; base += BLEND_BATCH, cur ^= 1u) {
-        const uint32_t left = s1 - base, cnt = min((uint32_t)BLEND_BATCH, left);
+        const uint32_t left = s1 - base, cnt = s_kept[cur];
         const bool more = lane + BLEND_BATCH < left;
-        float4 ra, rb, rc;
+        float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra, rc = ra;
         uint32_t nnid = 0;
         if (wave == BLEND_STAGE) {
             if (more) {
@@ -519,10 +548,10 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
                 if (h3) blend_over(cr, cg, cb, a3, c3);
             }
         }
-        if (wave == BLEND_STAGE && more) {
-            s_a[cur ^ 1u][lane] = ra;
-            s_b[cur ^ 1u][lane] = rb;
-            s_c[cur ^ 1u][lane] = rc;
+        if (wave == BLEND_STAGE && left > (uint32_t)BLEND_BATCH) {
+            const uint32_t kept = stage_batch(more, ra, rb, rc, fqx0, fqy0, s_a[cur ^ 1u], s_b[cur ^ 1u],
+                                              s_c[cur ^ 1u], lane);
+            if (lane == 0u) s_kept[cur ^ 1u] = kept;
         }
         nid = nnid;
         __syncthreads();
