@@ -466,13 +466,14 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
                                                             const uint32_t *__restrict__ ids,
                                                             const uint32_t *__restrict__ start,
                                                             const uint32_t *__restrict__ end,
+                                                            const uint32_t *__restrict__ order,
                                                             float4 *__restrict__ accum) {
     __shared__ float4 s_a[2][BLEND_BATCH], s_b[2][BLEND_BATCH], s_c[2][BLEND_BATCH];
     __shared__ float s_alpha[BLEND_BATCH][64];   // 0: the splat leaves the pixel unchanged
     __shared__ uint64_t s_any[BLEND_WAVES];      // bit q: some pixel of the quadrant takes kept splat q
     __shared__ uint32_t s_kept[2];               // kept splats per staged batch
     const uint32_t tw = (f.W + BIN - 1) / BIN;
-    const uint32_t tile = blockIdx.x >> 2, quad = blockIdx.x & 3u;
+    const uint32_t tile = order[blockIdx.x >> 2], quad = blockIdx.x & 3u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t qx0 = (tile % tw) * BIN + (quad & 1u) * BLEND_QUAD, qy0 = (tile / tw) * BIN + (quad >> 1) * BLEND_QUAD;
     const uint32_t x = qx0 + (lane % BLEND_QUAD), y = qy0 + (lane / BLEND_QUAD);
@@ -559,6 +560,31 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
     if (wave == 0u && inside) accum[(size_t)y * f.W + x] = make_float4(cr, cg, cb, 1.0f);
 }
 
+// The tiles in order of decreasing list length, by powers of two (the order within one power is whatever the LDS
+// atomics give; each pixel is written by one workgroup either way).  The blend's workgroups take their tiles in
+// this order, so the quadrants of the longest lists start first instead of behind thousands of short ones.
+constexpr int ORDER_THREADS = 1024;
+__global__ __launch_bounds__(ORDER_THREADS) void k_tile_order(const uint32_t *__restrict__ start,
+                                                              const uint32_t *__restrict__ end, uint32_t tiles,
+                                                              uint32_t *__restrict__ order) {
+    __shared__ uint32_t s_cnt[33];   // slot = leading zeros of the list length: the longest lists first
+    if (threadIdx.x < 33u) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < tiles; t += ORDER_THREADS) atomicAdd(&s_cnt[__clz(end[t] - start[t])], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        uint32_t run = 0;
+        for (int b = 0; b < 33; b++) {
+            const uint32_t c = s_cnt[b];
+            s_cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < tiles; t += ORDER_THREADS)
+        order[atomicAdd(&s_cnt[__clz(end[t] - start[t])], 1u)] = t;
+}
+
 __global__ void k_u32_fill(uint32_t *p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -584,7 +610,7 @@ struct SortedState {
     uint32_t *cnt = nullptr;          // per sorted rank: tiles touched, then exclusive offsets
     uint32_t *pk[2] = {nullptr, nullptr}, *pv[2] = {nullptr, nullptr};
     uint64_t pair_cap = 0;
-    uint32_t *tstart = nullptr, *tend = nullptr;
+    uint32_t *tstart = nullptr, *tend = nullptr, *torder = nullptr;
     uint32_t tiles_cap = 0;
     uint32_t epoch = 1;
     int result = 0;                   // which keys/vals buffer holds the last sort
@@ -614,6 +640,7 @@ void sorted_destroy(SortedState *s) {
     sfree(s->cnt);
     sfree(s->tstart);
     sfree(s->tend);
+    sfree(s->torder);
     delete s;
 }
 
@@ -702,6 +729,7 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
         sfree(s->cnt);
         sfree(s->tstart);
         sfree(s->tend);
+        sfree(s->torder);
         S_TRY(hipMalloc(&s->sp, (uint64_t)n * 48));
         for (int k = 0; k < 2; k++) {
             S_TRY(hipMalloc(&s->keys[k], (uint64_t)n * 4));
@@ -716,6 +744,7 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
         s->tiles_cap = ((f.W + BIN - 1) / BIN) * ((f.H + BIN - 1) / BIN);
         S_TRY(hipMalloc(&s->tstart, (uint64_t)s->tiles_cap * 4));
         S_TRY(hipMalloc(&s->tend, (uint64_t)s->tiles_cap * 4));
+        S_TRY(hipMalloc(&s->torder, (uint64_t)s->tiles_cap * 4));
         s->n = n;
         s->W = f.W;
         s->H = f.H;
@@ -765,6 +794,7 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
     }
     hipLaunchKernelGGL(k_u32_fill, dim3(nblk(s->tiles_cap, 256)), dim3(256), 0, st, s->tstart, s->tiles_cap, 0u);
     hipLaunchKernelGGL(k_u32_fill, dim3(nblk(s->tiles_cap, 256)), dim3(256), 0, st, s->tend, s->tiles_cap, 0u);
+    int pres = 0;   // which pair buffer holds the tile-sorted pairs (with none, nothing is read from it)
     if (npairs) {
         hipLaunchKernelGGL(k_sp_emit, dim3(nblk(n, 256)), dim3(256), 0, st, s->sp, svals, s->visible, n, f.W, f.H,
                            s->cnt, s->pk[0], s->pv[0]);
@@ -772,17 +802,16 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
         while ((1u << bits) < s->tiles_cap) bits++;
         bits = (bits + 7) / 8 * 8;
         if (bits == 0) bits = 8;
-        int pres = 0;
         if (radix_sort(s, st, s->pk, s->pv, (uint32_t)npairs, bits, pres, err, prof) != 0) return -1;
         hipLaunchKernelGGL(k_tile_ranges, dim3(nblk(npairs, 256)), dim3(256), 0, st, s->pk[pres], (uint32_t)npairs,
                            s->tstart, s->tend);
+    }
+    {
         SProf p(prof, st, "sp_blend");
-        hipLaunchKernelGGL(k_sp_blend, dim3(4u * s->tiles_cap), dim3(BLEND_THREADS), 0, st, f, s->sp, s->pv[pres],
-                           s->tstart, s->tend, accum);
-    } else {
-        SProf p(prof, st, "sp_blend");
-        hipLaunchKernelGGL(k_sp_blend, dim3(4u * s->tiles_cap), dim3(BLEND_THREADS), 0, st, f, s->sp, s->pv[0],
-                           s->tstart, s->tend, accum);
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(ORDER_THREADS), 0, st, s->tstart, s->tend, s->tiles_cap,
+                           s->torder);
+        hipLaunchKernelGGL(k_sp_blend, dim3(4u * s->tiles_cap), dim3(BLEND_THREADS), 0, st, f, s->sp,
+                           s->pv[pres], s->tstart, s->tend, s->torder, accum);
     }
     S_TRY(hipGetLastError());
     return 0;
