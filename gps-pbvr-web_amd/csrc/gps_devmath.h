@@ -69,10 +69,8 @@ GPS_DEV float sqrt_normal(float x) {
     return fma(-up, s, x) > 0.0f ? up : t;
 }
 
-// exp of x in [-103.97, 88.72] (the checked entry points below handle the rest)
-GPS_DEV float exp_core(float x) {
-    float kf = rint(x * LOG2E);
-    int k = (int)kf;
+// exp(x) / 2^kf for kf = rint(x log2 e): the reduction and the polynomial
+GPS_DEV float exp_reduced(float x, float kf) {
     float r = fma(-kf, LN2_HI, x);
     r = fma(-kf, LN2_LO, r);
     float p = 0x1.a01a02p-13f;
@@ -83,6 +81,14 @@ GPS_DEV float exp_core(float x) {
     p = fma(p, r, 0.5f);
     p = fma(p, r, 1.0f);
     p = fma(p, r, 1.0f);
+    return p;
+}
+
+// exp of x in [-103.97, 88.72] (the checked entry points below handle the rest)
+GPS_DEV float exp_core(float x) {
+    float kf = rint(x * LOG2E);
+    int k = (int)kf;
+    float p = exp_reduced(x, kf);
     if (k > 127) return p * fbits(0x7f000000u) * fbits((uint32_t)(k - 127 + 127) << 23);
     if (k < -126) return p * fbits((uint32_t)(k + 64 + 127) << 23) * 0x1p-64f;
     return p * fbits((uint32_t)(k + 127) << 23);
@@ -99,6 +105,14 @@ GPS_DEV float exp(float x) {
 GPS_DEV float exp_nonpos(float x) {
     if (x < -103.972084045410156f) return 0.0f;
     return exp_core(x);
+}
+
+// exp of a finite x in [-80, 0]: exp_core's steps with 2^k applied by v_ldexp_f32.  The result p 2^k is a
+// normal float there, so the scaling is exact, as the multiplication by the power of two is: the same bits as
+// exp() (checked for every such x by the device math tests), without its range checks.
+GPS_DEV float exp_mid(float x) {
+    const float kf = rint(x * LOG2E);
+    return __builtin_ldexpf(exp_reduced(x, kf), (int)kf);
 }
 
 // log of a positive, normal, finite x times 2^-e0 (e0 undoes a subnormal prescale)

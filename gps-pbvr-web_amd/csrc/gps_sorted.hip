@@ -448,9 +448,11 @@ GPS_DEV bool box_touches(float4 a, float4 b, float qx0, float qy0) {
            !(floorf(a.y - ey) > qy0 + (float)(BLEND_QUAD - 1)) && !(ceilf(a.y + ey) < qy0);
 }
 
-// staging wave: keep the lanes whose splat touches the quadrant, packed in lane (list) order; returns the count
+// staging wave: keep the lanes whose splat touches the quadrant, packed in lane (list) order, with the splat's
+// per-pixel constants (the two axis denominators and k^2, the same operations as in the per-pixel expressions);
+// returns the count
 GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0, float qy0, float4 *s_a,
-                             float4 *s_b, float4 *s_c, uint32_t lane) {
+                             float4 *s_b, float4 *s_c, float4 *s_d, uint32_t lane) {
     const bool keep = valid && box_touches(a, b, qx0, qy0);
     const uint64_t mask = __ballot(keep);
     if (keep) {
@@ -458,6 +460,7 @@ GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0
         s_a[r] = a;
         s_b[r] = b;
         s_c[r] = c;
+        s_d[r] = make_float4(b.x * b.x + b.y * b.y, b.z * b.z + b.w * b.w, a.w * a.w, 0.0f);
     }
     return (uint32_t)__popcll(mask);
 }
@@ -468,13 +471,14 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
                                                             const uint32_t *__restrict__ end,
                                                             const uint32_t *__restrict__ order,
                                                             float4 *__restrict__ accum) {
-    __shared__ float4 s_a[2][BLEND_BATCH], s_b[2][BLEND_BATCH], s_c[2][BLEND_BATCH];
+    __shared__ float4 s_a[2][BLEND_BATCH], s_b[2][BLEND_BATCH], s_c[2][BLEND_BATCH], s_d[2][BLEND_BATCH];
     __shared__ float s_alpha[BLEND_BATCH][64];   // 0: the splat leaves the pixel unchanged
     __shared__ uint64_t s_any[BLEND_WAVES];      // bit q: some pixel of the quadrant takes kept splat q
     __shared__ uint32_t s_kept[2];               // kept splats per staged batch
     const uint32_t tw = (f.W + BIN - 1) / BIN;
     const uint32_t tile = order[blockIdx.x >> 2], quad = blockIdx.x & 3u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t swave = __builtin_amdgcn_readfirstlane(wave);   // the same for the wave: a scalar loop counter
     const uint32_t qx0 = (tile % tw) * BIN + (quad & 1u) * BLEND_QUAD, qy0 = (tile / tw) * BIN + (quad >> 1) * BLEND_QUAD;
     const uint32_t x = qx0 + (lane % BLEND_QUAD), y = qy0 + (lane / BLEND_QUAD);
     const float fqx0 = (float)qx0, fqy0 = (float)qy0;
@@ -493,13 +497,13 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
             c = sp[3u * id + 2u];
         }
         if (lane + BLEND_BATCH < s1 - s0) nid = ids[s0 + BLEND_BATCH + lane];
-        const uint32_t kept = stage_batch(valid, a, b, c, fqx0, fqy0, s_a[0], s_b[0], s_c[0], lane);
+        const uint32_t kept = stage_batch(valid, a, b, c, fqx0, fqy0, s_a[0], s_b[0], s_c[0], s_d[0], lane);
         if (lane == 0u) s_kept[0] = kept;
     }
     __syncthreads();
     uint32_t cur = 0;
     for (uint32_t base = s0; base < s1; base += BLEND_BATCH, cur ^= 1u) {
-        const uint32_t left = s1 - base, cnt = s_kept[cur];
+        const uint32_t left = s1 - base, cnt = __builtin_amdgcn_readfirstlane(s_kept[cur]);
         const bool more = lane + BLEND_BATCH < left;
         float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra, rc = ra;
         uint32_t nnid = 0;
@@ -512,16 +516,20 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
             if (lane + 2u * BLEND_BATCH < left) nnid = ids[base + 2u * BLEND_BATCH + lane];
         }
         uint64_t any = 0;
-        for (uint32_t q = wave; q < cnt; q += BLEND_WAVES) {
+        for (uint32_t q = swave; q < cnt; q += BLEND_WAVES) {
             float alpha = 0.0f;
             if (inside) {
-                const float4 a = s_a[cur][q], b = s_b[cur][q];
+                const float4 a = s_a[cur][q], b = s_b[cur][q], d = s_d[cur][q];
                 const float dx = qx - a.x, dy = qy - a.y;
-                const float u = (dx * b.x + dy * b.y) / (b.x * b.x + b.y * b.y);
-                const float v = (dx * b.z + dy * b.w) / (b.z * b.z + b.w * b.w);
+                const float u = (dx * b.x + dy * b.y) / d.x;
+                const float v = (dx * b.z + dy * b.w) / d.y;
                 const float r2 = u * u + v * v;
-                if (!(r2 > a.w * a.w)) {
-                    const float al = fmin(a.z * gpsm::exp(-0.5f * r2), 0.99f);
+                if (!(r2 > d.z)) {
+                    // k_sp_project writes k = sqrt(2 ln(o / SP_MIN_ALPHA)) with o <= 1, so here r2 <= k^2 <= 11.1
+                    // or r2 is NaN: exp's argument is in [-5.6, 0], where exp_mid gives exp's bits; a NaN r2
+                    // gives min(NaN, 0.99) = 0.99, as exp(NaN) does
+                    const bool num = r2 == r2;
+                    const float al = num ? fmin(a.z * gpsm::exp_mid(num ? -0.5f * r2 : 0.0f), 0.99f) : 0.99f;
                     if (!(al < SP_MIN_ALPHA)) alpha = al;
                 }
             }
@@ -551,7 +559,7 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
         }
         if (wave == BLEND_STAGE && left > (uint32_t)BLEND_BATCH) {
             const uint32_t kept = stage_batch(more, ra, rb, rc, fqx0, fqy0, s_a[cur ^ 1u], s_b[cur ^ 1u],
-                                              s_c[cur ^ 1u], lane);
+                                              s_c[cur ^ 1u], s_d[cur ^ 1u], lane);
             if (lane == 0u) s_kept[cur ^ 1u] = kept;
         }
         nid = nnid;

@@ -73,6 +73,19 @@ __global__ __launch_bounds__(BLOCK) void k_bf16(unsigned long long *bad) {
     }
 }
 
+// exp_mid(x) == exp(x) for every x in [-80, 0] (bits 0x80000000 .. bits(-80))
+__global__ __launch_bounds__(BLOCK) void k_exp_mid(unsigned long long *bad) {
+    const uint32_t lo = 0x80000000u, n = __float_as_uint(-80.0f) - lo + 1u;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i - threadIdx.x < n; i += GRID * BLOCK) {
+        bool b = false;
+        if (i < n) {
+            const float x = __uint_as_float(lo + i);
+            b = __float_as_uint(gpsm::exp_mid(x)) != __float_as_uint(gpsm::exp(x));
+        }
+        count_bad(b, bad);
+    }
+}
+
 // sincos_fin's quadrant selects against a switch over the quadrant, for every x with bits in [0, hi]
 __global__ __launch_bounds__(BLOCK) void k_sincos(uint32_t hi, unsigned long long *bad) {
     const uint64_t n = (uint64_t)hi + 1u;
@@ -217,6 +230,10 @@ extern "C" int mathcheck_div_eol(unsigned long long *bad) {
 
 extern "C" int mathcheck_bf16(unsigned long long *bad) {
     return run(bad, [](unsigned long long *d) { hipLaunchKernelGGL(k_bf16, dim3(GRID), dim3(BLOCK), 0, 0, d); });
+}
+
+extern "C" int mathcheck_exp_mid(unsigned long long *bad) {
+    return run(bad, [](unsigned long long *d) { hipLaunchKernelGGL(k_exp_mid, dim3(GRID), dim3(BLOCK), 0, 0, d); });
 }
 
 extern "C" int mathcheck_sincos(uint32_t hi_bits, unsigned long long *bad) {

@@ -1,4 +1,4 @@
-"""Exhaustive checks of the specialised device math (gps_devmath.h: div_normal, sqrt_normal and the
+"""Exhaustive checks of the specialised device math (gps_devmath.h: div_normal, sqrt_normal, exp_mid and the
 branch-free quadrant selects of sincos_fin) against the general correctly rounded operations, bit for bit,
 over every argument the particle kernels give them; and of the denoiser's hardware bf16 rounding.  The kernels are in tests/gpu_math_check.hip."""
 import ctypes
@@ -21,7 +21,8 @@ def lib():
     if not os.path.exists(LIB):
         raise RuntimeError(f"{LIB} is missing: run __graft_entry__.build() first")
     lib = ctypes.CDLL(LIB)
-    for name in ("mathcheck_sqrt", "mathcheck_div_log", "mathcheck_div_eol", "mathcheck_sincos", "mathcheck_bf16"):
+    for name in ("mathcheck_sqrt", "mathcheck_div_log", "mathcheck_div_eol", "mathcheck_sincos", "mathcheck_bf16",
+                 "mathcheck_exp_mid"):
         getattr(lib, name).restype = ctypes.c_int
     return lib
 
@@ -52,6 +53,12 @@ def test_div_normal_extinction_quotient(lib):
 def test_sincos_selects(lib):
     # the particle kernels take sin/cos of 2 pi u, u in [0, 1): every float in [0, 6.2832]
     assert _bad(lib.mathcheck_sincos, ctypes.c_uint32(fbits(6.2832))) == 0
+
+
+def test_exp_mid_equals_exp(lib):
+    # the alpha blend's exp(-r2 / 2), r2 <= k^2 <= 2 ln 255: exp_mid (ldexp scaling, no range checks) against exp()
+    # for every float in [-80, 0]
+    assert _bad(lib.mathcheck_exp_mid) == 0
 
 
 def test_bf16_pack_rounds_to_nearest_even(lib):
