@@ -281,8 +281,11 @@ GPS_DEV uint64_t sat_add(uint64_t a, uint64_t b) {
     return s > m ? m : s;
 }
 
+// In-place exclusive scan of data[0..n) (decoupled look-back over 2048-item tiles, sums saturating at 2^36 - 1);
+// with total set, the inclusive sum of all n items is written there too.
 __global__ __launch_bounds__(256) void k_rs_scan(uint32_t *__restrict__ data, uint32_t n, uint32_t *ticket,
-                                                 uint64_t *status, uint32_t tiles, uint32_t epoch) {
+                                                 uint64_t *status, uint32_t tiles, uint32_t epoch,
+                                                 uint64_t *__restrict__ total) {
     constexpr int ITEMS = 8, TILE = 256 * ITEMS;
     __shared__ uint32_t s_tile;
     __shared__ uint64_t s_wave[4];
@@ -344,6 +347,7 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t *__restrict__ data, ui
         if (base + j < n) data[base + j] = (uint32_t)run;
         run += v[j];
     }
+    if (total && base < n && n - base <= (uint32_t)ITEMS) *total = run;   // the thread holding item n - 1
     if (tile == tiles - 1u && threadIdx.x == 0) *ticket = 0u;
 }
 
@@ -613,6 +617,7 @@ struct SortedState {
     float4 *sp = nullptr;
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
     uint32_t *hist = nullptr, *ticket = nullptr, *visible = nullptr;
+    uint64_t *total = nullptr;        // the tile-pair count (inclusive sum of the per-splat tile counts)
     uint64_t *status = nullptr;
     uint64_t status_cap = 0, hist_cap = 0;
     uint32_t *cnt = nullptr;          // per sorted rank: tiles touched, then exclusive offsets
@@ -644,6 +649,7 @@ void sorted_destroy(SortedState *s) {
     sfree(s->hist);
     sfree(s->ticket);
     sfree(s->visible);
+    sfree(s->total);
     sfree(s->status);
     sfree(s->cnt);
     sfree(s->tstart);
@@ -712,7 +718,7 @@ static int radix_sort(SortedState *s, hipStream_t st, uint32_t **kb, uint32_t **
         SProf p(prof, st, "radix_pass");
         hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], n, shift, ntiles, s->hist);
         hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->hist, (uint32_t)hn, s->ticket, s->status,
-                           stiles, s->epoch);
+                           stiles, s->epoch, (uint64_t *)nullptr);
         s->epoch = (s->epoch + 1u) & ((1u << 26) - 1u);
         if (!s->epoch) s->epoch = 1;
         hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], vb[cur], kb[cur ^ 1],
@@ -748,6 +754,7 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
             S_TRY(hipMalloc(&s->ticket, 4));
             S_TRY(hipMemset(s->ticket, 0, 4));
             S_TRY(hipMalloc(&s->visible, 4));
+            S_TRY(hipMalloc(&s->total, 8));
         }
         s->tiles_cap = ((f.W + BIN - 1) / BIN) * ((f.H + BIN - 1) / BIN);
         S_TRY(hipMalloc(&s->tstart, (uint64_t)s->tiles_cap * 4));
@@ -775,18 +782,15 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
     // binning: tiles per visible splat (sorted order) -> exclusive offsets -> pairs -> stable sort by tile
     hipLaunchKernelGGL(k_sp_tilecount, dim3(nblk(n, 256)), dim3(256), 0, st, s->sp, svals, s->visible, n, f.W, f.H,
                        s->cnt);
-    uint32_t last_cnt = 0;
-    S_TRY(hipMemcpyAsync(&last_cnt, s->cnt + (n - 1), 4, hipMemcpyDeviceToHost, st));
-    S_TRY(hipStreamSynchronize(st));
     uint32_t stiles = (n + 2047) / 2048;
     if (ensure_status(s, stiles, err) != 0) return -1;
-    hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->cnt, n, s->ticket, s->status, stiles, s->epoch);
+    hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->cnt, n, s->ticket, s->status, stiles, s->epoch,
+                       s->total);
     s->epoch = (s->epoch + 1u) & ((1u << 26) - 1u);
     if (!s->epoch) s->epoch = 1;
-    uint32_t last_off = 0;
-    S_TRY(hipMemcpyAsync(&last_off, s->cnt + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    uint64_t npairs = 0;   // one host read per frame: the pair buffers are sized from it
+    S_TRY(hipMemcpyAsync(&npairs, s->total, 8, hipMemcpyDeviceToHost, st));
     S_TRY(hipStreamSynchronize(st));
-    uint64_t npairs = (uint64_t)last_off + last_cnt;
     if (npairs > 0xFFFFFFFFull) {
         err = "sorted path: too many tile pairs";
         return -1;
