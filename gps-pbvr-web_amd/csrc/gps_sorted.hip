@@ -139,8 +139,7 @@ GPS_DEV float4 splat_colour(const GpsFrame &f, uint32_t id, float4 g0, float4 g3
 __global__ __launch_bounds__(256) void k_sp_project(GpsFrame f, const float4 *__restrict__ splats,
                                                     const float *__restrict__ sh, float4 *__restrict__ sp,
                                                     uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                    uint32_t *__restrict__ visible, uint64_t *__restrict__ stats,
-                                                    int probe_only) {
+                                                    uint4 *__restrict__ part, int probe_only) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t demand = 0, vis = 0;
     if (i < f.N) {
@@ -189,7 +188,10 @@ __global__ __launch_bounds__(256) void k_sp_project(GpsFrame f, const float4 *__
             }
         }
     }
-    // one atomic per wave for each counter
+    // the block's demand and visible count as one partial (k_sp_totals adds them up): an atomic per wave on the
+    // same two words would serialize thousands of waves in one L2 channel
+    __shared__ uint64_t s_dem[4];
+    __shared__ uint32_t s_vis[4];
     uint64_t d64 = demand;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -197,8 +199,49 @@ __global__ __launch_bounds__(256) void k_sp_project(GpsFrame f, const float4 *__
         vis += __shfl_xor(vis, off, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        if (d64) atomicAdd((unsigned long long *)&stats[ST_PROBE], (unsigned long long)d64);
-        if (vis && !probe_only) atomicAdd(visible, vis);
+        s_dem[threadIdx.x >> 6] = d64;
+        s_vis[threadIdx.x >> 6] = vis;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t d = s_dem[0] + s_dem[1] + s_dem[2] + s_dem[3];
+        part[blockIdx.x] = make_uint4((uint32_t)d, (uint32_t)(d >> 32), s_vis[0] + s_vis[1] + s_vis[2] + s_vis[3], 0u);
+    }
+}
+
+// the frame's probe demand (stats[ST_PROBE]) and visible splat count from k_sp_project's per-block partials
+constexpr int TOTALS_THREADS = 1024;
+__global__ __launch_bounds__(TOTALS_THREADS) void k_sp_totals(const uint4 *__restrict__ part, uint32_t nparts,
+                                                              uint64_t *__restrict__ stats,
+                                                              uint32_t *__restrict__ visible, int probe_only) {
+    __shared__ uint64_t s_dem[TOTALS_THREADS / 64];
+    __shared__ uint32_t s_vis[TOTALS_THREADS / 64];
+    uint64_t d = 0;
+    uint32_t v = 0;
+    for (uint32_t b = threadIdx.x; b < nparts; b += TOTALS_THREADS) {
+        const uint4 p = part[b];
+        d += ((uint64_t)p.y << 32) | p.x;
+        v += p.z;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        d += __shfl_xor(d, off, 64);
+        v += __shfl_xor(v, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_dem[threadIdx.x >> 6] = d;
+        s_vis[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        d = 0;
+        v = 0;
+        for (int w = 0; w < TOTALS_THREADS / 64; w++) {
+            d += s_dem[w];
+            v += s_vis[w];
+        }
+        stats[ST_PROBE] = d;
+        *visible = probe_only ? 0u : v;
     }
 }
 
@@ -602,11 +645,6 @@ __global__ void k_u32_fill(uint32_t *p, uint32_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_probe_reset(uint64_t *stats, uint32_t *visible) {
-    stats[ST_PROBE] = 0ull;
-    *visible = 0u;
-}
-
 unsigned nblk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -621,6 +659,7 @@ struct SortedState {
     uint64_t *status = nullptr;
     uint64_t status_cap = 0, hist_cap = 0;
     uint32_t *cnt = nullptr;          // per sorted rank: tiles touched, then exclusive offsets
+    uint4 *part = nullptr;            // per projection block: probe demand (lo, hi) and visible count
     uint32_t *pk[2] = {nullptr, nullptr}, *pv[2] = {nullptr, nullptr};
     uint64_t pair_cap = 0;
     uint32_t *tstart = nullptr, *tend = nullptr, *torder = nullptr;
@@ -652,6 +691,7 @@ void sorted_destroy(SortedState *s) {
     sfree(s->total);
     sfree(s->status);
     sfree(s->cnt);
+    sfree(s->part);
     sfree(s->tstart);
     sfree(s->tend);
     sfree(s->torder);
@@ -741,6 +781,7 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
             sfree(s->vals[k]);
         }
         sfree(s->cnt);
+        sfree(s->part);
         sfree(s->tstart);
         sfree(s->tend);
         sfree(s->torder);
@@ -750,6 +791,7 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
             S_TRY(hipMalloc(&s->vals[k], (uint64_t)n * 4));
         }
         S_TRY(hipMalloc(&s->cnt, (uint64_t)n * 4));
+        S_TRY(hipMalloc(&s->part, (uint64_t)nblk(n, 256) * sizeof(uint4)));
         if (!s->ticket) {
             S_TRY(hipMalloc(&s->ticket, 4));
             S_TRY(hipMemset(s->ticket, 0, 4));
@@ -764,11 +806,12 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
         s->W = f.W;
         s->H = f.H;
     }
-    hipLaunchKernelGGL(k_probe_reset, dim3(1), dim3(1), 0, st, stats, s->visible);
     {
         SProf p(prof, st, "sp_project");
         hipLaunchKernelGGL(k_sp_project, dim3(nblk(n, 256)), dim3(256), 0, st, f, splats, sh, s->sp, s->keys[0],
-                           s->vals[0], s->visible, stats, probe_only);
+                           s->vals[0], s->part, probe_only);
+        hipLaunchKernelGGL(k_sp_totals, dim3(1), dim3(TOTALS_THREADS), 0, st, s->part, nblk(n, 256), stats,
+                           s->visible, probe_only);
     }
     if (probe_only) {
         S_TRY(hipGetLastError());
