@@ -197,8 +197,8 @@ def bench_valu(a):
 def bench_alpha(a):
     """The sorted alpha path (render_sorted: project, 24-bit radix sort of the splat keys, tile binning and its
     sort, per-tile f32 blending) on the clustered 560k-Gaussian SH3 scene at 1280x720, at the three orbit
-    views: host wall time per frame (it includes the path's two host reads of the tile-pair count), the summed
-    HIP-event time of its profiled kernels, and the splat and tile-pair sorts' share."""
+    views: host wall time per frame (it includes the path's host read of the tile-pair count), then, in as many
+    frames again, the HIP-event time of its profiled stages."""
     import gps_hip as gh
     from gps_hip import scenes as S
     W, H = 1280, 720
@@ -213,12 +213,16 @@ def bench_alpha(a):
         for _ in range(max(a.warmup, 1)):
             r.render_sorted(cam, p)
         r.wait_idle()
-        r.profile(True)
         t0 = time.perf_counter()
         for _ in range(frames):
             r.render_sorted(cam, p)
         r.wait_idle()
         wall = (time.perf_counter() - t0) * 1000.0 / frames
+        # the per-stage breakdown in frames of their own: its events add gaps between the stages
+        r.profile(True)
+        for _ in range(frames):
+            r.render_sorted(cam, p)
+        r.wait_idle()
         prof = r.profile_read()
         r.profile(False)
         out[v] = {"wall_ms_per_frame": round(wall, 3),

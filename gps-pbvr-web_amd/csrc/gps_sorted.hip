@@ -424,25 +424,50 @@ __global__ __launch_bounds__(256) void k_sp_tilecount(const float4 *__restrict__
     cnt[j] = c;
 }
 
+// Pairs of splat j at off[j] .. off[j] + count - 1, row-major over its tile box.  A lane whose splat touches at most
+// EMIT_SOLO tiles writes them itself; larger splats (the near view's have hundreds) are written by their whole wave,
+// 64 pairs per step, so that one lane's long loop does not hold the kernel.
+constexpr uint32_t EMIT_SOLO = 16;
 __global__ __launch_bounds__(256) void k_sp_emit(const float4 *__restrict__ sp, const uint32_t *__restrict__ vals,
                                                  const uint32_t *__restrict__ visible, uint32_t n, uint32_t W,
                                                  uint32_t H, const uint32_t *__restrict__ off,
                                                  uint32_t *__restrict__ pkeys, uint32_t *__restrict__ pvals) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n || j >= *visible) return;
-    uint32_t tx0, ty0, tx1, ty1;
-    const uint32_t id = vals[j];
-    quad_tiles(sp, id, W, H, tx0, ty0, tx1, ty1);
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
+    const bool live = j < n && j < *visible;
+    uint32_t tx0 = 0, ty0 = 0, bw = 0, cnt = 0, o = 0, id = 0;
+    if (live) {
+        uint32_t tx1, ty1;
+        id = vals[j];
+        quad_tiles(sp, id, W, H, tx0, ty0, tx1, ty1);
+        bw = tx1 - tx0 + 1u;
+        cnt = bw * (ty1 - ty0 + 1u);
+        o = off[j];
+    }
     const uint32_t tw = (W + BIN - 1) / BIN;
-    uint32_t o = off[j];
-    // the pairs of splat j sit at offsets above those of splats 0..j-1, so the stable sort by tile keeps each
-    // tile's splats in sorted (back-to-front) order; the value is the splat itself, read directly by the blend
-    for (uint32_t ty = ty0; ty <= ty1; ty++)
-        for (uint32_t tx = tx0; tx <= tx1; tx++) {
-            pkeys[o] = ty * tw + tx;
-            pvals[o] = id;
-            o++;
+    // the stable sort by tile keeps each tile's splats in sorted (back-to-front) order: the pairs of splat j sit
+    // above those of splats 0..j-1; the value is the splat itself, read directly by the blend
+    if (cnt <= EMIT_SOLO) {
+        for (uint32_t t = 0, ty = ty0, tx = tx0; t < cnt; t++) {
+            pkeys[o + t] = ty * tw + tx;
+            pvals[o + t] = id;
+            if (++tx == tx0 + bw) {
+                tx = tx0;
+                ty++;
+            }
         }
+    }
+    uint64_t big = __ballot(cnt > EMIT_SOLO);
+    while (big) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(big);
+        big &= big - 1u;
+        const uint32_t bo = __shfl(o, b, 64), bcnt = __shfl(cnt, b, 64), bbw = __shfl(bw, b, 64);
+        const uint32_t btx0 = __shfl(tx0, b, 64), bty0 = __shfl(ty0, b, 64), bid = __shfl(id, b, 64);
+        for (uint32_t t = lane; t < bcnt; t += 64u) {
+            const uint32_t r = t / bbw;
+            pkeys[bo + t] = (bty0 + r) * tw + btx0 + (t - r * bbw);
+            pvals[bo + t] = bid;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict__ pkeys, uint32_t npairs,
