@@ -665,9 +665,13 @@ __global__ __launch_bounds__(ORDER_THREADS) void k_tile_order(const uint32_t *__
         order[atomicAdd(&s_cnt[__clz(end[t] - start[t])], 1u)] = t;
 }
 
-__global__ void k_u32_fill(uint32_t *p, uint32_t n, uint32_t v) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+// empty tile ranges (start = end = 0): k_tile_ranges then sets those of the tiles that have pairs
+__global__ void k_tile_clear(uint32_t *__restrict__ start, uint32_t *__restrict__ end, uint32_t tiles) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < tiles) {
+        start[i] = 0u;
+        end[i] = 0u;
+    }
 }
 
 unsigned nblk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -680,7 +684,9 @@ struct SortedState {
     float4 *sp = nullptr;
     uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
     uint32_t *hist = nullptr, *ticket = nullptr, *visible = nullptr;
-    uint64_t *total = nullptr;        // the tile-pair count (inclusive sum of the per-splat tile counts)
+    uint64_t *total_host = nullptr;   // the tile-pair count (inclusive sum of the per-splat tile counts), in pinned
+    uint64_t *total = nullptr;        // host memory that the scan writes through its device address
+    hipEvent_t scanned = nullptr;     // recorded after that scan; the host polls it
     uint64_t *status = nullptr;
     uint64_t status_cap = 0, hist_cap = 0;
     uint32_t *cnt = nullptr;          // per sorted rank: tiles touched, then exclusive offsets
@@ -713,7 +719,8 @@ void sorted_destroy(SortedState *s) {
     sfree(s->hist);
     sfree(s->ticket);
     sfree(s->visible);
-    sfree(s->total);
+    if (s->total_host) (void)hipHostFree(s->total_host);
+    if (s->scanned) (void)hipEventDestroy(s->scanned);
     sfree(s->status);
     sfree(s->cnt);
     sfree(s->part);
@@ -821,7 +828,9 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
             S_TRY(hipMalloc(&s->ticket, 4));
             S_TRY(hipMemset(s->ticket, 0, 4));
             S_TRY(hipMalloc(&s->visible, 4));
-            S_TRY(hipMalloc(&s->total, 8));
+            S_TRY(hipHostMalloc((void **)&s->total_host, 8, hipHostMallocMapped));
+            S_TRY(hipHostGetDevicePointer((void **)&s->total, s->total_host, 0));
+            S_TRY(hipEventCreateWithFlags(&s->scanned, hipEventDisableTiming));
         }
         s->tiles_cap = ((f.W + BIN - 1) / BIN) * ((f.H + BIN - 1) / BIN);
         S_TRY(hipMalloc(&s->tstart, (uint64_t)s->tiles_cap * 4));
@@ -856,9 +865,17 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
                        s->total);
     s->epoch = (s->epoch + 1u) & ((1u << 26) - 1u);
     if (!s->epoch) s->epoch = 1;
-    uint64_t npairs = 0;   // one host read per frame: the pair buffers are sized from it
-    S_TRY(hipMemcpyAsync(&npairs, s->total, 8, hipMemcpyDeviceToHost, st));
-    S_TRY(hipStreamSynchronize(st));
+    hipLaunchKernelGGL(k_tile_clear, dim3(nblk(s->tiles_cap, 256)), dim3(256), 0, st, s->tstart, s->tend,
+                       s->tiles_cap);
+    // one host read per frame, the pair buffers are sized from it: poll the event (a blocking wait adds the
+    // thread's wake-up to every frame) and read the count the scan wrote to pinned memory
+    S_TRY(hipEventRecord(s->scanned, st));
+    for (;;) {
+        const hipError_t q = hipEventQuery(s->scanned);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) S_TRY(q);
+    }
+    const uint64_t npairs = *(volatile uint64_t *)s->total_host;
     if (npairs > 0xFFFFFFFFull) {
         err = "sorted path: too many tile pairs";
         return -1;
@@ -872,8 +889,6 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
         }
         s->pair_cap = npairs;
     }
-    hipLaunchKernelGGL(k_u32_fill, dim3(nblk(s->tiles_cap, 256)), dim3(256), 0, st, s->tstart, s->tiles_cap, 0u);
-    hipLaunchKernelGGL(k_u32_fill, dim3(nblk(s->tiles_cap, 256)), dim3(256), 0, st, s->tend, s->tiles_cap, 0u);
     int pres = 0;   // which pair buffer holds the tile-sorted pairs (with none, nothing is read from it)
     if (npairs) {
         hipLaunchKernelGGL(k_sp_emit, dim3(nblk(n, 256)), dim3(256), 0, st, s->sp, svals, s->visible, n, f.W, f.H,
