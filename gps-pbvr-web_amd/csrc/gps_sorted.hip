@@ -424,49 +424,63 @@ __global__ __launch_bounds__(256) void k_sp_tilecount(const float4 *__restrict__
     cnt[j] = c;
 }
 
-// Pairs of splat j at off[j] .. off[j] + count - 1, row-major over its tile box.  A lane whose splat touches at most
-// EMIT_SOLO tiles writes them itself; larger splats (the near view's have hundreds) are written by their whole wave,
-// 64 pairs per step, so that one lane's long loop does not hold the kernel.
-constexpr uint32_t EMIT_SOLO = 16;
-__global__ __launch_bounds__(256) void k_sp_emit(const float4 *__restrict__ sp, const uint32_t *__restrict__ vals,
-                                                 const uint32_t *__restrict__ visible, uint32_t n, uint32_t W,
-                                                 uint32_t H, const uint32_t *__restrict__ off,
-                                                 uint32_t *__restrict__ pkeys, uint32_t *__restrict__ pvals) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63u;
-    const bool live = j < n && j < *visible;
-    uint32_t tx0 = 0, ty0 = 0, bw = 0, cnt = 0, o = 0, id = 0;
-    if (live) {
-        uint32_t tx1, ty1;
-        id = vals[j];
+// Tile pairs, balanced by pairs rather than by splats: the splats touch from one to thousands of tiles (at the
+// near view a few dozen large splats hold a fifth of the pairs, next to each other at the end of the depth order).
+// The pairs are cut into blocks of EMIT_PAIRS; k_pair_owner records, for each block, the splat owning its first
+// pair; k_sp_emit stages the offsets and tile boxes of the block's splats in LDS and writes one pair per item,
+// finding the item's splat by binary search there.  Splat j's pairs sit at off[j] .. off[j + 1] - 1, row-major
+// over its tile box; the stable sort by tile then keeps each tile's splats in sorted (back-to-front) order.  The
+// value is the splat itself, read directly by the blend.
+constexpr uint32_t EMIT_PAIRS = 1024;
+constexpr int EMIT_THREADS = 256;
+
+__global__ __launch_bounds__(256) void k_pair_owner(const uint32_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ visible, uint32_t n,
+                                                    uint32_t npairs, uint32_t *__restrict__ owner) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t vis = *visible;
+    if (j >= n || j >= vis) return;
+    const uint32_t s = off[j], e = j + 1u < vis ? off[j + 1u] : npairs;   // every visible splat has a pair
+    for (uint64_t blk = ((uint64_t)s + EMIT_PAIRS - 1u) / EMIT_PAIRS; blk * EMIT_PAIRS < e; blk++)
+        owner[blk] = j;
+}
+
+__global__ __launch_bounds__(EMIT_THREADS) void k_sp_emit(const float4 *__restrict__ sp,
+                                                          const uint32_t *__restrict__ vals,
+                                                          const uint32_t *__restrict__ visible, uint32_t W,
+                                                          uint32_t H, const uint32_t *__restrict__ off,
+                                                          const uint32_t *__restrict__ owner, uint32_t npairs,
+                                                          uint32_t *__restrict__ pkeys, uint32_t *__restrict__ pvals) {
+    __shared__ uint32_t s_off[EMIT_PAIRS + 1u], s_tx0[EMIT_PAIRS + 1u], s_ty0[EMIT_PAIRS + 1u],
+        s_bw[EMIT_PAIRS + 1u], s_id[EMIT_PAIRS + 1u];
+    const uint32_t nblocks = (uint32_t)(((uint64_t)npairs + EMIT_PAIRS - 1u) / EMIT_PAIRS);
+    const uint32_t p0 = blockIdx.x * EMIT_PAIRS, p1 = min(npairs - p0, EMIT_PAIRS) + p0;
+    // the block's splats: the owner of its first pair through the owner of the next block's first pair (or the
+    // last visible splat); at most EMIT_PAIRS + 1, since each splat has at least one pair
+    const uint32_t j0 = owner[blockIdx.x], j1 = blockIdx.x + 1u < nblocks ? owner[blockIdx.x + 1u] : *visible - 1u;
+    const uint32_t ns = j1 - j0 + 1u;
+    for (uint32_t k = threadIdx.x; k < ns; k += EMIT_THREADS) {
+        const uint32_t id = vals[j0 + k];
+        uint32_t tx0, ty0, tx1, ty1;
         quad_tiles(sp, id, W, H, tx0, ty0, tx1, ty1);
-        bw = tx1 - tx0 + 1u;
-        cnt = bw * (ty1 - ty0 + 1u);
-        o = off[j];
+        s_off[k] = off[j0 + k];
+        s_tx0[k] = tx0;
+        s_ty0[k] = ty0;
+        s_bw[k] = tx1 - tx0 + 1u;
+        s_id[k] = id;
     }
+    __syncthreads();
     const uint32_t tw = (W + BIN - 1) / BIN;
-    // the stable sort by tile keeps each tile's splats in sorted (back-to-front) order: the pairs of splat j sit
-    // above those of splats 0..j-1; the value is the splat itself, read directly by the blend
-    if (cnt <= EMIT_SOLO) {
-        for (uint32_t t = 0, ty = ty0, tx = tx0; t < cnt; t++) {
-            pkeys[o + t] = ty * tw + tx;
-            pvals[o + t] = id;
-            if (++tx == tx0 + bw) {
-                tx = tx0;
-                ty++;
-            }
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += EMIT_THREADS) {
+        uint32_t lo = 0, hi = ns - 1u;   // the last k with s_off[k] <= p (s_off[0] <= p0)
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (s_off[mid] <= p) lo = mid;
+            else hi = mid - 1u;
         }
-    }
-    uint64_t big = __ballot(cnt > EMIT_SOLO);
-    while (big) {
-        const uint32_t b = (uint32_t)__builtin_ctzll(big);
-        big &= big - 1u;
-        const uint32_t bo = __shfl(o, b, 64), bcnt = __shfl(cnt, b, 64), bbw = __shfl(bw, b, 64);
-        const uint32_t btx0 = __shfl(tx0, b, 64), bty0 = __shfl(ty0, b, 64), bid = __shfl(id, b, 64);
-        for (uint32_t t = lane; t < bcnt; t += 64u) {
-            const uint32_t r = t / bbw;
-            pkeys[bo + t] = (bty0 + r) * tw + btx0 + (t - r * bbw);
-            pvals[bo + t] = bid;
-        }
+        const uint32_t t = p - s_off[lo], bw = s_bw[lo], r = t / bw;
+        pkeys[p] = (s_ty0[lo] + r) * tw + s_tx0[lo] + (t - r * bw);
+        pvals[p] = s_id[lo];
     }
 }
 
@@ -693,6 +707,7 @@ struct SortedState {
     uint4 *part = nullptr;            // per projection block: probe demand (lo, hi) and visible count
     uint32_t *pk[2] = {nullptr, nullptr}, *pv[2] = {nullptr, nullptr};
     uint64_t pair_cap = 0;
+    uint32_t *owner = nullptr;        // per block of EMIT_PAIRS tile pairs: the splat owning its first pair
     uint32_t *tstart = nullptr, *tend = nullptr, *torder = nullptr;
     uint32_t tiles_cap = 0;
     uint32_t epoch = 1;
@@ -716,6 +731,7 @@ void sorted_destroy(SortedState *s) {
         sfree(s->pk[k]);
         sfree(s->pv[k]);
     }
+    sfree(s->owner);
     sfree(s->hist);
     sfree(s->ticket);
     sfree(s->visible);
@@ -887,12 +903,17 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
             S_TRY(hipMalloc(&s->pk[k], npairs * 4));
             S_TRY(hipMalloc(&s->pv[k], npairs * 4));
         }
+        sfree(s->owner);
+        S_TRY(hipMalloc(&s->owner, ((npairs + EMIT_PAIRS - 1) / EMIT_PAIRS) * 4));
         s->pair_cap = npairs;
     }
     int pres = 0;   // which pair buffer holds the tile-sorted pairs (with none, nothing is read from it)
     if (npairs) {
-        hipLaunchKernelGGL(k_sp_emit, dim3(nblk(n, 256)), dim3(256), 0, st, s->sp, svals, s->visible, n, f.W, f.H,
-                           s->cnt, s->pk[0], s->pv[0]);
+        const uint32_t eblocks = nblk(npairs, EMIT_PAIRS);
+        hipLaunchKernelGGL(k_pair_owner, dim3(nblk(n, 256)), dim3(256), 0, st, s->cnt, s->visible, n,
+                           (uint32_t)npairs, s->owner);
+        hipLaunchKernelGGL(k_sp_emit, dim3(eblocks), dim3(EMIT_THREADS), 0, st, s->sp, svals, s->visible, f.W, f.H,
+                           s->cnt, s->owner, (uint32_t)npairs, s->pk[0], s->pv[0]);
         uint32_t bits = 0;
         while ((1u << bits) < s->tiles_cap) bits++;
         bits = (bits + 7) / 8 * 8;
