@@ -497,8 +497,9 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict_
 // One workgroup per 8x8 quadrant of a 16x16 tile (four per tile, so a tile's list is spread over four CUs), four
 // waves per quadrant.  The list is walked in batches of BLEND_BATCH entries.  The staging wave loads a batch's
 // records while the previous batch is evaluated and keeps, in list order, only the splats whose binning box
-// (quad_tiles' box, the one that decides which tiles list a splat) overlaps the quadrant: outside that box a
-// splat's alpha is zero, as the tile lists already assume.  Wave w then evaluates the alpha of kept splats w,
+// (quad_tiles' box, the one that decides which tiles list a splat) overlaps the quadrant, outside which a splat's
+// alpha is zero as the tile lists already assume, and whose ellipse, bounded in its own axes, reaches a pixel
+// centre of the quadrant (axes_touch).  Wave w then evaluates the alpha of kept splats w,
 // w + 4, ... at its 64 pixels (the divisions and the exp, the costly part, independent across splats) into LDS,
 // with one bit per splat saying whether any of its alphas is non-zero; wave 0 runs the back-to-front recurrence
 // over the flagged splats in list order, reading them four at a time.  The arithmetic and its order per pixel
@@ -534,12 +535,34 @@ GPS_DEV bool box_touches(float4 a, float4 b, float qx0, float qy0) {
            !(floorf(a.y - ey) > qy0 + (float)(BLEND_QUAD - 1)) && !(ceilf(a.y + ey) < qy0);
 }
 
+// The ellipse r2 <= k^2 in the splat's own axes against the quadrant's pixel centres: false only if at every
+// centre |d.b1| > k |b1|^2 or |d.b2| > k |b2|^2 by a margin, so that the per-pixel |u| or |v| computed in the blend
+// exceeds k and its r2 exceeds k^2 (alpha 0).  The margin, 2^-16 of k |b|^2 plus 2^-16 |b| times the distances
+// involved, is far above the rounding of the per-pixel dx, dy, u, v (a few ulp, about 2^-22 relative) and of these
+// bounds themselves.  It only rejects when the centre and both denominators are finite and well inside the normal
+// range (no infinite or NaN u, v at any pixel); otherwise the splat is kept and evaluated as everywhere else.
+GPS_DEV bool axes_touch(float4 a, float4 b, float qx0, float qy0) {
+    const float cx0 = qx0 + 0.5f - a.x, cx1 = qx0 + (float)BLEND_QUAD - 0.5f - a.x;
+    const float cy0 = qy0 + 0.5f - a.y, cy1 = qy0 + (float)BLEND_QUAD - 0.5f - a.y;
+    const float den1 = b.x * b.x + b.y * b.y, den2 = b.z * b.z + b.w * b.w;
+    const bool ok = den1 > 0x1p-100f && den1 < 0x1p100f && den2 > 0x1p-100f && den2 < 0x1p100f &&
+                    fabsf(a.x) < 0x1p40f && fabsf(a.y) < 0x1p40f && a.w >= 0.0f && a.w < 0x1p20f;
+    const float s1 = gpsm::sqrt(den1), s2 = gpsm::sqrt(den2);
+    const float m = 0x1p-16f * (a.w * fmax(s1, s2) + fabsf(cx0) + fabsf(cy0) + 2.0f * (float)BLEND_QUAD + 16.0f);
+    const float t1 = a.w * den1 * (1.0f + 0x1p-16f) + m * s1, t2 = a.w * den2 * (1.0f + 0x1p-16f) + m * s2;
+    const float lo1 = fmin(cx0 * b.x, cx1 * b.x) + fmin(cy0 * b.y, cy1 * b.y);
+    const float hi1 = fmax(cx0 * b.x, cx1 * b.x) + fmax(cy0 * b.y, cy1 * b.y);
+    const float lo2 = fmin(cx0 * b.z, cx1 * b.z) + fmin(cy0 * b.w, cy1 * b.w);
+    const float hi2 = fmax(cx0 * b.z, cx1 * b.z) + fmax(cy0 * b.w, cy1 * b.w);
+    return !(ok && (lo1 > t1 || hi1 < -t1 || lo2 > t2 || hi2 < -t2));
+}
+
 // staging wave: keep the lanes whose splat touches the quadrant, packed in lane (list) order, with the splat's
 // per-pixel constants (the two axis denominators and k^2, the same operations as in the per-pixel expressions);
 // returns the count
 GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0, float qy0, float4 *s_a,
                              float4 *s_b, float4 *s_c, float4 *s_d, uint32_t lane) {
-    const bool keep = valid && box_touches(a, b, qx0, qy0);
+    const bool keep = valid && box_touches(a, b, qx0, qy0) && axes_touch(a, b, qx0, qy0);
     const uint64_t mask = __ballot(keep);
     if (keep) {
         const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
