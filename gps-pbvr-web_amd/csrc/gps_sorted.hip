@@ -498,8 +498,8 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict_
 // waves per quadrant.  The list is walked in batches of BLEND_BATCH entries.  The staging wave loads a batch's
 // records while the previous batch is evaluated and keeps, in list order, only the splats whose binning box
 // (quad_tiles' box, the one that decides which tiles list a splat) overlaps the quadrant, outside which a splat's
-// alpha is zero as the tile lists already assume, and whose ellipse, bounded in its own axes, reaches a pixel
-// centre of the quadrant (axes_touch).  Wave w then evaluates the alpha of kept splats w,
+// alpha is zero as the tile lists already assume, and whose ellipse may reach a pixel centre of the quadrant
+// (ellipse_touch).  Wave w then evaluates the alpha of kept splats w,
 // w + 4, ... at its 64 pixels (the divisions and the exp, the costly part, independent across splats) into LDS,
 // with one bit per splat saying whether any of its alphas is non-zero; wave 0 runs the back-to-front recurrence
 // over the flagged splats in list order, reading them four at a time.  The arithmetic and its order per pixel
@@ -535,26 +535,47 @@ GPS_DEV bool box_touches(float4 a, float4 b, float qx0, float qy0) {
            !(floorf(a.y - ey) > qy0 + (float)(BLEND_QUAD - 1)) && !(ceilf(a.y + ey) < qy0);
 }
 
-// The ellipse r2 <= k^2 in the splat's own axes against the quadrant's pixel centres: false only if at every
-// centre |d.b1| > k |b1|^2 or |d.b2| > k |b2|^2 by a margin, so that the per-pixel |u| or |v| computed in the blend
-// exceeds k and its r2 exceeds k^2 (alpha 0).  The margin, 2^-16 of k |b|^2 plus 2^-16 |b| times the distances
-// involved, is far above the rounding of the per-pixel dx, dy, u, v (a few ulp, about 2^-22 relative) and of these
-// bounds themselves.  It only rejects when the centre and both denominators are finite and well inside the normal
-// range (no infinite or NaN u, v at any pixel); otherwise the splat is kept and evaluated as everywhere else.
-GPS_DEV bool axes_touch(float4 a, float4 b, float qx0, float qy0) {
-    const float cx0 = qx0 + 0.5f - a.x, cx1 = qx0 + (float)BLEND_QUAD - 0.5f - a.x;
-    const float cy0 = qy0 + 0.5f - a.y, cy1 = qy0 + (float)BLEND_QUAD - 0.5f - a.y;
+// The ellipse r2 <= k^2 against the quadrant's pixel centres: false only if at every centre the r2 the blend
+// computes exceeds k^2 (alpha 0).  With d a centre's offset from the splat centre and, in real arithmetic,
+// U = d.b1 / |b1|^2, V = d.b2 / |b2|^2, U^2 + V^2 = d^T M d is a positive semi-definite form.  Its minimum over the
+// rectangle of pixel centres is bounded from below: 0 if the splat centre lies inside; otherwise the least over
+// the four edges of the edge's minimum (at an end where the minimiser lies clearly outside the edge, else the
+// unconstrained minimum det c^2 / M_tt, never larger), less 2^-15 T, T = (M_xx + M_yy + 2 |M_xy|) (|x| + |y|)^2
+// over the rectangle: the float rounding of M (hardware reciprocals, 1 ulp) and of these sums, cancellation
+// included, stays below 2^-17 T.  The blend's float u, v differ from U, V by at most
+// 2^-21 (|dx| + |dy|) (1 / |b1| + 1 / |b2|) + 2^-21 (|U| + |V|); the test rejects only when the bound exceeds
+// (k (1 + 2^-14) + that distance term)^2 / (1 - 2^-14), and only when the centre and both denominators lie well
+// inside the float range, so that no u, v is infinite or NaN at any pixel and no term of the bound overflows.
+GPS_DEV float edge_min(float c, float lo, float hi, float m_cc, float m_tt, float m_ct, float r_tt, float det) {
+    const float t = -m_ct * c * r_tt, tol = 0x1p-12f * (fabsf(lo) + fabsf(hi) + 1.0f);
+    if (!(t < lo - tol) && !(t > hi + tol)) return (det * r_tt) * (c * c);   // the unconstrained minimum
+    const float e = t < lo ? lo : hi;
+    return m_tt * e * e + 2.0f * m_ct * e * c + m_cc * c * c;
+}
+
+GPS_DEV bool ellipse_touch(float4 a, float4 b, float qx0, float qy0) {
     const float den1 = b.x * b.x + b.y * b.y, den2 = b.z * b.z + b.w * b.w;
-    const bool ok = den1 > 0x1p-100f && den1 < 0x1p100f && den2 > 0x1p-100f && den2 < 0x1p100f &&
-                    fabsf(a.x) < 0x1p40f && fabsf(a.y) < 0x1p40f && a.w >= 0.0f && a.w < 0x1p20f;
-    const float s1 = gpsm::sqrt(den1), s2 = gpsm::sqrt(den2);
-    const float m = 0x1p-16f * (a.w * fmax(s1, s2) + fabsf(cx0) + fabsf(cy0) + 2.0f * (float)BLEND_QUAD + 16.0f);
-    const float t1 = a.w * den1 * (1.0f + 0x1p-16f) + m * s1, t2 = a.w * den2 * (1.0f + 0x1p-16f) + m * s2;
-    const float lo1 = fmin(cx0 * b.x, cx1 * b.x) + fmin(cy0 * b.y, cy1 * b.y);
-    const float hi1 = fmax(cx0 * b.x, cx1 * b.x) + fmax(cy0 * b.y, cy1 * b.y);
-    const float lo2 = fmin(cx0 * b.z, cx1 * b.z) + fmin(cy0 * b.w, cy1 * b.w);
-    const float hi2 = fmax(cx0 * b.z, cx1 * b.z) + fmax(cy0 * b.w, cy1 * b.w);
-    return !(ok && (lo1 > t1 || hi1 < -t1 || lo2 > t2 || hi2 < -t2));
+    // in these ranges every product below stays under 2^113: no overflow, no NaN
+    const bool ok = den1 > 0x1p-20f && den1 < 0x1p20f && den2 > 0x1p-20f && den2 < 0x1p20f &&
+                    fabsf(a.x) < 0x1p24f && fabsf(a.y) < 0x1p24f && a.w >= 0.0f && a.w < 0x1p10f;
+    if (!ok) return true;
+    const float x0 = qx0 + 0.5f - a.x, x1 = qx0 + (float)BLEND_QUAD - 0.5f - a.x;
+    const float y0 = qy0 + 0.5f - a.y, y1 = qy0 + (float)BLEND_QUAD - 0.5f - a.y;
+    if (x0 <= 0.0f && x1 >= 0.0f && y0 <= 0.0f && y1 >= 0.0f) return true;
+    const float i1 = __builtin_amdgcn_rcpf(den1 * den1), i2 = __builtin_amdgcn_rcpf(den2 * den2);
+    const float mxx = b.x * b.x * i1 + b.z * b.z * i2, mxy = b.x * b.y * i1 + b.z * b.w * i2,
+                myy = b.y * b.y * i1 + b.w * b.w * i2;
+    if (!(mxx > 0.0f) || !(myy > 0.0f)) return true;
+    const float rxx = __builtin_amdgcn_rcpf(mxx), ryy = __builtin_amdgcn_rcpf(myy);
+    const float det = fmax(mxx * myy - mxy * mxy, 0.0f);
+    const float f = fmin(fmin(edge_min(y0, x0, x1, myy, mxx, mxy, rxx, det), edge_min(y1, x0, x1, myy, mxx, mxy, rxx, det)),
+                         fmin(edge_min(x0, y0, y1, mxx, myy, mxy, ryy, det), edge_min(x1, y0, y1, mxx, myy, mxy, ryy, det)));
+    const float xm = fmax(fabsf(x0), fabsf(x1)), ym = fmax(fabsf(y0), fabsf(y1));
+    const float f_lo = f - 0x1p-15f * (mxx + myy + 2.0f * fabsf(mxy)) * ((xm + ym) * (xm + ym));
+    // 1 / |b| from the hardware reciprocal square root, inflated well past its error
+    const float dist = 0x1p-21f * 1.01f * (xm + ym) * (__builtin_amdgcn_rsqf(den1) + __builtin_amdgcn_rsqf(den2));
+    const float bound = a.w * (1.0f + 0x1p-14f) + dist;
+    return !(f_lo * (1.0f - 0x1p-14f) > bound * bound);
 }
 
 // staging wave: keep the lanes whose splat touches the quadrant, packed in lane (list) order, with the splat's
@@ -562,7 +583,7 @@ GPS_DEV bool axes_touch(float4 a, float4 b, float qx0, float qy0) {
 // returns the count
 GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0, float qy0, float4 *s_a,
                              float4 *s_b, float4 *s_c, float4 *s_d, uint32_t lane) {
-    const bool keep = valid && box_touches(a, b, qx0, qy0) && axes_touch(a, b, qx0, qy0);
+    const bool keep = valid && box_touches(a, b, qx0, qy0) && ellipse_touch(a, b, qx0, qy0);
     const uint64_t mask = __ballot(keep);
     if (keep) {
         const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
