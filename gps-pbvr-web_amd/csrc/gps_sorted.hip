@@ -505,19 +505,25 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t *__restrict_
 // over the flagged splats in list order, reading them four at a time.  The arithmetic and its order per pixel
 // are those of a one-thread-per-pixel walk of the whole list, so the image has the same bits.
 constexpr int BLEND_QUAD = 8;                                   // quadrant side in pixels
-constexpr int BLEND_WAVES = 4;
+#ifndef GPS_BLEND_WAVES
+#define GPS_BLEND_WAVES 4
+#endif
+constexpr int BLEND_WAVES = GPS_BLEND_WAVES;
 constexpr int BLEND_BATCH = 64;
 constexpr int BLEND_THREADS = 64 * BLEND_WAVES;
 constexpr uint32_t BLEND_STAGE = BLEND_WAVES - 1;               // the wave that loads the next batch
 static_assert(BLEND_QUAD * BLEND_QUAD == 64 && BIN == 2 * BLEND_QUAD, "one wave per quadrant, four per tile");
 static_assert(BLEND_BATCH == 64, "a batch is staged by one wave, one flag bit per splat");
 
+// one step of the recurrence; alpha 0 (the splat left out) keeps the colour as it is.  Computed and selected
+// rather than branched on, so that consecutive steps' independent products overlap.
 GPS_DEV void blend_over(float &cr, float &cg, float &cb, float alpha, float4 c) {
-    if (alpha > 0.0f) {
-        cr = c.x * alpha + cr * (1.0f - alpha);
-        cg = c.y * alpha + cg * (1.0f - alpha);
-        cb = c.z * alpha + cb * (1.0f - alpha);
-    }
+    const float na = 1.0f - alpha;
+    const float nr = c.x * alpha + cr * na, ng = c.y * alpha + cg * na, nb = c.z * alpha + cb * na;
+    const bool take = alpha > 0.0f;
+    cr = take ? nr : cr;
+    cg = take ? ng : cg;
+    cb = take ? nb : cb;
 }
 
 GPS_DEV uint32_t take_bit(uint64_t &m, bool &has) {
@@ -595,6 +601,29 @@ GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0
     return (uint32_t)__popcll(mask);
 }
 
+// a kept splat's r2 = u^2 + v^2 at pixel centre (qx, qy) (the two correctly rounded divisions by the staged
+// denominators)
+GPS_DEV float splat_r2(float4 a, float4 b, float4 d, float qx, float qy) {
+    const float dx = qx - a.x, dy = qy - a.y;
+    const float u = (dx * b.x + dy * b.y) / d.x;
+    const float v = (dx * b.z + dy * b.w) / d.y;
+    return u * u + v * v;
+}
+
+// its alpha from r2 (o the opacity, kk = k^2): 0 where it leaves the pixel unchanged (outside the image, r2 > k^2,
+// or below 1/255).  k_sp_project writes k = sqrt(2 ln(o / SP_MIN_ALPHA)) with o <= 1, so here r2 <= k^2 <= 11.1 or
+// r2 is NaN: exp's argument is in [-5.6, 0], where exp_mid gives exp's bits; a NaN r2 gives min(NaN, 0.99) = 0.99,
+// as exp(NaN) does.
+GPS_DEV float splat_alpha(float r2, float o, float kk, bool inside) {
+    float alpha = 0.0f;
+    if (inside && !(r2 > kk)) {
+        const bool num = r2 == r2;
+        const float al = num ? fmin(o * gpsm::exp_mid(num ? -0.5f * r2 : 0.0f), 0.99f) : 0.99f;
+        if (!(al < SP_MIN_ALPHA)) alpha = al;
+    }
+    return alpha;
+}
+
 __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const float4 *__restrict__ sp,
                                                             const uint32_t *__restrict__ ids,
                                                             const uint32_t *__restrict__ start,
@@ -646,30 +675,26 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
             if (lane + 2u * BLEND_BATCH < left) nnid = ids[base + 2u * BLEND_BATCH + lane];
         }
         uint64_t any = 0;
-        for (uint32_t q = swave; q < cnt; q += BLEND_WAVES) {
-            float alpha = 0.0f;
-            if (inside) {
-                const float4 a = s_a[cur][q], b = s_b[cur][q], d = s_d[cur][q];
-                const float dx = qx - a.x, dy = qy - a.y;
-                const float u = (dx * b.x + dy * b.y) / d.x;
-                const float v = (dx * b.z + dy * b.w) / d.y;
-                const float r2 = u * u + v * v;
-                if (!(r2 > d.z)) {
-                    // k_sp_project writes k = sqrt(2 ln(o / SP_MIN_ALPHA)) with o <= 1, so here r2 <= k^2 <= 11.1
-                    // or r2 is NaN: exp's argument is in [-5.6, 0], where exp_mid gives exp's bits; a NaN r2
-                    // gives min(NaN, 0.99) = 0.99, as exp(NaN) does
-                    const bool num = r2 == r2;
-                    const float al = num ? fmin(a.z * gpsm::exp_mid(num ? -0.5f * r2 : 0.0f), 0.99f) : 0.99f;
-                    if (!(al < SP_MIN_ALPHA)) alpha = al;
-                }
+        // two splats per iteration: their divisions interleave; the exp is skipped where no lane needs it
+        for (uint32_t q = swave; q < cnt; q += 2u * BLEND_WAVES) {
+            const uint32_t q2 = q + BLEND_WAVES < cnt ? q + BLEND_WAVES : q;
+            const float4 a0 = s_a[cur][q], d0 = s_d[cur][q], a1 = s_a[cur][q2], d1 = s_d[cur][q2];
+            const float r20 = splat_r2(a0, s_b[cur][q], d0, qx, qy), r21 = splat_r2(a1, s_b[cur][q2], d1, qx, qy);
+            const float al0 = splat_alpha(r20, a0.z, d0.z, inside);
+            s_alpha[q][lane] = al0;
+            if (__ballot(al0 > 0.0f)) any |= 1ull << q;
+            if (q2 != q) {
+                const float al1 = splat_alpha(r21, a1.z, d1.z, inside);
+                s_alpha[q2][lane] = al1;
+                if (__ballot(al1 > 0.0f)) any |= 1ull << q2;
             }
-            s_alpha[q][lane] = alpha;
-            if (__ballot(alpha > 0.0f)) any |= 1ull << q;
         }
         if (lane == 0u) s_any[wave] = any;
         __syncthreads();
         if (wave == 0u) {
-            const uint64_t mv = s_any[0] | s_any[1] | s_any[2] | s_any[3];
+            uint64_t mv = 0;
+#pragma unroll
+            for (int w = 0; w < BLEND_WAVES; w++) mv |= s_any[w];
             // readfirstlane returns int: widen through uint32_t, or bit 31 would fill the upper half
             const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(mv >> 32)),
                            lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)mv);
@@ -681,10 +706,11 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
                 const float a0 = s_alpha[q0][lane], a1 = s_alpha[q1][lane], a2 = s_alpha[q2][lane],
                             a3 = s_alpha[q3][lane];
                 const float4 c0 = s_c[cur][q0], c1 = s_c[cur][q1], c2 = s_c[cur][q2], c3 = s_c[cur][q3];
+                // a slot past the last flagged splat (h false) reads splat 0's record and takes alpha 0
                 blend_over(cr, cg, cb, a0, c0);
-                if (h1) blend_over(cr, cg, cb, a1, c1);
-                if (h2) blend_over(cr, cg, cb, a2, c2);
-                if (h3) blend_over(cr, cg, cb, a3, c3);
+                blend_over(cr, cg, cb, h1 ? a1 : 0.0f, c1);
+                blend_over(cr, cg, cb, h2 ? a2 : 0.0f, c2);
+                blend_over(cr, cg, cb, h3 ? a3 : 0.0f, c3);
             }
         }
         if (wave == BLEND_STAGE && left > (uint32_t)BLEND_BATCH) {
