@@ -512,6 +512,10 @@ constexpr int BLEND_WAVES = GPS_BLEND_WAVES;
 constexpr int BLEND_BATCH = 64;
 constexpr int BLEND_THREADS = 64 * BLEND_WAVES;
 constexpr uint32_t BLEND_STAGE = BLEND_WAVES - 1;               // the wave that loads the next batch
+#ifndef GPS_BLEND_RGROUP
+#define GPS_BLEND_RGROUP 4
+#endif
+constexpr int RGROUP = GPS_BLEND_RGROUP;                        // flagged splats per LDS round trip of the recurrence
 static_assert(BLEND_QUAD * BLEND_QUAD == 64 && BIN == 2 * BLEND_QUAD, "one wave per quadrant, four per tile");
 static_assert(BLEND_BATCH == 64, "a batch is staged by one wave, one flag bit per splat");
 
@@ -700,17 +704,21 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
                            lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)mv);
             uint64_t m = ((uint64_t)hi << 32) | lo;
             while (m) {
-                bool h0, h1, h2, h3;
-                const uint32_t q0 = take_bit(m, h0), q1 = take_bit(m, h1), q2 = take_bit(m, h2),
-                               q3 = take_bit(m, h3);
-                const float a0 = s_alpha[q0][lane], a1 = s_alpha[q1][lane], a2 = s_alpha[q2][lane],
-                            a3 = s_alpha[q3][lane];
-                const float4 c0 = s_c[cur][q0], c1 = s_c[cur][q1], c2 = s_c[cur][q2], c3 = s_c[cur][q3];
-                // a slot past the last flagged splat (h false) reads splat 0's record and takes alpha 0
-                blend_over(cr, cg, cb, a0, c0);
-                blend_over(cr, cg, cb, h1 ? a1 : 0.0f, c1);
-                blend_over(cr, cg, cb, h2 ? a2 : 0.0f, c2);
-                blend_over(cr, cg, cb, h3 ? a3 : 0.0f, c3);
+                // RGROUP flagged splats per LDS round trip; a slot past the last flagged splat (h false) reads
+                // splat 0's record and takes alpha 0
+                uint32_t q[RGROUP];
+                bool h[RGROUP];
+#pragma unroll
+                for (int g = 0; g < RGROUP; g++) q[g] = take_bit(m, h[g]);
+                float al[RGROUP];
+                float4 c[RGROUP];
+#pragma unroll
+                for (int g = 0; g < RGROUP; g++) {
+                    al[g] = s_alpha[q[g]][lane];
+                    c[g] = s_c[cur][q[g]];
+                }
+#pragma unroll
+                for (int g = 0; g < RGROUP; g++) blend_over(cr, cg, cb, h[g] ? al[g] : 0.0f, c[g]);
             }
         }
         if (wave == BLEND_STAGE && left > (uint32_t)BLEND_BATCH) {
