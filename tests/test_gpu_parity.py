@@ -420,6 +420,34 @@ def test_sorted_clustered_sh3_views_match_oracle(view):
     assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
 
 
+def test_sorted_empty_frames_and_buffer_reuse():
+    """One renderer through frames of the sorted path on scenes of different sizes, some seen from behind (no splat
+    visible, no tile pairs): every frame's keys, values and accumulation buffer equal the oracle's, whatever the
+    buffers the previous frames sized, and a frame with nothing visible is the background."""
+    p = gh.RenderParams(spp_side=1, background=BG)
+    back = gh.default_camera(200.0)
+    back.view_rot[:] = [-1, 0, 0, 0, 1, 0, 0, 0, -1]   # turned half a turn about y: every splat behind
+    big, small, other = crowd(), crowd(1200, seed=7), phase_scenes()[1][1]
+    r = gh.GpsRenderer(W, H, 1)
+    for i, (scene, cam) in enumerate([(big, CAM), (big, back), (other, CAM), (small, back), (small, CAM),
+                                      (big, CAM)]):
+        say(f"[sorted reuse] frame {i}")
+        arr = to_gpu(scene)
+        r.set_gaussians(arr)
+        r.render_sorted(cam, p, 1)
+        acc = r.read_accum()
+        keys, vals = r.read_sorted()
+        m = O.mirror_sorted(arr, None, 0, ocam(cam), mparams(p), W, H)
+        assert np.array_equal(keys, m["keys"]) and np.array_equal(vals, m["vals"])
+        assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+        if cam is back:
+            assert m["visible"] == 0
+            assert (acc[..., :3] == np.asarray(BG, np.float32)).all()
+        else:
+            assert m["visible"] > 0
+    r.close()
+
+
 def test_radix_sort_300k_multilevel():
     """verify.rs:827-853: 300k small splats."""
     say("[sorted] 300k")
