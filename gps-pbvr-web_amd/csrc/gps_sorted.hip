@@ -246,32 +246,38 @@ __global__ __launch_bounds__(TOTALS_THREADS) void k_sp_totals(const uint4 *__res
 }
 
 // ------------------------------------------------------------------ radix sort
+// RB-bit digits (4 to 8): a pass sorts by (key >> shift) & (2^RB - 1)
+template <int RB>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
                                                       uint32_t ntiles, uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    constexpr uint32_t NB = 1u << RB;
+    static_assert(RB >= 4 && RB <= 8 && (int)NB <= RS_BLOCK, "digit width");
+    __shared__ uint32_t h[NB];
+    if (threadIdx.x < NB) h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t tile = blockIdx.x;
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
         uint32_t i = tile * RS_TILE + j * RS_BLOCK + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (NB - 1u)], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
+    if (threadIdx.x < NB) hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
 }
 
 // Stable scatter.  Wave w owns items [tile*TILE + w*64*ITEMS, +64*ITEMS) in (j, lane) order, so ranks
 // follow the input order: earlier waves, then earlier j, then lower lanes.
+template <int RB>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint32_t *__restrict__ kin,
                                                          const uint32_t *__restrict__ vin,
                                                          uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                          uint32_t n, uint32_t shift, uint32_t ntiles,
                                                          const uint32_t *__restrict__ hist) {
     constexpr int NW = RS_BLOCK / 64;
-    __shared__ uint32_t wh[NW][257];
+    constexpr uint32_t NB = 1u << RB;   // digits; NB itself marks an item beyond n
+    __shared__ uint32_t wh[NW][NB + 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (uint32_t d = lane; d < 257; d += 64) wh[w][d] = 0;
+    for (uint32_t d = lane; d < NB + 1u; d += 64) wh[w][d] = 0;
     __syncthreads();
     const uint32_t tile = blockIdx.x;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -282,11 +288,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint32_t *__restr
         bool valid = i < n;
         key[j] = valid ? kin[i] : 0u;
         val[j] = valid && vin ? vin[i] : 0u;
-        uint32_t d = valid ? ((key[j] >> shift) & 255u) : 256u;
+        uint32_t d = valid ? ((key[j] >> shift) & (NB - 1u)) : NB;
         dig[j] = d;
         uint64_t peers = ~0ull;
 #pragma unroll
-        for (int b = 0; b < 9; b++) {
+        for (int b = 0; b <= RB; b++) {
             uint64_t bb = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bb : ~bb;
         }
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint32_t *__restr
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    if (threadIdx.x < 256) {
+    if (threadIdx.x < NB) {
         uint32_t run = 0;
         for (int ww = 0; ww < NW; ww++) {
             uint32_t c = wh[ww][threadIdx.x];
@@ -309,7 +315,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint32_t *__restr
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
         uint32_t d = dig[j];
-        if (d < 256u) {
+        if (d < NB) {
             uint32_t pos = hist[d * ntiles + tile] + wh[w][d] + rank[j];
             kout[pos] = key[j];
             if (vout) vout[pos] = val[j];
@@ -866,12 +872,27 @@ static int ensure_status(SortedState *s, uint64_t tiles, std::string &err) {
 }
 
 // Stable LSD radix sort of n (key, value) pairs over bits [0, bits); result in buffers[result].
+template <int RB>
+static void radix_pass(SortedState *s, hipStream_t st, uint32_t **kb, uint32_t **vb, int cur, uint32_t n,
+                       uint32_t shift, uint32_t ntiles, uint64_t hn, uint32_t stiles) {
+    hipLaunchKernelGGL(k_rs_hist<RB>, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], n, shift, ntiles, s->hist);
+    hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->hist, (uint32_t)hn, s->ticket, s->status, stiles,
+                       s->epoch, (uint64_t *)nullptr);
+    s->epoch = (s->epoch + 1u) & ((1u << 26) - 1u);
+    if (!s->epoch) s->epoch = 1;
+    hipLaunchKernelGGL(k_rs_scatter<RB>, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], vb[cur], kb[cur ^ 1],
+                       vb[cur ^ 1], n, shift, ntiles, s->hist);
+}
+
+// Stable LSD sort of n (key, value) pairs by the keys' bits 0..bits-1, in the fewest passes of at most 8-bit digits,
+// of equal width (24 bits: three of 8; the tile pairs' 12: two of 6, a quarter of the histogram per pass).
 static int radix_sort(SortedState *s, hipStream_t st, uint32_t **kb, uint32_t **vb, uint32_t n, uint32_t bits,
                       int &result, std::string &err, ProfList *prof) {
     result = 0;
-    if (n <= 1) return 0;
+    if (n <= 1 || bits == 0) return 0;
+    const uint32_t npass = (bits + 7u) / 8u, rb = max((bits + npass - 1u) / npass, 4u);
     uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-    uint64_t hn = 256ull * ntiles;
+    uint64_t hn = (1ull << rb) * ntiles;
     if (hn > s->hist_cap) {
         sfree(s->hist);
         S_TRY(hipMalloc(&s->hist, hn * 4));
@@ -880,15 +901,14 @@ static int radix_sort(SortedState *s, hipStream_t st, uint32_t **kb, uint32_t **
     uint32_t stiles = (uint32_t)((hn + 2047) / 2048);
     if (ensure_status(s, stiles, err) != 0) return -1;
     int cur = 0;
-    for (uint32_t shift = 0; shift < bits; shift += 8) {
+    for (uint32_t d = 0; d < npass; d++) {
         SProf p(prof, st, "radix_pass");
-        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], n, shift, ntiles, s->hist);
-        hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->hist, (uint32_t)hn, s->ticket, s->status,
-                           stiles, s->epoch, (uint64_t *)nullptr);
-        s->epoch = (s->epoch + 1u) & ((1u << 26) - 1u);
-        if (!s->epoch) s->epoch = 1;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, st, kb[cur], vb[cur], kb[cur ^ 1],
-                           vb[cur ^ 1], n, shift, ntiles, s->hist);
+        const uint32_t shift = d * rb;
+        if (rb == 4) radix_pass<4>(s, st, kb, vb, cur, n, shift, ntiles, hn, stiles);
+        else if (rb == 5) radix_pass<5>(s, st, kb, vb, cur, n, shift, ntiles, hn, stiles);
+        else if (rb == 6) radix_pass<6>(s, st, kb, vb, cur, n, shift, ntiles, hn, stiles);
+        else if (rb == 7) radix_pass<7>(s, st, kb, vb, cur, n, shift, ntiles, hn, stiles);
+        else radix_pass<8>(s, st, kb, vb, cur, n, shift, ntiles, hn, stiles);
         cur ^= 1;
     }
     result = cur;
@@ -992,10 +1012,8 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
                            (uint32_t)npairs, s->owner);
         hipLaunchKernelGGL(k_sp_emit, dim3(eblocks), dim3(EMIT_THREADS), 0, st, s->sp, svals, s->visible, f.W, f.H,
                            s->cnt, s->owner, (uint32_t)npairs, s->pk[0], s->pv[0]);
-        uint32_t bits = 0;
-        while ((1u << bits) < s->tiles_cap) bits++;
-        bits = (bits + 7) / 8 * 8;
-        if (bits == 0) bits = 8;
+        uint32_t bits = 1;   // the tile keys' width: tile ids below tiles_cap
+        while (bits < 32u && (1u << bits) < s->tiles_cap) bits++;
         if (radix_sort(s, st, s->pk, s->pv, (uint32_t)npairs, bits, pres, err, prof) != 0) return -1;
         hipLaunchKernelGGL(k_tile_ranges, dim3(nblk(npairs, 256)), dim3(256), 0, st, s->pk[pres], (uint32_t)npairs,
                            s->tstart, s->tend);
