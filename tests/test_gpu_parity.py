@@ -448,6 +448,26 @@ def test_sorted_empty_frames_and_buffer_reuse():
     r.close()
 
 
+def test_sorted_large_frame_matches_oracle():
+    """A 2000 x 1100 frame of the sorted path: 125 x 69 tiles (the last row partial), tile ids of 14 bits sorted in
+    two 7-bit passes, 34,500 quadrant workgroups; keys, values and the image equal the oracle's."""
+    w, h = 2000, 1100
+    arr = to_gpu(crowd(600, seed=11))
+    cam = gh.default_camera(1100.0)
+    p = gh.RenderParams(spp_side=1, background=BG)
+    r = gh.GpsRenderer(w, h, 1)
+    r.set_gaussians(arr)
+    r.render_sorted(cam, p, 1)
+    acc = r.read_accum()
+    keys, vals = r.read_sorted()
+    r.close()
+    m = O.mirror_sorted(arr, None, 0, ocam(cam), mparams(p), w, h)
+    assert m["visible"] == 600
+    assert np.array_equal(keys, m["keys"]) and np.array_equal(vals, m["vals"])
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+    assert (acc[..., :3] != np.asarray(BG, np.float32)).any(axis=-1).mean() > 0.2
+
+
 def test_radix_sort_300k_multilevel():
     """verify.rs:827-853: 300k small splats."""
     say("[sorted] 300k")
