@@ -4,13 +4,15 @@
 // CDNA has no rasterizer, so the reference's instanced-quad draw with hardware SrcAlpha /
 // OneMinusSrcAlpha blending becomes compute:
 //   k_sp_project   per splat: cull, GPS demand probe, quad axes, SH colour, 24-bit far->near key
-//   radix sort     stable LSD, 8-bit digits: per-tile digit histograms (digit-major), a decoupled
-//                  look-back scan over them, then a stable scatter ranked with wave64 ballots
+//   radix sort     stable LSD, digits of equal width of at most 8 bits: per-tile digit histograms
+//                  (digit-major), a decoupled look-back scan over them, then a stable scatter ranked with
+//                  wave64 ballots
 //   binning        each visible splat (in sorted order) emits one (tile, splat) pair per 16x16 tile its
-//                  quad touches; a stable sort by tile keeps the back-to-front order inside each tile
-//   k_sp_blend     one workgroup per 8x8 quadrant of a tile: four waves evaluate the splats' alphas, one runs
-//                  the back-to-front blend of the tile's splats at pixel centres in f32, written to the
-//                  accumulation buffer with count 1 (sp_resolve)
+//                  quad touches, balanced by pairs; a stable sort by tile keeps the back-to-front order
+//                  inside each tile; the tiles are then ordered longest list first
+//   k_sp_blend     one workgroup per 8x8 quadrant of a tile: the splats that may reach the quadrant (binning
+//                  box, ellipse bound) are kept, four waves evaluate their alphas, one runs the back-to-front
+//                  blend at pixel centres in f32, written to the accumulation buffer with count 1 (sp_resolve)
 #include <string>
 #include <vector>
 
