@@ -30,6 +30,11 @@ constexpr int RS_BLOCK = 256;
 constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
 constexpr int BIN = 16;                      // screen tile side in pixels
+#ifndef GPS_SP_SCAN_ITEMS
+#define GPS_SP_SCAN_ITEMS 8
+#endif
+constexpr int SP_SCAN_ITEMS = GPS_SP_SCAN_ITEMS;   // items per thread of k_rs_scan (256 threads per tile)
+constexpr uint32_t SP_SCAN_TILE = 256u * SP_SCAN_ITEMS;
 constexpr uint32_t SP_KEY_INVALID = 0xFFFFFFu;
 constexpr float SP_MIN_ALPHA = 0.00392156862745098f;
 constexpr float PI_F = 3.14159265358979f;
@@ -337,7 +342,7 @@ GPS_DEV uint64_t sat_add(uint64_t a, uint64_t b) {
 __global__ __launch_bounds__(256) void k_rs_scan(uint32_t *__restrict__ data, uint32_t n, uint32_t *ticket,
                                                  uint64_t *status, uint32_t tiles, uint32_t epoch,
                                                  uint64_t *__restrict__ total) {
-    constexpr int ITEMS = 8, TILE = 256 * ITEMS;
+    constexpr int ITEMS = SP_SCAN_ITEMS, TILE = 256 * ITEMS;
     __shared__ uint32_t s_tile;
     __shared__ uint64_t s_wave[4];
     __shared__ uint64_t s_prefix;
@@ -900,7 +905,7 @@ static int radix_sort(SortedState *s, hipStream_t st, uint32_t **kb, uint32_t **
         S_TRY(hipMalloc(&s->hist, hn * 4));
         s->hist_cap = hn;
     }
-    uint32_t stiles = (uint32_t)((hn + 2047) / 2048);
+    uint32_t stiles = (uint32_t)((hn + SP_SCAN_TILE - 1) / SP_SCAN_TILE);
     if (ensure_status(s, stiles, err) != 0) return -1;
     int cur = 0;
     for (uint32_t d = 0; d < npass; d++) {
@@ -975,7 +980,7 @@ int sorted_render(SortedState *s, hipStream_t st, const GpsFrame &f, const float
     // binning: tiles per visible splat (sorted order) -> exclusive offsets -> pairs -> stable sort by tile
     hipLaunchKernelGGL(k_sp_tilecount, dim3(nblk(n, 256)), dim3(256), 0, st, s->sp, svals, s->visible, n, f.W, f.H,
                        s->cnt);
-    uint32_t stiles = (n + 2047) / 2048;
+    uint32_t stiles = (n + SP_SCAN_TILE - 1) / SP_SCAN_TILE;
     if (ensure_status(s, stiles, err) != 0) return -1;
     hipLaunchKernelGGL(k_rs_scan, dim3(stiles), dim3(256), 0, st, s->cnt, n, s->ticket, s->status, stiles, s->epoch,
                        s->total);
