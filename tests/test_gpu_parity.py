@@ -420,6 +420,28 @@ def test_sorted_clustered_sh3_views_match_oracle(view):
     assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
 
 
+@pytest.mark.parametrize("seed", [21, 22, 23, 24])
+def test_sorted_random_views_match_oracle(seed):
+    """Random orbit views (distance 1 to 12, any yaw, pitch within +-60 degrees) of a 6,000-splat clustered SH3 scene
+    at 256 x 144, some from inside the clusters: keys, values and the image of the sorted path equal the oracle's."""
+    rng = np.random.default_rng(seed)
+    w, h = 256, 144
+    g, sh = S.clustered(6_000, seed=seed, sh_degree=3)
+    cam = gh.orbit_camera((0.0, 0.0, 0.0), float(rng.uniform(1.0, 12.0)), float(rng.uniform(0.0, 2.0 * math.pi)),
+                          float(rng.uniform(-1.05, 1.05)), (0.0, 0.0, 1.0), S.FOV_60, w, h)
+    p = gh.RenderParams(spp_side=1, background=BG)
+    r = gh.GpsRenderer(w, h, 1)
+    r.set_gaussians_sh(g, sh, 3)
+    r.render_sorted(cam, p, 1)
+    acc = r.read_accum()
+    keys, vals = r.read_sorted()
+    r.close()
+    m = O.mirror_sorted(g, sh, 3, ocam(cam), mparams(p), w, h)
+    say(f"[sorted random view] seed {seed}: {m['visible']} visible")
+    assert np.array_equal(keys, m["keys"]) and np.array_equal(vals, m["vals"])
+    assert np.array_equal(acc.view(np.uint32), m["accum"].view(np.uint32))
+
+
 def test_sorted_empty_frames_and_buffer_reuse():
     """One renderer through frames of the sorted path on scenes of different sizes, some seen from behind (no splat
     visible, no tile pairs): every frame's keys, values and accumulation buffer equal the oracle's, whatever the
