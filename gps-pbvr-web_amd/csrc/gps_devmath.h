@@ -59,6 +59,23 @@ GPS_DEV float div_normal(float a, float b) {
     return fma(e, r, q);
 }
 
+// div_normal's first steps, which depend on b alone: 1 / b refined once (for a divisor shared by many divisions)
+GPS_DEV float rcp_refined(float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    return fma(fma(-b, r, 1.0f), r, r);
+}
+
+// div_normal's remaining steps with r = rcp_refined(b): the same operations, so the same bits as div_normal(a, b);
+// for b in [2^-60, 2^60] and |a| <= 2^57 it is the correctly rounded a / b wherever |a| >= 2^-100 (checked by the
+// device math tests), and within an ulp below
+GPS_DEV float div_by(float a, float b, float r) {
+    float q = a * r;
+    float e = fma(-b, q, a);
+    q = fma(e, r, q);
+    e = fma(-b, q, a);
+    return fma(e, r, q);
+}
+
 // sqrt(x) for a positive normal x in [2^-96, 2^126]: v_sqrt_f32 is within one ulp, and the neighbour whose
 // residual x - s' s changes sign is the correctly rounded root.  These are the correctly rounded sqrt's steps
 // without its subnormal prescale and its zero/infinity select, which do not apply to such x: the same bits.

@@ -602,18 +602,27 @@ GPS_DEV bool ellipse_touch(float4 a, float4 b, float qx0, float qy0) {
 }
 
 // staging wave: keep the lanes whose splat touches the quadrant, packed in lane (list) order, with the splat's
-// per-pixel constants (the two axis denominators and k^2, the same operations as in the per-pixel expressions);
-// returns the count
+// per-pixel constants: k^2 in place of k (s_a.w) and the two axis denominators (the same operations as in the
+// per-pixel expressions) with their refined reciprocals, or 0 where the exact division is kept (s_d); returns the
+// count
 GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0, float qy0, float4 *s_a,
                              float4 *s_b, float4 *s_c, float4 *s_d, uint32_t lane) {
     const bool keep = valid && box_touches(a, b, qx0, qy0) && ellipse_touch(a, b, qx0, qy0);
     const uint64_t mask = __ballot(keep);
     if (keep) {
         const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        s_a[r] = a;
+        const float den1 = b.x * b.x + b.y * b.y, den2 = b.z * b.z + b.w * b.w, kk = a.w * a.w;
+        // the per-pixel divisions by den1, den2 use a reciprocal refined here where div_by gives the correctly
+        // rounded quotient: denominators in [2^-60, 2^60], numerators below 2^57 (|centre| < 2^24) and not under
+        // 2^-100 unless 0.  A numerator is dx b.x + dy b.y with pixel offsets 0 or above 2^-25 and the projection's
+        // axis components 0 or above 1e-20 |b| >= 1e-20 2^-30, so a non-zero one stays above 2^-100; a zero one
+        // gives a zero u^2 on both paths.  k^2 >= 2^-60 keeps splats at the k = 0 edge on the exact path.
+        const bool fast = den1 >= 0x1p-60f && den1 <= 0x1p60f && den2 >= 0x1p-60f && den2 <= 0x1p60f &&
+                          fabsf(a.x) < 0x1p24f && fabsf(a.y) < 0x1p24f && kk >= 0x1p-60f;
+        s_a[r] = make_float4(a.x, a.y, a.z, kk);
         s_b[r] = b;
         s_c[r] = c;
-        s_d[r] = make_float4(b.x * b.x + b.y * b.y, b.z * b.z + b.w * b.w, a.w * a.w, 0.0f);
+        s_d[r] = make_float4(den1, den2, fast ? gpsm::rcp_refined(den1) : 0.0f, fast ? gpsm::rcp_refined(den2) : 0.0f);
     }
     return (uint32_t)__popcll(mask);
 }
@@ -622,8 +631,15 @@ GPS_DEV uint32_t stage_batch(bool valid, float4 a, float4 b, float4 c, float qx0
 // denominators)
 GPS_DEV float splat_r2(float4 a, float4 b, float4 d, float qx, float qy) {
     const float dx = qx - a.x, dy = qy - a.y;
-    const float u = (dx * b.x + dy * b.y) / d.x;
-    const float v = (dx * b.z + dy * b.w) / d.y;
+    const float n1 = dx * b.x + dy * b.y, n2 = dx * b.z + dy * b.w;
+    float u, v;
+    if (d.z != 0.0f) {   // the staged reciprocals (the same for every lane)
+        u = gpsm::div_by(n1, d.x, d.z);
+        v = gpsm::div_by(n2, d.y, d.w);
+    } else {
+        u = n1 / d.x;
+        v = n2 / d.y;
+    }
     return u * u + v * v;
 }
 
@@ -697,11 +713,11 @@ __global__ __launch_bounds__(BLEND_THREADS) void k_sp_blend(GpsFrame f, const fl
             const uint32_t q2 = q + BLEND_WAVES < cnt ? q + BLEND_WAVES : q;
             const float4 a0 = s_a[cur][q], d0 = s_d[cur][q], a1 = s_a[cur][q2], d1 = s_d[cur][q2];
             const float r20 = splat_r2(a0, s_b[cur][q], d0, qx, qy), r21 = splat_r2(a1, s_b[cur][q2], d1, qx, qy);
-            const float al0 = splat_alpha(r20, a0.z, d0.z, inside);
+            const float al0 = splat_alpha(r20, a0.z, a0.w, inside);
             s_alpha[q][lane] = al0;
             if (__ballot(al0 > 0.0f)) any |= 1ull << q;
             if (q2 != q) {
-                const float al1 = splat_alpha(r21, a1.z, d1.z, inside);
+                const float al1 = splat_alpha(r21, a1.z, a1.w, inside);
                 s_alpha[q2][lane] = al1;
                 if (__ballot(al1 > 0.0f)) any |= 1ull << q2;
             }

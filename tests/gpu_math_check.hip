@@ -86,6 +86,44 @@ __global__ __launch_bounds__(BLOCK) void k_exp_mid(unsigned long long *bad) {
     }
 }
 
+// div_by(a, b, rcp_refined(b)) against the correctly rounded a / b over pseudo-random pairs (a PCG hash of the
+// index): b with any mantissa and sign and an exponent in [-60, 60] (extremes: only -60, -59, 59, 60), a with an
+// exponent in [-149, 57] (extremes: -100, -99, 56, 57), a quarter of them below 2^-100, zeros and subnormals
+// included.  Bad: different bits where |a| >= 2^-100; where |a| < 2^-100, more than an ulp apart or |q| >= 2^-39.
+__device__ uint32_t mix32(uint32_t v) {
+    uint32_t s = v * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_div_by(uint32_t pass, int extremes, unsigned long long *bad) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x + pass * (uint32_t)(GRID * BLOCK);
+    const uint32_t h1 = mix32(i * 2u + 1u), h2 = mix32(h1 ^ 0x9e3779b9u), h3 = mix32(h2 + 7u);
+    int eb, ea;
+    if (extremes) {
+        const int eb4[4] = {-60, -59, 59, 60}, ea4[4] = {-100, -99, 56, 57};
+        eb = eb4[h3 & 3u];
+        ea = ea4[(h3 >> 2) & 3u];
+    } else {
+        eb = -60 + (int)(h3 % 121u);
+        ea = ((h3 >> 8) & 3u) == 0u ? -149 + (int)((h3 >> 10) % 49u) : -100 + (int)((h3 >> 10) % 158u);
+    }
+    const float b = __uint_as_float((h1 & 0x807FFFFFu) | ((uint32_t)(eb + 127) << 23));
+    float a;
+    if (ea < -126) a = __uint_as_float((h2 & 0x807FFFFFu) >> (uint32_t)(-126 - ea));   // subnormal or zero
+    else a = __uint_as_float((h2 & 0x807FFFFFu) | ((uint32_t)(ea + 127) << 23));
+    if ((h3 >> 26) == 63u) a = 0.0f;
+    const float q = gpsm::div_by(a, b, gpsm::rcp_refined(b)), e = a / b;
+    bool b_ad;
+    if (fabsf(a) >= 0x1p-100f) {
+        b_ad = __float_as_uint(q) != __float_as_uint(e);
+    } else {
+        const uint32_t uq = __float_as_uint(q) & 0x7FFFFFFFu, ue = __float_as_uint(e) & 0x7FFFFFFFu;
+        b_ad = (uq > ue ? uq - ue : ue - uq) > 1u || !(fabsf(q) < 0x1p-39f);
+    }
+    count_bad(b_ad, bad);
+}
+
 // sincos_fin's quadrant selects against a switch over the quadrant, for every x with bits in [0, hi]
 __global__ __launch_bounds__(BLOCK) void k_sincos(uint32_t hi, unsigned long long *bad) {
     const uint64_t n = (uint64_t)hi + 1u;
@@ -234,6 +272,14 @@ extern "C" int mathcheck_bf16(unsigned long long *bad) {
 
 extern "C" int mathcheck_exp_mid(unsigned long long *bad) {
     return run(bad, [](unsigned long long *d) { hipLaunchKernelGGL(k_exp_mid, dim3(GRID), dim3(BLOCK), 0, 0, d); });
+}
+
+extern "C" int mathcheck_div_by(int extremes, unsigned long long *bad) {
+    // 2^32 pairs: 2048 passes of the GRID x BLOCK threads
+    return run(bad, [=](unsigned long long *d) {
+        for (uint32_t pass = 0; pass < 2048u; pass++)
+            hipLaunchKernelGGL(k_div_by, dim3(GRID), dim3(BLOCK), 0, 0, pass, extremes, d);
+    });
 }
 
 extern "C" int mathcheck_sincos(uint32_t hi_bits, unsigned long long *bad) {

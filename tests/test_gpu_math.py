@@ -1,4 +1,4 @@
-"""Exhaustive checks of the specialised device math (gps_devmath.h: div_normal, sqrt_normal, exp_mid and the
+"""Exhaustive checks of the specialised device math (gps_devmath.h: div_normal, div_by, sqrt_normal, exp_mid and the
 branch-free quadrant selects of sincos_fin) against the general correctly rounded operations, bit for bit,
 over every argument the particle kernels give them; and of the denoiser's hardware bf16 rounding.  The kernels are in tests/gpu_math_check.hip."""
 import ctypes
@@ -22,7 +22,7 @@ def lib():
         raise RuntimeError(f"{LIB} is missing: run __graft_entry__.build() first")
     lib = ctypes.CDLL(LIB)
     for name in ("mathcheck_sqrt", "mathcheck_div_log", "mathcheck_div_eol", "mathcheck_sincos", "mathcheck_bf16",
-                 "mathcheck_exp_mid"):
+                 "mathcheck_exp_mid", "mathcheck_div_by"):
         getattr(lib, name).restype = ctypes.c_int
     return lib
 
@@ -59,6 +59,13 @@ def test_exp_mid_equals_exp(lib):
     # the alpha blend's exp(-r2 / 2), r2 <= k^2 <= 2 ln 255: exp_mid (ldexp scaling, no range checks) against exp()
     # for every float in [-80, 0]
     assert _bad(lib.mathcheck_exp_mid) == 0
+
+
+@pytest.mark.parametrize("extremes", [0, 1])
+def test_div_by_shared_reciprocal(lib, extremes):
+    # the alpha blend's u = N / |b1|^2 and v = N / |b2|^2 with the reciprocal refined once per splat: equal to the
+    # correctly rounded quotient over 2^32 pairs in the guarded ranges (wherever |N| >= 2^-100)
+    assert _bad(lib.mathcheck_div_by, ctypes.c_int(extremes)) == 0
 
 
 def test_bf16_pack_rounds_to_nearest_even(lib):
